@@ -1,0 +1,185 @@
+"""ctypes binding to the C oracle (oracle/libadco.so).
+
+Test infrastructure only: tests, the smoke check and the benchmark's CPU
+baseline leg use the oracle as the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(REPO, "oracle", "libadco.so")
+
+SEQUENTIAL = 0
+BLOCKED = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Setup(C.Structure):
+    _fields_ = [
+        ("np", C.c_int), ("ne", C.c_int), ("mnei", C.c_int), ("neta", C.c_int),
+        ("nvel", C.c_int), ("nvelme", C.c_int), ("nbfr", C.c_int),
+        ("nm", _ip), ("x", _dp), ("y", _dp), ("dp", _dp), ("fdxe", _dp), ("fdye", _dp),
+        ("areas", _dp), ("nneigh", _ip), ("neitab", _ip), ("csr", _ip), ("nbd", _ip),
+        ("nbv", _ip), ("lbcodei", _ip), ("csii", _dp), ("siii", _dp), ("me2gw", _ip),
+        ("emo", _dp), ("efa", _dp), ("amig", _dp), ("ff", _dp), ("face", _dp),
+        ("tau0var", _dp), ("evm", _dp), ("fric", _dp), ("corif", _dp),
+        ("dtdp", C.c_double), ("statim", C.c_double), ("reftim", C.c_double),
+        ("g", C.c_double), ("h0", C.c_double), ("velmin", C.c_double),
+        ("a00", C.c_double), ("b00", C.c_double), ("c00", C.c_double),
+        ("convcr", C.c_double), ("drampelev", C.c_double), ("prbckgrnd_mh2o", C.c_double),
+        ("nramp", C.c_int), ("nolifa", C.c_int), ("nolica", C.c_int), ("nolicat", C.c_int),
+        ("nolibf", C.c_int), ("itmax", C.c_int), ("nws", C.c_int),
+        ("nodecode0", _ip), ("noff0", _ip), ("eta2_0", _dp),
+    ]
+
+
+class Itpack(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("in_", "is_", "isym", "itmax", "level", "adapt",
+                                       "betadt", "caseii", "halt", "partad")] + \
+               [(n, C.c_double) for n in ("bdelnm", "betab", "cme", "delnnm", "delsnm", "ff",
+                                          "gamma", "omega", "qa", "qt", "rho", "rrr", "sige",
+                                          "sme", "specr", "spr", "srelpr", "stptst", "udnm",
+                                          "zeta")] + \
+               [("q1", C.c_int), ("gamold", C.c_double), ("rhoold", C.c_double),
+                ("cmold", C.c_double), ("order", C.c_int)]
+
+
+def load() -> C.CDLL:
+    lib = C.CDLL(LIB_PATH)
+    lib.adco_create.restype = C.c_void_p
+    lib.adco_create.argtypes = [C.POINTER(Setup), C.c_int]
+    lib.adco_destroy.argtypes = [C.c_void_p]
+    lib.adco_step.restype = C.c_int
+    lib.adco_step.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+    lib.adco_array.restype = _dp
+    lib.adco_array.argtypes = [C.c_void_p, C.c_char_p]
+    lib.adco_iarray.restype = _ip
+    lib.adco_iarray.argtypes = [C.c_void_p, C.c_char_p]
+    lib.adco_numitr.argtypes = [C.c_void_p]
+    lib.adco_ncchange.argtypes = [C.c_void_p]
+    lib.adco_state_hash.restype = C.c_uint64
+    lib.adco_state_hash.argtypes = [C.c_void_p]
+    lib.adco_holland.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
+    lib.adco_itpack_init.argtypes = [C.POINTER(Itpack), C.c_int]
+    lib.adco_dfault.argtypes = [_ip, _dp]
+    lib.adco_jcg.restype = C.c_int
+    lib.adco_jcg.argtypes = [C.POINTER(Itpack), C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, _dp,
+                             C.c_int, _dp, _ip, _dp]
+    lib.adco_dot.restype = C.c_double
+    lib.adco_dot.argtypes = [C.c_int, C.c_int, _dp, _dp]
+    lib.adco_eta_bc.argtypes = [C.POINTER(Setup), C.c_double, C.c_double, _dp]
+    return lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class OracleModel:
+    """One oracle model instance built from a setup dictionary (the same
+    arrays the reference harness writes)."""
+
+    def __init__(self, setup: Dict[str, np.ndarray], run, order: int = SEQUENTIAL,
+                 lib: Optional[C.CDLL] = None):
+        self.lib = lib or load()
+        self.keep = {}
+        s = Setup()
+
+        def ia(name, a):
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            self.keep[name] = a
+            return _p(a, _ip)
+
+        def da(name, a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            self.keep[name] = a
+            return _p(a, _dp)
+
+        d = setup
+        self.np_, self.ne = int(d["np"]), int(d["ne"])
+        s.np, s.ne, s.mnei = self.np_, self.ne, int(d["mnei"])
+        s.neta, s.nvel, s.nvelme = int(d["neta"]), int(d["nvel"]), int(d["nvelme"])
+        s.nbfr = run.nbfr
+        s.nm = ia("nm", d["nm"]); s.x = da("x", d["x"]); s.y = da("y", d["y"])
+        s.dp = da("dp", d["dp"]); s.fdxe = da("fdxe", d["fdxe"]); s.fdye = da("fdye", d["fdye"])
+        s.areas = da("areas", d["areas"]); s.nneigh = ia("nneigh", d["nneigh"])
+        s.neitab = ia("neitab", d["neitab"]); s.csr = ia("csr", d["csr"])
+        s.nbd = ia("nbd", d["nbd"]); s.nbv = ia("nbv", d["nbv"])
+        s.lbcodei = ia("lbcodei", d["lbcodei"]); s.csii = da("csii", d["csii"])
+        s.siii = da("siii", d["siii"]); s.me2gw = ia("me2gw", d["me2gw"])
+        s.emo = da("emo", d["emo"]); s.efa = da("efa", d["efa"])
+        s.amig = da("amig", np.array(run.amig, np.float64))
+        s.ff = da("ff", np.array(run.ff, np.float64))
+        s.face = da("face", np.radians(np.array(run.face, np.float64)) if "face_rad" not in d
+                    else d["face_rad"])
+        s.tau0var = da("tau0var", d["tau0var"]); s.evm = da("evm", d["evm"])
+        s.fric = da("fric", d["fric"])
+        s.corif = da("corif", np.full(self.np_, run.cori))
+        s.dtdp, s.statim, s.reftim, s.g = run.dtdp, run.statim, run.reftim, run.g
+        s.h0, s.velmin, s.a00, s.b00, s.c00 = run.h0, run.velmin, run.a00, run.b00, run.c00
+        s.convcr, s.drampelev = run.convcr, run.drampelev
+        s.prbckgrnd_mh2o = float(d["prbckgrnd_mh2o"])
+        s.nramp, s.nolifa, s.nolica, s.nolicat = run.nramp, run.nolifa, run.nolica, run.nolicat
+        s.nolibf, s.itmax, s.nws = run.nolibf, run.itmax, run.nws
+        s.nodecode0 = ia("nodecode0", d["nodecode0"]); s.noff0 = ia("noff0", d["noff0"])
+        s.eta2_0 = da("eta2_0", d["eta2_0"])
+        self.setup = s
+        self.run = run
+        self.h = self.lib.adco_create(C.byref(s), order)
+        self.hol = None
+        if run.holland is not None:
+            self.hol = np.array(list(run.holland) + [0.0] * (16 - len(run.holland)))
+            self.wsx = np.zeros(self.np_); self.wsy = np.zeros(self.np_)
+            self.pr = np.zeros(self.np_)
+            # the cold-start forcing (time 0) becomes WSX1/PR1 of step 1
+            self.forcing(0.0)
+            for name, arr in (("wsx2", self.wsx), ("wsy2", self.wsy), ("pr2", self.pr)):
+                ptr = self.lib.adco_array(self.h, name.encode())
+                np.ctypeslib.as_array(ptr, (self.np_,))[:] = arr
+
+    def forcing(self, t: float):
+        self.lib.adco_holland(self.np_, _p(self.keep["x"], _dp), _p(self.keep["y"], _dp), t,
+                              _p(self.hol, _dp), _p(self.wsx, _dp), _p(self.wsy, _dp),
+                              _p(self.pr, _dp))
+        return self.wsx, self.wsy, self.pr
+
+    def step(self, it: int) -> int:
+        if self.hol is not None:
+            t = it * self.run.dtdp + self.run.statim * 86400.0
+            wsx, wsy, pr = self.forcing(t)
+            return self.lib.adco_step(self.h, it, _p(wsx, _dp), _p(wsy, _dp), _p(pr, _dp))
+        return self.lib.adco_step(self.h, it, None, None, None)
+
+    def array(self, name: str, n: Optional[int] = None) -> np.ndarray:
+        p = self.lib.adco_array(self.h, name.encode())
+        if n is None:
+            n = self.ne if name == "alphal" else self.np_
+        return np.ctypeslib.as_array(p, (n,)).copy()
+
+    def iarray(self, name: str) -> np.ndarray:
+        p = self.lib.adco_iarray(self.h, name.encode())
+        n = self.ne if name == "noff" else self.np_
+        return np.ctypeslib.as_array(p, (n,)).copy()
+
+    def numitr(self) -> int:
+        return self.lib.adco_numitr(self.h)
+
+    def ncchange(self) -> int:
+        return self.lib.adco_ncchange(self.h)
+
+    def state_hash(self) -> int:
+        v = self.lib.adco_state_hash(self.h)
+        return int(np.array([v], dtype=np.uint64).view(np.int64)[0])
+
+    def __del__(self):
+        try:
+            self.lib.adco_destroy(self.h)
+        except Exception:
+            pass
