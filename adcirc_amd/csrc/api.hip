@@ -1,0 +1,828 @@
+// Host side of the C ABI declared in include/adcgpu.h: context set-up
+// (host preprocessing of the reference's mesh tables into the device layout),
+// state transfer, per-step launch sequence and the drop-in JCG.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/adcgpu.h"
+#include "kernels.h"
+
+using namespace adcgpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(ADCGPU_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dmalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(ADCGPU_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+template <class T>
+int upload(T** p, const std::vector<T>& v) {
+  int rc = dmalloc(p, v.size());
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+struct adcgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int np = 0, ne = 0, mnei = 0, neta = 0, nch = 0;
+  adcgpu_params prm{};
+  double ifnlfa = 1, ifnlct = 1, ifnlcat = 1, iflinbf = 0, ifnlbf = 1, tkmax = 1;
+  double habsmin = 0, hoff = 0;
+  // mesh
+  int *nm = nullptr, *inc_ptr = nullptr, *inc = nullptr, *nneigh = nullptr, *col = nullptr;
+  int *csrpos = nullptr, *bnd_index = nullptr, *nbd = nullptr, *obc_slot = nullptr;
+  int *obcg_ptr = nullptr, *obcg = nullptr, *vbc_ptr = nullptr, *vbc = nullptr, *lbcodei = nullptr;
+  double *fdx = nullptr, *fdy = nullptr, *areas = nullptr, *x = nullptr, *y = nullptr, *dp = nullptr;
+  double *tau0 = nullptr, *evm = nullptr, *corif = nullptr, *csii = nullptr, *siii = nullptr;
+  // state (pointers rotate between time levels)
+  double *eta1, *eta2, *etas, *uu1, *vv1, *uu2, *vv2, *qx1, *qy1, *qx2, *qy2, *h1, *h2, *tk;
+  double *etamax, *totalarea, *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2, *gwce_lv, *alphal;
+  double *coef, *obc, *fric;
+  int *nodecode, *nnodecode, *noff, *noffold, *mju;
+  // work
+  double *contrib = nullptr, *cx = nullptr, *cy = nullptr, *ep_part = nullptr, *scal = nullptr;
+  double *eta_bc = nullptr, *eta_bc_table = nullptr;
+  int eta_bc_table_steps = 0;
+  WdCand* cand = nullptr;
+  int cand_cap = 0;
+  int* flags = nullptr;
+  // JCG work
+  double *cs = nullptr, *ju1 = nullptr, *jd0 = nullptr, *jd1 = nullptr, *jw = nullptr;
+  double *jpart = nullptr, *jsaved = nullptr;
+  unsigned* jbar = nullptr;
+  int jcg_grid = 0;
+  int64_t run_iters = 0;
+  // optional timing of the JCG launches with HIP events on the context stream
+  int profile = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> jcg_events;
+  std::vector<void*> allocs;
+
+  DevMesh mesh() const {
+    DevMesh m;
+    m.np = np; m.ne = ne; m.mnei = mnei; m.neta = neta;
+    m.dt = prm.dtdp; m.g = prm.g; m.a00 = prm.a00; m.b00 = prm.b00; m.c00 = prm.c00;
+    m.nm = nm; m.fdx = fdx; m.fdy = fdy; m.areas = areas; m.x = x; m.y = y; m.dp = dp;
+    m.tau0 = tau0; m.evm = evm; m.corif = corif; m.inc_ptr = inc_ptr; m.inc = inc;
+    m.nneigh = nneigh; m.col = col; m.csrpos = csrpos; m.bnd_index = bnd_index; m.nbd = nbd;
+    m.obc_slot = obc_slot; m.obcg_ptr = obcg_ptr; m.obcg = obcg; m.vbc_ptr = vbc_ptr;
+    m.vbc = vbc; m.lbcodei = lbcodei; m.csii = csii; m.siii = siii;
+    return m;
+  }
+  DevState state() const {
+    DevState s;
+    s.eta1 = eta1; s.eta2 = eta2; s.etas = etas; s.uu1 = uu1; s.vv1 = vv1; s.uu2 = uu2;
+    s.vv2 = vv2; s.qx1 = qx1; s.qy1 = qy1; s.qx2 = qx2; s.qy2 = qy2; s.h1 = h1; s.h2 = h2;
+    s.tk = tk; s.etamax = etamax; s.totalarea = totalarea; s.wsx1 = wsx1; s.wsy1 = wsy1;
+    s.wsx2 = wsx2; s.wsy2 = wsy2; s.pr1 = pr1; s.pr2 = pr2; s.gwce_lv = gwce_lv;
+    s.alphal = alphal; s.coef = coef; s.obc = obc; s.fric = fric; s.nodecode = nodecode;
+    s.nnodecode = nnodecode; s.noff = noff; s.noffold = noffold; s.mju = mju;
+    return s;
+  }
+};
+
+namespace {
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ADCGPU_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+int setup_jcg_grid(adcgpu_ctx* c) {
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jcg, JCG_BLOCK, 0));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, c->device));
+  int cap = per_cu * prop.multiProcessorCount;
+  if (cap <= 0) return fail(ADCGPU_E_HIP, "JCG kernel cannot be made resident");
+  int need = (c->nch + JCG_MAXK - 1) / JCG_MAXK;
+  int grid = std::min(c->nch, cap);
+  if (grid < need)
+    return fail(ADCGPU_E_CAPACITY, "mesh too large for the register-resident JCG (" +
+                                       std::to_string(c->nch) + " chunks, " +
+                                       std::to_string(cap) + " resident blocks)");
+  c->jcg_grid = std::max(grid, 1);
+  return 0;
+}
+
+int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
+               double* u, int n, int itmax, double zeta, double* eta2, const double* eta1,
+               const int* nodecode) {
+  JcgArgs a;
+  a.n = n;
+  a.itmax = itmax;
+  a.maxfn = std::max(itmax, 50);
+  // zeta is raised to 512*srelpr if smaller (src/itpackv.F:223-232)
+  const double srelpr = std::ldexp(1.0, -52);
+  a.zeta = zeta >= 512.0 * srelpr ? zeta : 512.0 * srelpr;
+  // eigvss: nsig = max(int(-log10|zeta|), 4); zbrent: t = 10**(-nsig)
+  int itmp = (int)(-std::log10(std::fabs(a.zeta)));
+  int nsig = std::max(itmp, 4);
+  double p10 = 1.0;
+  for (int i = 0; i < nsig; ++i) p10 *= 10.0;
+  a.tz = 1.0 / p10;
+  a.coef = coef;
+  a.cs = c->cs;
+  a.col = col;
+  a.nneigh = nneigh;
+  a.rhs = rhs;
+  a.u0 = u;
+  a.u1 = c->ju1;
+  a.d0 = c->jd0;
+  a.d1 = c->jd1;
+  a.w = c->jw;
+  a.part = c->jpart;
+  a.saved = c->jsaved;
+  a.barrier = c->jbar;
+  a.flags = c->flags;
+  a.etas = u;
+  a.eta2 = eta2;
+  a.eta1 = eta1;
+  a.nodecode = nodecode;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->jbar, 0u, 1);
+  void* args[] = {&a};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->profile) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->stream));
+  }
+  HIPCHK(hipLaunchCooperativeKernel((const void*)k_jcg, dim3(c->jcg_grid), dim3(JCG_BLOCK), args,
+                                    0, c->stream));
+  if (c->profile) {
+    HIPCHK(hipEventRecord(e1, c->stream));
+    c->jcg_events.emplace_back(e0, e1);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* adcgpu_last_error(void) { return g_err.c_str(); }
+
+int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcgpu_ctx** out) {
+  if (!M || !P || !out) return fail(ADCGPU_E_ARG, "null argument");
+  *out = nullptr;
+  const int np = M->np, ne = M->ne, mnei = M->mnei, neta = M->neta;
+  if (np <= 0 || ne <= 0 || mnei < 2 || neta < 0 || M->nvel < 0 || M->nvelme < 0)
+    return fail(ADCGPU_E_ARG, "invalid sizes");
+  if (P->nolibf == 2) return fail(ADCGPU_E_UNSUPPORTED, "hybrid bottom friction (NOLIBF=2)");
+  if (P->nolibf < 0 || P->nolibf > 3) return fail(ADCGPU_E_ARG, "NOLIBF");
+  if (P->itmax <= 0 || P->itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
+  if (P->precision != ADCGPU_FP64) return fail(ADCGPU_E_UNSUPPORTED, "only the FP64 path");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+    return fail(ADCGPU_E_HIP, "no HIP device " + std::to_string(device));
+  HIPCHK(hipSetDevice(device));
+
+  // ---- host preprocessing of the reference's tables
+  std::vector<int> nm0(3 * (size_t)ne);
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k) {
+      int v = M->nm[e + (size_t)ne * k] - 1;
+      if (v < 0 || v >= np) return fail(ADCGPU_E_ARG, "NM out of range");
+      nm0[3 * (size_t)e + k] = v;
+    }
+  std::vector<int> cnt(np + 1, 0);
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k) cnt[nm0[3 * (size_t)e + k] + 1]++;
+  for (int i = 0; i < np; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int> inc(3 * (size_t)ne), fillp(cnt.begin(), cnt.end() - 1);
+  for (int e = 0; e < ne; ++e)  // ascending element order per node
+    for (int k = 0; k < 3; ++k) inc[fillp[nm0[3 * (size_t)e + k]]++] = 3 * e + k;
+  std::vector<int> nneigh(M->nneigh, M->nneigh + np);
+  std::vector<int> col((size_t)mnei * np, -1);
+  for (int i = 0; i < np; ++i) {
+    if (nneigh[i] < 1 || nneigh[i] > mnei) return fail(ADCGPU_E_ARG, "NNEIGH out of range");
+    if (M->neitab[i] != i + 1) return fail(ADCGPU_E_ARG, "NEITAB(i,1) must be i");
+    for (int j = 0; j < nneigh[i]; ++j) {
+      int v = M->neitab[i + (size_t)np * j] - 1;
+      if (v < 0 || v >= np) return fail(ADCGPU_E_ARG, "NEITAB out of range");
+      col[(size_t)j * np + i] = v;
+    }
+  }
+  std::vector<int> csrpos(6 * (size_t)ne);
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l < 2; ++l) {
+        int J = M->csr_position[e + (size_t)ne * k + (size_t)3 * ne * l];
+        int node = nm0[3 * (size_t)e + k];
+        if (J < 2 || J > nneigh[node]) return fail(ADCGPU_E_ARG, "CSR_POSITION out of range");
+        csrpos[(3 * (size_t)e + k) * 2 + l] = J - 1;
+      }
+  std::vector<double> fdx(3 * (size_t)ne), fdy(3 * (size_t)ne);
+  for (size_t q = 0; q < 3 * (size_t)ne; ++q) {
+    fdx[q] = M->fdxe[q];
+    fdy[q] = M->fdye[q];
+  }
+  std::vector<int> bnd_index(np, -1), nbd0(std::max(neta, 1), 0);
+  for (int I = 0; I < neta; ++I) {
+    int b = M->nbd[I] - 1;
+    if (b < 0 || b >= np) return fail(ADCGPU_E_ARG, "NBD out of range");
+    if (bnd_index[b] >= 0) return fail(ADCGPU_E_UNSUPPORTED, "repeated elevation-boundary node");
+    bnd_index[b] = I;
+    nbd0[I] = b;
+  }
+  std::vector<int> obc_slot((size_t)mnei * np, -1);
+  std::vector<std::vector<int>> obcl(np);
+  for (int I = 0; I < neta; ++I) {
+    int b = nbd0[I];
+    for (int J = 2; J <= nneigh[b]; ++J) {
+      int nb = col[(size_t)(J - 1) * np + b];
+      int found = 0;
+      for (int IJ = 2; IJ <= nneigh[nb]; ++IJ)
+        if (col[(size_t)(IJ - 1) * np + nb] == b) {
+          obc_slot[(size_t)(IJ - 1) * np + nb] = I * mnei + (J - 2);
+          ++found;
+        }
+      if (found != 1) return fail(ADCGPU_E_ARG, "asymmetric neighbour table at a boundary");
+      obcl[nb].push_back(I * mnei + (J - 2));
+    }
+  }
+  std::vector<int> obcg_ptr(np + 1, 0), obcg;
+  for (int i = 0; i < np; ++i) {
+    obcg_ptr[i + 1] = obcg_ptr[i] + (int)obcl[i].size();
+    obcg.insert(obcg.end(), obcl[i].begin(), obcl[i].end());
+  }
+  std::vector<std::vector<int>> vbl(np);
+  std::vector<int> lbc(std::max(M->nvel, 1), 0);
+  std::vector<double> csii(std::max(M->nvel, 1), 0.0), siii(std::max(M->nvel, 1), 0.0);
+  for (int I = 0; I < M->nvel; ++I) {
+    lbc[I] = M->lbcodei[I];
+    csii[I] = M->csii[I];
+    siii[I] = M->siii[I];
+    if (lbc[I] < 0 || lbc[I] > 29)
+      return fail(ADCGPU_E_UNSUPPORTED, "flow boundary type " + std::to_string(lbc[I]));
+  }
+  for (int J = 0; J < M->nvelme; ++J) {
+    int I = M->me2gw[J] - 1;
+    if (I < 0 || I >= M->nvel) return fail(ADCGPU_E_ARG, "ME2GW out of range");
+    int b = M->nbv[I] - 1;
+    if (b < 0 || b >= np) return fail(ADCGPU_E_ARG, "NBV out of range");
+    vbl[b].push_back(I);
+  }
+  std::vector<int> vbc_ptr(np + 1, 0), vbc;
+  for (int i = 0; i < np; ++i) {
+    vbc_ptr[i + 1] = vbc_ptr[i] + (int)vbl[i].size();
+    vbc.insert(vbc.end(), vbl[i].begin(), vbl[i].end());
+  }
+
+  adcgpu_ctx* c = new adcgpu_ctx();
+  c->device = device;
+  c->np = np; c->ne = ne; c->mnei = mnei; c->neta = neta;
+  c->nch = (np + RED_CHUNK - 1) / RED_CHUNK;
+  c->prm = *P;
+  c->ifnlfa = P->nolifa == 0 ? 0.0 : 1.0;
+  c->ifnlct = P->nolica == 0 ? 0.0 : 1.0;
+  c->ifnlcat = P->nolicat == 0 ? 0.0 : 1.0;
+  c->iflinbf = P->nolibf == 0 ? 1.0 : 0.0;
+  c->ifnlbf = (P->nolibf == 1 || P->nolibf == 3) ? 1.0 : 0.0;
+  c->tkmax = (double)2.0f / P->dtdp;
+  c->habsmin = 0.8 * P->h0;
+  c->hoff = 1.2 * P->h0;
+  int rc = 0;
+  auto U = [&](auto** p, const auto& v) {
+    if (!rc) rc = upload(p, v);
+    if (!rc) c->allocs.push_back((void*)*p);
+  };
+  auto A = [&](auto** p, size_t n) {
+    if (!rc) rc = dmalloc(p, n);
+    if (!rc) {
+      c->allocs.push_back((void*)*p);
+      (void)hipMemset(*p, 0, n * sizeof(**p));
+    }
+  };
+  std::vector<double> vx(M->x, M->x + np), vy(M->y, M->y + np), vdp(M->dp, M->dp + np);
+  std::vector<double> vt(P->tau0var, P->tau0var + np), ve(P->evm, P->evm + np);
+  std::vector<double> vcor(P->corif, P->corif + np), vfric(P->fric, P->fric + np);
+  std::vector<double> var(M->areas, M->areas + ne);
+  U(&c->nm, nm0); U(&c->inc_ptr, cnt); U(&c->inc, inc); U(&c->nneigh, nneigh); U(&c->col, col);
+  U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_slot, obc_slot);
+  U(&c->obcg_ptr, obcg_ptr); U(&c->obcg, obcg); U(&c->vbc_ptr, vbc_ptr); U(&c->vbc, vbc);
+  U(&c->lbcodei, lbc); U(&c->fdx, fdx); U(&c->fdy, fdy); U(&c->areas, var); U(&c->x, vx);
+  U(&c->y, vy); U(&c->dp, vdp); U(&c->tau0, vt); U(&c->evm, ve); U(&c->corif, vcor);
+  U(&c->csii, csii); U(&c->siii, siii); U(&c->fric, vfric);
+  double** nodal[] = {&c->eta1, &c->eta2, &c->etas, &c->uu1, &c->vv1, &c->uu2, &c->vv2,
+                      &c->qx1, &c->qy1, &c->qx2, &c->qy2, &c->h1, &c->h2, &c->tk,
+                      &c->etamax, &c->totalarea, &c->wsx1, &c->wsy1, &c->wsx2, &c->wsy2,
+                      &c->pr1, &c->pr2, &c->gwce_lv, &c->ju1, &c->jd0, &c->jd1, &c->jw};
+  for (auto p : nodal) A(p, np);
+  A(&c->alphal, ne); A(&c->coef, (size_t)mnei * np); A(&c->cs, (size_t)mnei * np);
+  A(&c->obc, (size_t)std::max(neta, 1) * mnei);
+  A(&c->nodecode, np); A(&c->nnodecode, np); A(&c->noff, ne); A(&c->noffold, ne);
+  A(&c->mju, np);
+  A(&c->contrib, 3 * (size_t)ne); A(&c->cx, 3 * (size_t)ne); A(&c->cy, 3 * (size_t)ne);
+  A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
+  c->cand_cap = WD_SORT_CAP;
+  A(&c->cand, c->cand_cap);
+  A(&c->flags, F_COUNT); A(&c->jpart, (size_t)c->nch * 12); A(&c->jsaved, 8); A(&c->jbar, 1);
+  if (!rc) {
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
+  }
+  if (!rc) rc = setup_jcg_grid(c);
+  if (!rc) {
+    std::vector<double> ones(ne, 1.0), pr(np, P->prbckgrnd_mh2o), emax(np, -99999.0);
+    (void)hipMemcpy(c->alphal, ones.data(), ne * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->pr1, pr.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->pr2, pr.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->etamax, emax.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    std::vector<int> one_i(std::max(np, ne), 1);
+    (void)hipMemcpy(c->nodecode, one_i.data(), np * sizeof(int), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->noff, one_i.data(), ne * sizeof(int), hipMemcpyHostToDevice);
+    int f[F_COUNT] = {0};
+    f[F_NCCHANGE] = 1;
+    (void)hipMemcpy(c->flags, f, sizeof(f), hipMemcpyHostToDevice);
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
+  }
+  if (rc) {
+    adcgpu_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+void adcgpu_destroy(adcgpu_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->eta_bc_table) hipFree(c->eta_bc_table);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static double** field_ptr(adcgpu_ctx* c, int f) {
+  switch (f) {
+    case ADCGPU_ETA1: return &c->eta1;
+    case ADCGPU_ETA2: return &c->eta2;
+    case ADCGPU_ETAS: return &c->etas;
+    case ADCGPU_UU1: return &c->uu1;
+    case ADCGPU_VV1: return &c->vv1;
+    case ADCGPU_UU2: return &c->uu2;
+    case ADCGPU_VV2: return &c->vv2;
+    case ADCGPU_QX1: return &c->qx1;
+    case ADCGPU_QY1: return &c->qy1;
+    case ADCGPU_QX2: return &c->qx2;
+    case ADCGPU_QY2: return &c->qy2;
+    case ADCGPU_H1: return &c->h1;
+    case ADCGPU_H2: return &c->h2;
+    case ADCGPU_TK: return &c->tk;
+    case ADCGPU_ETAMAX: return &c->etamax;
+    case ADCGPU_TOTALAREA: return &c->totalarea;
+    case ADCGPU_WSX1: return &c->wsx1;
+    case ADCGPU_WSY1: return &c->wsy1;
+    case ADCGPU_WSX2: return &c->wsx2;
+    case ADCGPU_WSY2: return &c->wsy2;
+    case ADCGPU_PR1: return &c->pr1;
+    case ADCGPU_PR2: return &c->pr2;
+    case ADCGPU_GWCE_LV: return &c->gwce_lv;
+    case ADCGPU_COEF: return &c->coef;
+    case ADCGPU_ALPHAL: return &c->alphal;
+  }
+  return nullptr;
+}
+
+static int** ifield_ptr(adcgpu_ctx* c, int f) {
+  switch (f) {
+    case ADCGPU_NODECODE: return &c->nodecode;
+    case ADCGPU_NOFF: return &c->noff;
+    case ADCGPU_MJU: return &c->mju;
+  }
+  return nullptr;
+}
+
+static int64_t field_len(adcgpu_ctx* c, int f) {
+  if (f == ADCGPU_COEF) return (int64_t)c->np * c->mnei;
+  if (f == ADCGPU_NOFF || f == ADCGPU_ALPHAL) return c->ne;
+  return c->np;
+}
+
+int adcgpu_set_field(adcgpu_ctx* c, int f, const void* host, int64_t count) {
+  if (!c || !host) return fail(ADCGPU_E_ARG, "null argument");
+  if (count != field_len(c, f)) return fail(ADCGPU_E_ARG, "field length");
+  HIPCHK(hipSetDevice(c->device));
+  if (double** p = field_ptr(c, f)) {
+    if (f == ADCGPU_COEF) {
+      // host COEF(NP,MNEI) column-major -> device rows (same layout, unused
+      // padding slots stay as given)
+    }
+    HIPCHK(hipMemcpyAsync(*p, host, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else if (int** q = ifield_ptr(c, f)) {
+    HIPCHK(hipMemcpyAsync(*q, host, count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  } else {
+    return fail(ADCGPU_E_ARG, "unknown field");
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int adcgpu_get_field(adcgpu_ctx* c, int f, void* host, int64_t count) {
+  if (!c || !host) return fail(ADCGPU_E_ARG, "null argument");
+  if (count != field_len(c, f)) return fail(ADCGPU_E_ARG, "field length");
+  HIPCHK(hipSetDevice(c->device));
+  if (double** p = field_ptr(c, f)) {
+    HIPCHK(hipMemcpyAsync(host, *p, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  } else if (int** q = ifield_ptr(c, f)) {
+    HIPCHK(hipMemcpyAsync(host, *q, count * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    return fail(ADCGPU_E_ARG, "unknown field");
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode,
+                      const int32_t* noff) {
+  if (!c || !eta2 || !nodecode || !noff) return fail(ADCGPU_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t np = c->np, ne = c->ne;
+  HIPCHK(hipMemcpyAsync(c->eta2, eta2, np * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->eta1, eta2, np * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->nodecode, nodecode, np * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->nnodecode, nodecode, np * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->noff, noff, ne * 4, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_total_area, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state());
+  int f[F_COUNT] = {0};
+  f[F_NCCHANGE] = 1;
+  HIPCHK(hipMemcpyAsync(c->flags, f, sizeof(f), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return check_launch("cold start");
+}
+
+int adcgpu_begin_step(adcgpu_ctx* c, int it) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  if (it == 1)
+    hipLaunchKernelGGL(k_init_h, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->dp,
+                       c->eta2, c->ifnlfa, c->h1, c->h2);
+  // QX1 <- QX2, UU1 <- UU2, ... WSX1 <- WSX2, PR1 <- PR2 (src/timestep.F:385-431)
+  std::swap(c->qx1, c->qx2); std::swap(c->qy1, c->qy2);
+  std::swap(c->uu1, c->uu2); std::swap(c->vv1, c->vv2);
+  if (c->prm.nws != 0) {
+    std::swap(c->wsx1, c->wsx2); std::swap(c->wsy1, c->wsy2); std::swap(c->pr1, c->pr2);
+  }
+  return check_launch("begin_step");
+}
+
+int adcgpu_friction(adcgpu_ctx* c) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  hipLaunchKernelGGL(k_friction, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->uu1,
+                     c->vv1, c->dp, c->eta2, c->fric, c->ifnlfa, c->iflinbf, c->ifnlbf, c->tkmax,
+                     c->tk);
+  return check_launch("friction");
+}
+
+int adcgpu_set_forcing(adcgpu_ctx* c, const double* wsx2, const double* wsy2, const double* pr2) {
+  if (!c || !wsx2 || !wsy2 || !pr2) return fail(ADCGPU_E_ARG, "null argument");
+  const size_t n = (size_t)c->np * 8;
+  HIPCHK(hipMemcpyAsync(c->wsx2, wsx2, n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->wsy2, wsy2, n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->pr2, pr2, n, hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc) {
+  const int np = c->np, ne = c->ne;
+  hipLaunchKernelGGL(k_gwce_flags, dim3(1), dim3(1), 0, c->stream, c->flags);
+  hipLaunchKernelGGL(k_lhs_rows, dim3(c->nch), dim3(RED_THREADS), 0, c->stream, c->mesh(),
+                     c->state(), c->flags, c->ep_part);
+  hipLaunchKernelGGL(k_ep, dim3(1), dim3(RED_THREADS), 0, c->stream, c->nch, np, c->ep_part,
+                     c->flags, c->scal);
+  hipLaunchKernelGGL(k_lhs_bc, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->flags, c->scal);
+  RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, c->prm.nws};
+  hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), q, c->contrib);
+  // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
+  std::swap(c->eta1, c->eta2);
+  std::swap(c->h1, c->h2);
+  hipLaunchKernelGGL(k_rhs_node, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->contrib, d_eta_bc, c->scal);
+  int rc = check_launch("gwce assembly");
+  if (rc) return rc;
+  return launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, np, c->prm.itmax,
+                    c->prm.convcr, c->eta2, c->eta1, c->nodecode);
+}
+
+int adcgpu_solve_gwce(adcgpu_ctx* c, int it, const double* eta_bc) {
+  (void)it;
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  if (c->neta > 0) {
+    if (!eta_bc) return fail(ADCGPU_E_ARG, "eta_bc required");
+    HIPCHK(hipMemcpyAsync(c->eta_bc, eta_bc, c->neta * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  return gwce_device(c, c->eta_bc);
+}
+
+int adcgpu_wetdry(adcgpu_ctx* c, int it) {
+  (void)it;
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  const int np = c->np, ne = c->ne;
+  if (c->prm.nolifa != 2) {
+    hipLaunchKernelGGL(k_h2, dim3(nblk(np, 256)), dim3(256), 0, c->stream, np, c->dp, c->eta2,
+                       c->ifnlfa, c->h2);
+    return check_launch("h2");
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_NCAND, 0, 1);
+  hipLaunchKernelGGL(k_wd_d1, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->flags, c->prm.h0, c->habsmin);
+  WdPrm q{c->hoff, c->prm.velmin, c->iflinbf, c->ifnlbf, c->tkmax, c->cand_cap};
+  hipLaunchKernelGGL(k_wd_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), q, c->cand, c->flags);
+  hipLaunchKernelGGL(k_wd_scan, dim3(1), dim3(1024), 0, c->stream, c->cand, c->flags, c->tk);
+  hipLaunchKernelGGL(k_wd_d2, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->flags, c->ifnlfa);
+  hipLaunchKernelGGL(k_wd_alpha, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->flags, 1.0e9);
+  return check_launch("wetdry");
+}
+
+int adcgpu_momentum(adcgpu_ctx* c) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  MomPrm q{c->ifnlct, c->prm.nws};
+  hipLaunchKernelGGL(k_mom_elem, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), q, c->cx, c->cy);
+  hipLaunchKernelGGL(k_mom_node, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), q, c->cx, c->cy);
+  return check_launch("momentum");
+}
+
+int adcgpu_timestep(adcgpu_ctx* c, int it, const double* eta_bc, const double* wsx2,
+                    const double* wsy2, const double* pr2) {
+  int rc = adcgpu_begin_step(c, it);
+  if (!rc) rc = adcgpu_friction(c);
+  if (!rc && c->prm.nws != 0) rc = adcgpu_set_forcing(c, wsx2, wsy2, pr2);
+  if (!rc) rc = adcgpu_solve_gwce(c, it, eta_bc);
+  if (!rc) rc = adcgpu_wetdry(c, it);
+  if (!rc) rc = adcgpu_momentum(c);
+  return rc;
+}
+
+int adcgpu_last_numitr(adcgpu_ctx* c, int* numitr, int* ier) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  int f[F_COUNT];
+  HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (numitr) *numitr = f[F_NUMITR];
+  if (ier) *ier = f[F_IER];
+  if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
+  if (f[F_ERROR] & ERR_CAPACITY) return fail(ADCGPU_E_CAPACITY, "wetting candidate capacity");
+  if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
+  return 0;
+}
+
+int adcgpu_synchronize(adcgpu_ctx* c) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int adcgpu_holland(adcgpu_ctx* c, double t, const double* prm13) {
+  if (!c || !prm13) return fail(ADCGPU_E_ARG, "null argument");
+  HollandPrm p;
+  for (int k = 0; k < 13; ++k) p.v[k] = prm13[k];
+  p.t = t;
+  hipLaunchKernelGGL(k_holland, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->x,
+                     c->y, (const double*)nullptr, 0, p, c->wsx2, c->wsy2, c->pr2);
+  return check_launch("holland");
+}
+
+int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
+               const double* holland, int use_graph) {
+  (void)use_graph;
+  if (!c || nsteps < 0) return fail(ADCGPU_E_ARG, "invalid argument");
+  if (c->prm.nws != 0 && !holland) return fail(ADCGPU_E_ARG, "met forcing required");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->neta > 0) {
+    if (!eta_bc_table) return fail(ADCGPU_E_ARG, "eta_bc_table required");
+    if (c->eta_bc_table_steps < nsteps) {
+      if (c->eta_bc_table) hipFree(c->eta_bc_table);
+      c->eta_bc_table = nullptr;
+      HIPCHK(hipMalloc(&c->eta_bc_table, (size_t)nsteps * c->neta * sizeof(double)));
+      c->eta_bc_table_steps = nsteps;
+    }
+    HIPCHK(hipMemcpyAsync(c->eta_bc_table, eta_bc_table, (size_t)nsteps * c->neta * 8,
+                          hipMemcpyHostToDevice, c->stream));
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_ITSUM, 0, 1);
+  for (int s = 0; s < nsteps; ++s) {
+    const int it = it0 + s;
+    int rc = adcgpu_begin_step(c, it);
+    if (!rc) rc = adcgpu_friction(c);
+    if (!rc && holland) rc = adcgpu_holland(c, it * c->prm.dtdp, holland);
+    if (!rc) rc = gwce_device(c, c->neta > 0 ? c->eta_bc_table + (size_t)s * c->neta : c->eta_bc);
+    if (!rc) rc = adcgpu_wetdry(c, it);
+    if (!rc) rc = adcgpu_momentum(c);
+    if (rc) return rc;
+  }
+  int f[F_COUNT];
+  HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->run_iters = f[F_ITSUM];
+  if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
+  if (f[F_ERROR] & ERR_CAPACITY) return fail(ADCGPU_E_CAPACITY, "wetting candidate capacity");
+  return 0;
+}
+
+int64_t adcgpu_run_iterations(adcgpu_ctx* c) { return c ? c->run_iters : -1; }
+
+int adcgpu_profile(adcgpu_ctx* c, int enable) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& ev : c->jcg_events) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
+  c->jcg_events.clear();
+  c->profile = enable;
+  return 0;
+}
+
+int adcgpu_profile_read(adcgpu_ctx* c, double* jcg_ms, int64_t* launches) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  for (auto& ev : c->jcg_events) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev.first, ev.second));
+    tot += ms;
+  }
+  if (jcg_ms) *jcg_ms = tot;
+  if (launches) *launches = (int64_t)c->jcg_events.size();
+  return 0;
+}
+
+// Periodic elevation forcing at the NBD nodes (src/gwce.F:1638-1651), on the
+// host with the C library's cos like the reference.
+int adcgpu_elevation_bc(int neta, int nbfr, const double* amig, const double* ff,
+                        const double* face, const double* emo, const double* efa, double timeh,
+                        double rampelev, double* out) {
+  if (neta < 0 || nbfr < 0 || (neta > 0 && !out)) return fail(ADCGPU_E_ARG, "invalid argument");
+  const double pi = 3.141592653589793;
+  for (int I = 0; I < neta; ++I) out[I] = 0.0;
+  for (int J = 0; J < nbfr; ++J) {
+    const double per = amig[J] == 0.0 ? 0.0 : 2.0 * pi / amig[J];
+    int ncyc = 0;
+    if (per != 0.0) ncyc = (int)(timeh / per);
+    const double argj = amig[J] * (timeh - ncyc * per) + face[J];
+    const double rff = ff[J] * rampelev;
+    for (int I = 0; I < neta; ++I) {
+      const double arg = argj - efa[(size_t)I * nbfr + J];
+      out[I] = out[I] + emo[(size_t)I * nbfr + J] * rff * std::cos(arg);
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- drop-in JCG
+int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double* rhs, double* u,
+               int32_t* iparm, double* rparm) {
+  if (n <= 0) return 11;
+  if (!jcoef || !coef || !rhs || !u || !iparm || !rparm || ndim < n || maxnz < 1)
+    return fail(ADCGPU_E_ARG, "invalid argument");
+  if (iparm[0] <= 0) return 0;
+  if (iparm[5] != 1 || iparm[8] != -2 || iparm[9] != 0)
+    return fail(ADCGPU_E_UNSUPPORTED, "only adaptive JCG without red-black or row removal");
+  const int itmax = iparm[0];
+  if (itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
+  // compact the ELLPACK rows: diagonal first (scal moves it there,
+  // src/itpackv.F:2331-2356), padding (jcoef <= 0) dropped
+  int width = 1;
+  std::vector<int> nn(n, 0);
+  std::vector<std::vector<int>> pos(n);
+  for (int i = 0; i < n; ++i) {
+    int dj = -1;
+    for (int j = 0; j < maxnz; ++j)
+      if (jcoef[i + (size_t)ndim * j] == i + 1) dj = j;
+    if (dj < 0) return 402;
+    pos[i].push_back(dj);
+    for (int j = 0; j < maxnz; ++j) {
+      int v = jcoef[i + (size_t)ndim * j];
+      if (j != dj && v > 0) pos[i].push_back(j);
+      if (v > n) return fail(ADCGPU_E_ARG, "jcoef out of range");
+    }
+    nn[i] = (int)pos[i].size();
+    width = std::max(width, nn[i]);
+    double d = coef[i + (size_t)ndim * dj];
+    if (d == 0.0) return 401;
+    if (d < 0.0) return fail(ADCGPU_E_UNSUPPORTED, "negative diagonal (nonsymmetric scaling)");
+  }
+  std::vector<double> hc((size_t)width * n, 0.0);
+  std::vector<int> hcol((size_t)width * n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < nn[i]; ++k) {
+      hc[(size_t)k * n + i] = coef[i + (size_t)ndim * pos[i][k]];
+      hcol[(size_t)k * n + i] = jcoef[i + (size_t)ndim * pos[i][k]] - 1;
+    }
+  // one persistent scratch context per device, holding ITPACK's saved state
+  static adcgpu_ctx* jc[64] = {nullptr};
+  static int jc_n[64] = {0}, jc_w[64] = {0};
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(ADCGPU_E_ARG, "device ordinal");
+  adcgpu_ctx*& c = jc[dev];
+  if (!c || jc_n[dev] < n || jc_w[dev] < width) {
+    double saved[8] = {0};
+    if (c) {
+      (void)hipMemcpy(saved, c->jsaved, sizeof(saved), hipMemcpyDeviceToHost);
+      for (void* p : c->allocs) hipFree(p);
+      if (c->stream) hipStreamDestroy(c->stream);
+      delete c;
+    }
+    c = new adcgpu_ctx();
+    c->device = dev;
+    c->np = n;
+    c->mnei = width;
+    c->nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+    int rc = 0;
+    auto A = [&](auto** p, size_t cnt) {
+      if (!rc) rc = dmalloc(p, cnt);
+      if (!rc) c->allocs.push_back((void*)*p);
+    };
+    A(&c->coef, (size_t)width * n); A(&c->cs, (size_t)width * n); A(&c->col, (size_t)width * n);
+    A(&c->nneigh, n); A(&c->gwce_lv, n); A(&c->etas, n); A(&c->ju1, n); A(&c->jd0, n);
+    A(&c->jd1, n); A(&c->jw, n); A(&c->jpart, (size_t)c->nch * 12); A(&c->jsaved, 8);
+    A(&c->jbar, 1); A(&c->flags, F_COUNT);
+    if (!rc) HIPCHK(hipStreamCreate(&c->stream));
+    if (!rc) rc = setup_jcg_grid(c);
+    if (rc) {
+      jc[dev] = nullptr;
+      return rc;
+    }
+    HIPCHK(hipMemcpy(c->jsaved, saved, sizeof(saved), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->flags, 0, F_COUNT * sizeof(int)));
+    jc_n[dev] = n;
+    jc_w[dev] = width;
+  }
+  c->nch = (n + RED_CHUNK - 1) / RED_CHUNK;
+  if (c->jcg_grid * JCG_MAXK < c->nch) return fail(ADCGPU_E_CAPACITY, "n too large");
+  HIPCHK(hipMemcpyAsync(c->coef, hc.data(), hc.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->col, hcol.data(), hcol.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->nneigh, nn.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->gwce_lv, rhs, n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->etas, u, n * 8, hipMemcpyHostToDevice, c->stream));
+  // the device matrix is stored with leading dimension n
+  int rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, n, itmax, rparm[0],
+                      nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  double saved[8];
+  int f[F_COUNT];
+  HIPCHK(hipMemcpyAsync(hc.data(), c->coef, hc.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(rhs, c->gwce_lv, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(u, c->etas, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(saved, c->jsaved, sizeof(saved), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < nn[i]; ++k) coef[i + (size_t)ndim * pos[i][k]] = hc[(size_t)k * n + i];
+  const int in = f[F_NUMITR];
+  const int ier = f[F_IER];
+  iparm[7] = 4 * n + 4 * in;
+  if (ier == 13 && iparm[2] == 0) rparm[0] = saved[5];
+  if (iparm[2] == 0) {
+    iparm[0] = in;
+    iparm[8] = -2;
+    rparm[1] = saved[4];
+    rparm[2] = rparm[2] == 0.0 ? -1.0 : rparm[2];
+    rparm[10] = saved[5] > 0.0 ? -std::log10(std::fabs(saved[5])) : -std::log10(std::ldexp(1.0, -52));
+  }
+  return ier;
+}
+
+}  // extern "C"

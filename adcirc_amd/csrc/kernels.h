@@ -1,0 +1,137 @@
+// Device-side data layout shared by the kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace adcgpu {
+
+// fixed blocked reduction: 2048-row chunks, 256 threads x 8 rows
+constexpr int RED_THREADS = 256;
+constexpr int RED_PER_THREAD = 8;
+constexpr int RED_CHUNK = RED_THREADS * RED_PER_THREAD;
+
+// persistent JCG launch: 4 row waves + 1 wave for the eigenvalue estimate
+constexpr int JCG_BLOCK = RED_THREADS + 64;
+constexpr int JCG_MAXK = 2;          // chunks per block kept in registers
+constexpr int JCG_MAX_ITMAX = 256;   // tridiagonal history length (itmax)
+constexpr int WD_SORT_CAP = 4096;    // wetting candidates per step
+
+// device flag slots
+enum { F_NCCHANGE = 0, F_REBUILD, F_NUMITR, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_COUNT = 16 };
+enum { ERR_DIAG = 1, ERR_CAPACITY = 2, ERR_BARRIER = 4 };
+// device scalar slots
+enum { S_EP = 0, S_COUNT = 8 };
+
+struct DevMesh {
+  int np, ne, mnei, neta;
+  double dt, g, a00, b00, c00;
+  const int* nm;        // [ne*3] 0-based
+  const double* fdx;    // [ne*3]
+  const double* fdy;    // [ne*3]
+  const double* areas;  // [ne]
+  const double* x;
+  const double* y;
+  const double* dp;
+  const double* tau0;
+  const double* evm;
+  const double* corif;
+  const int* inc_ptr;   // [np+1] node -> incident (element, local) codes 3e+k
+  const int* inc;       //        sorted by element id
+  const int* nneigh;    // [np]
+  const int* col;       // [mnei*np] column-major, 0-based (slot 0 = self)
+  const int* csrpos;    // [ne*3*2] 0-based column slots
+  const int* bnd_index; // [np] elevation-boundary index or -1
+  const int* nbd;       // [neta] 0-based
+  const int* obc_slot;  // [mnei*np] boundary-column slot I*mnei+(J-2) or -1
+  const int* obcg_ptr;  // [np+1] node -> boundary-column slots, (I,J) order
+  const int* obcg;
+  const int* vbc_ptr;   // [np+1] node -> flow-boundary entries in ME2GW order
+  const int* vbc;
+  const int* lbcodei;
+  const double* csii;
+  const double* siii;
+};
+
+struct DevState {
+  double *eta1, *eta2, *etas, *uu1, *vv1, *uu2, *vv2, *qx1, *qy1, *qx2, *qy2;
+  double *h1, *h2, *tk, *etamax, *totalarea, *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2;
+  double *gwce_lv, *alphal, *coef, *obc, *fric;
+  int *nodecode, *nnodecode, *noff, *noffold, *mju;
+};
+
+struct RhsPrm {
+  double ifnlfa, ifnlct, ifnlcat;
+  int nws;
+};
+
+struct MomPrm {
+  double ifnlct;
+  int nws;
+};
+
+struct WdPrm {
+  double hoff, velmin, iflinbf, ifnlbf, tkmax;
+  int cand_cap;
+};
+
+struct WdCand {
+  int e, a, p;
+  double tk;
+};
+
+struct HollandPrm {
+  double v[13];
+  double t;
+};
+
+struct JcgArgs {
+  int n, itmax, maxfn;
+  double zeta, tz;
+  double* coef;       // [mnei*n] unscaled in/out (diag in slot 0)
+  double* cs;         // [mnei*n] scaled work copy
+  const int* col;
+  const int* nneigh;
+  double* rhs;
+  double* u0;         // initial guess in, ETAS out
+  double* u1;
+  double* d0;
+  double* d1;
+  double* w;
+  double* part;       // [nch*3*4] wave partials
+  double* saved;      // ITPACK SAVE state: q1, gamold, rhoold, cmold, cme, stptst
+  unsigned* barrier;
+  int* flags;
+  double* etas;       // = u0 (kept for clarity)
+  double* eta2;       // optional: ETA2 = NODECODE*(ETAS+0)+ETA1
+  const double* eta1;
+  const int* nodecode;
+};
+
+__global__ void k_init_h(int np, const double* dp, const double* eta2, double ifnlfa, double* h1,
+                         double* h2);
+__global__ void k_friction(int np, const double* uu1, const double* vv1, const double* dp,
+                           const double* eta2, const double* fric, double ifnlfa, double iflinbf,
+                           double ifnlbf, double tkmax, double* tk);
+__global__ void k_holland(int np, const double* x, const double* y, const double* tptr, int tstep,
+                          HollandPrm p, double* wsx, double* wsy, double* pr);
+__global__ void k_gwce_flags(int* flags);
+__global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, double* ep_part);
+__global__ void k_ep(int nch, int np, const double* part, const int* flags, double* scal);
+__global__ void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* scal);
+__global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, double* contrib);
+__global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const double* eta_bc,
+                           const double* scal);
+__global__ void k_jcg(JcgArgs a);
+__global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
+__global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdCand* cand, int* flags);
+__global__ void k_wd_scan(const WdCand* cand, int* flags, double* tk);
+__global__ void k_wd_d2(DevMesh m, DevState s, int* flags, double ifnlfa);
+__global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim);
+__global__ void k_h2(int np, const double* dp, const double* eta2, double ifnlfa, double* h2);
+__global__ void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* cx, double* cy);
+__global__ void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* cx, const double* cy);
+__global__ void k_fill_u32(unsigned* p, unsigned v, int n);
+__global__ void k_fill_i32(int* p, int v, int n);
+__global__ void k_total_area(DevMesh m, DevState s);
+
+}  // namespace adcgpu

@@ -1,0 +1,164 @@
+"""Benchmark: simulated seconds per wall second on the 1M-node coastal mesh.
+
+One step is one 2DDI timestep of the hot path (friction, met forcing, GWCE
+LHS when wet/dry changed, GWCE RHS, elevation BCs, JCG to CONVCR = 1e-8,
+wet/dry, momentum, peak surge) on the whole mesh, inputs resident in HBM.
+Workload: the synthetic coastal mesh (Delaunay of seed-sampled points, ~1.0M
+nodes / ~2.0M elements, 5 km -> 100 m), tanh shelf 4000 m -> -3 m, wet/dry
+with H0 = 0.05 m, M2 + K1 on the open boundary, Holland vortex (Pc 940 hPa,
+Rmax 40 km, B 1.3, 5 m/s shoreward), dt = 1 s.
+
+    python bench.py [--gpus 1] [--steps 600] [--warmup 60] [--nodes 1000000]
+
+Prints one JSON line.  The dominant kernel's roofline entry is computed from
+the JCG launches timed with HIP events on the library's stream; the CPU
+baseline is the oracle (serial C restatement of the reference, bit-exact to
+it) timed on a bounded sample of the same run on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_case(nodes: int):
+    from adcirc_amd import meshgen, prep
+
+    t0 = time.time()
+    case = meshgen.coastal(nodes)
+    tables = prep.build_tables(case.mesh, case.run.nolifa, case.run.h0, case.run.anginn)
+    log(f"mesh {tables['np']} nodes / {tables['ne']} elements in {time.time() - t0:.1f} s")
+    return case, tables
+
+
+def cpu_baseline(case, tables, eta2, nodecode, noff, budget_s: float):
+    """The oracle (test infrastructure: the checker) timed on this host's core
+    on the first steps of the same run."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib
+
+    run = case.run
+    npn = int(tables["np"])
+    d = dict(tables)
+    d["emo"] = np.asarray(case.emo, np.float64).ravel()
+    d["efa"] = (np.asarray(case.efa, np.float64) * (3.141592653589793 / 180.0)).ravel()
+    d["tau0var"] = np.full(npn, run.tau0)
+    d["evm"] = np.full(npn, run.eslm)
+    d["fric"] = np.full(npn, run.cf)
+    d["nodecode0"], d["noff0"], d["eta2_0"] = nodecode, noff, eta2
+    d["prbckgrnd_mh2o"] = 100.0 * 1013.0 / (1000.0 * run.g)
+    m = oracle_lib.OracleModel(d, run, oracle_lib.SEQUENTIAL)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        m.step(steps + 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    value = steps * run.dtdp / el
+    return {"value": value, "unit": "sim-s/wall-s", "cores": 1, "kind": "port",
+            "sample": f"first {steps} steps of the same {npn}-node coastal run from cold start, "
+                      f"serial oracle (C restatement bit-exact to the reference build), "
+                      f"{el:.1f} s wall"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--nodes", type=int, default=1_000_000)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the single-mesh and ensemble sharding designs are described in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    from adcirc_amd.model import Model, cold_wetdry_state
+
+    case, tables = build_case(a.nodes)
+    run = case.run
+    eta2, nodecode, noff = cold_wetdry_state(tables, run)
+    g = Model(tables, run, emo=case.emo, efa=case.efa * (3.141592653589793 / 180.0))
+    g.cold_start(eta2, nodecode, noff)
+    g.holland(0.0, run.holland)
+    nsteps = a.warmup + a.steps
+    log(f"elevation boundary table for {nsteps} steps")
+    table = g.eta_bc_table(1, nsteps)
+    log(f"warmup {a.warmup} steps")
+    g.run_resident(1, a.warmup, holland=run.holland, table=table[:a.warmup])
+    g.synchronize()
+    g.profile(True)
+    log(f"timed {a.steps} steps")
+    t0 = time.perf_counter()
+    iters = g.run_resident(a.warmup + 1, a.steps, holland=run.holland, table=table[a.warmup:])
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    jcg_ms, launches = g.profile_read()
+    g.profile(False)
+    npn = int(tables["np"])
+    Z = int(np.asarray(tables["nneigh"]).sum()) - npn
+    # algorithmic bytes of the JCG launches: k*(12 Z + 48 NP) + (12 Z + 64 NP)
+    jcg_bytes = iters * (12.0 * Z + 48.0 * npn) + launches * (12.0 * Z + 64.0 * npn)
+    achieved = jcg_bytes / (jcg_ms * 1e-3) / 1e9 if jcg_ms > 0 else 0.0
+    wet = int(g.get("nodecode").sum())
+    etamax = g.get("etamax")
+    ms_step = wall * 1e3 / a.steps
+    out = {
+        "metric": "sim-s per wall-s, 1M-node mesh: plain vs warm-started JCG at same tol",
+        "value": a.steps * run.dtdp / wall,
+        "unit": "sim-s/wall-s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"coastal Delaunay mesh {npn} nodes / {int(tables['ne'])} elements, "
+                        f"wet/dry H0=0.05, M2+K1, Holland vortex, dt=1 s, JCG tol 1e-8, "
+                        f"plain extrapolated start",
+            "nodes": npn, "elements": int(tables["ne"]), "dt": run.dtdp, "convcr": run.convcr,
+            "warm_start": "plain",
+        },
+        "timesteps_per_s": a.steps / wall,
+        "jcg_iterations_per_step": iters / a.steps,
+        "wet_nodes_end": wet,
+        "peak_surge_max_m": float(etamax.max()),
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_jcg", "kernel_ms_avg": jcg_ms / max(launches, 1),
+            "kernel_share_of_step": (jcg_ms / 1e3) / wall,
+        },
+    }
+    if not a.no_cpu:
+        log("cpu baseline (oracle, 1 core)")
+        out["cpu_baseline"] = cpu_baseline(case, tables, eta2, nodecode, noff, a.cpu_budget)
+    g.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -33,4 +33,7 @@ done
 echo "  FC harness.F90"
 "$FC" -c -O2 -module-dir "$OUT/obj" -I"$OUT/obj" "$HERE/harness.F90" -o "$OUT/obj/harness.o"
 "$FC" -o "$OUT/adchp_ref" "$OUT/obj/harness.o" $OBJS "$@" -lm
-echo "built $OUT/adchp_ref"
+echo "  FC jcg_harness.F90"
+"$FC" -c -O2 -module-dir "$OUT/obj" -I"$OUT/obj" "$HERE/jcg_harness.F90" -o "$OUT/obj/jcg_harness.o"
+"$FC" -o "$OUT/jcg_ref" "$OUT/obj/jcg_harness.o" $OBJS "$@" -lm
+echo "built $OUT/adchp_ref $OUT/jcg_ref"

@@ -1,0 +1,39 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+for p in (REPO, HERE):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs libadcgpu.so on the device)")
+
+
+def load_golden(name):
+    path = os.path.join(GOLDEN, f"{name}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"golden fixture {name}.npz missing (tests/golden/make_golden.py)")
+    return dict(np.load(path))
+
+
+@pytest.fixture(scope="session")
+def beach_golden():
+    return load_golden("beach")
+
+
+@pytest.fixture(scope="session")
+def basin_golden():
+    return load_golden("basin")
+
+
+@pytest.fixture(scope="session")
+def jcg_golden():
+    return load_golden("jcg")

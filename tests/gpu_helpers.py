@@ -1,0 +1,70 @@
+"""Shared helpers of the GPU parity tests: build the product model and the
+oracle (blocked reduction order) from the same reference tables and step
+them side by side."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+
+import oracle_lib
+from adcirc_amd.model import Model
+
+STATE_F = ("eta2", "etas", "uu2", "vv2", "tk")
+STATE_I = ("nodecode", "noff")
+
+
+def golden_tables(d):
+    t = {k: d[k] for k in ("nm", "x", "y", "dp", "fdxe", "fdye", "areas", "nneigh", "neitab",
+                           "csr", "nbd", "nbv", "lbcodei", "csii", "siii", "me2gw")}
+    for k in ("np", "ne", "mnei", "neta", "nvel", "nvelme"):
+        t[k] = int(d[k])
+    return t
+
+
+def build_pair(d, run, order=oracle_lib.BLOCKED):
+    """Product model and oracle model on the same tables and cold start."""
+    t = golden_tables(d)
+    np_, nbfr = t["np"], run.nbfr
+    emo = np.asarray(d["emo"]).reshape(t["neta"], nbfr) if t["neta"] else None
+    efa = np.asarray(d["efa"]).reshape(t["neta"], nbfr) if t["neta"] else None
+    g = Model(t, run, tau0var=d["tau0var"], evm=d["evm"], fric=d["fric"],
+              corif=np.full(np_, run.cori), emo=emo, efa=efa)
+    g.cold_start(d["eta2_0"], d["nodecode0"], d["noff0"])
+    o = oracle_lib.OracleModel(d, run, order)
+    if run.holland is not None:
+        # the cold-start forcing (time 0) is the first step's WSX1/PR1
+        g.set("wsx2", o.wsx); g.set("wsy2", o.wsy); g.set("pr2", o.pr)
+    return g, o
+
+
+def step_pair(g, o, it):
+    """One step on both; the GPU gets the oracle's host forcing."""
+    o.step(it)
+    if o.hol is not None:
+        g.timestep(it, (o.wsx, o.wsy, o.pr))
+    else:
+        g.timestep(it)
+
+
+def compare(g, o, it, fields=STATE_F, ifields=STATE_I):
+    bad = []
+    for k in fields:
+        a, b = g.get(k), o.array(k)
+        if not np.array_equal(a, b):
+            i = int(np.argmax(a != b))
+            bad.append(f"{k}: {int((a != b).sum())} differ, max |d| {np.abs(a - b).max():.3e} "
+                       f"(node {i}: {a[i]!r} vs {b[i]!r})")
+    for k in ifields:
+        a, b = g.get(k), o.iarray(k)
+        if not np.array_equal(a, b):
+            bad.append(f"{k}: {int((a != b).sum())} differ")
+    n, ier = g.numitr()
+    if n != o.numitr():
+        bad.append(f"numitr {n} vs {o.numitr()}")
+    return bad
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()

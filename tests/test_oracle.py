@@ -1,0 +1,224 @@
+"""The oracle against the reference build (golden fixtures) and against
+closed-form results that pin it independently of the reference.
+
+The fixtures were written by the reference's own Fortran modules
+(tests/golden/make_golden.py); the oracle must reproduce them bit for bit:
+the 64-bit hash of the carried state (ETA2, UU2, VV2, ETAS, TK, NODECODE,
+NOFF) after every step, the JCG iteration count of every step, and full
+snapshots.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from adcirc_amd import meshgen
+
+SNAP_F = ("eta1", "eta2", "etas", "uu2", "vv2", "tk", "etamax", "totalarea")
+
+
+def _check_snapshot(m, d, s):
+    for k in SNAP_F:
+        a, b = m.array(k), d[f"s{s}_{k}"]
+        assert np.array_equal(a, b), f"step {s}: {k} differs (max {np.abs(a - b).max():.3e})"
+    assert np.array_equal(m.iarray("nodecode"), d[f"s{s}_nodecode"])
+    assert np.array_equal(m.iarray("noff"), d[f"s{s}_noff"])
+
+
+def _run(d, run, nsteps):
+    m = oracle_lib.OracleModel(d, run, oracle_lib.SEQUENTIAL)
+    snaps = set(int(s) for s in d["snap_steps"])
+    for it in range(1, nsteps + 1):
+        m.step(it)
+        assert m.numitr() == d["step_numitr"][it - 1], f"step {it}: NUMITR"
+        assert m.state_hash() == d["step_hash"][it - 1], f"step {it}: state hash"
+        if it in snaps:
+            _check_snapshot(m, d, it)
+        if it == 1:
+            coef = m.array("coef", int(d["np"]) * int(d["mnei"]))
+            assert np.array_equal(coef, d["s1_coef"]), "COEF after step 1"
+    return m
+
+
+def test_beach_wetdry_storm_hour_matches_reference(beach_golden):
+    """Wet/dry + Holland storm, 3600 steps: bit-identical every step."""
+    _run(beach_golden, meshgen.beach().run, 3600)
+
+
+def test_basin_matches_reference_first_3000_steps(basin_golden):
+    _run(basin_golden, meshgen.basin().run, 3000)
+
+
+def test_basin_full_day_matches_reference(basin_golden):
+    """The 1-day, 43,200-step basin run: every step's hash and the final state."""
+    d = basin_golden
+    _run(d, meshgen.basin().run, int(d["run_nsteps"]))
+
+
+# ---------------------------------------------------------------- JCG
+def _jcg_call(lib, st, n, J, Cm, rhs, u, itmax, zeta):
+    J = np.asfortranarray(J.astype(np.int32)).T.copy()   # (maxnz, n) rows = columns
+    mz = J.shape[0]
+    jc = np.ascontiguousarray(J.ravel())
+    co = np.ascontiguousarray(np.asarray(Cm, np.float64).T.ravel())
+    r = np.ascontiguousarray(rhs, np.float64).copy()
+    uu = np.ascontiguousarray(u, np.float64).copy()
+    iparm = np.zeros(12, np.int32)
+    rparm = np.zeros(12, np.float64)
+    lib.adco_dfault(iparm.ctypes.data_as(oracle_lib._ip), rparm.ctypes.data_as(oracle_lib._dp))
+    iparm[0], iparm[1], iparm[3] = itmax, 0, 16
+    rparm[0] = zeta
+    nw = 4 * n + 4 * itmax
+    wk = np.zeros(nw)
+    ier = lib.adco_jcg(st, n, n, mz, jc.ctypes.data_as(oracle_lib._ip),
+                       co.ctypes.data_as(oracle_lib._dp), r.ctypes.data_as(oracle_lib._dp),
+                       uu.ctypes.data_as(oracle_lib._dp), nw, wk.ctypes.data_as(oracle_lib._dp),
+                       iparm.ctypes.data_as(oracle_lib._ip), rparm.ctypes.data_as(oracle_lib._dp))
+    return ier, iparm, rparm, uu, r, co.reshape(mz, n).T, jc.reshape(mz, n).T
+
+
+def test_jcg_sequence_matches_reference(jcg_golden):
+    """Back-to-back solves in one ITPACK state (q1 and the eigenvalue history
+    carry over, COEF is rescaled in place) reproduce the reference's JCG."""
+    g = jcg_golden
+    lib = oracle_lib.load()
+    st = oracle_lib.Itpack()
+    lib.adco_itpack_init(st, oracle_lib.SEQUENTIAL)
+    prev = {}
+    for name in [str(s) for s in g["names"]]:
+        J, Cm = g[f"{name}_in_jcoef"], g[f"{name}_in_coef"]
+        n = J.shape[0]
+        itmax, zeta = g[f"{name}_prm"]
+        ier, iparm, rparm, u, rhs, co, jc = _jcg_call(lib, st, n, J, Cm, g[f"{name}_in_rhs"],
+                                                     g[f"{name}_in_u"], int(itmax), zeta)
+        assert ier == int(g[f"{name}_ier"])
+        assert iparm[0] == int(g[f"{name}_iparm1"]), name
+        assert iparm[7] == int(g[f"{name}_iparm8"])
+        assert np.array_equal(u, g[f"{name}_u"]), name
+        assert np.array_equal(rhs, g[f"{name}_rhs"]), name
+        assert np.array_equal(co, g[f"{name}_coef"]), name
+        assert np.array_equal(jc, g[f"{name}_jcoef"]), name
+        assert np.array_equal(rparm[[0, 1, 2, 10, 11]], g[f"{name}_rparm"]), name
+        prev = {"coef": co}
+
+
+def test_jcg_closed_forms():
+    """Pins independent of the reference: a diagonal system is solved exactly
+    in one iteration; a Laplacian-plus-mass system agrees with a dense direct
+    solve to the stopping tolerance; the returned matrix is the input up to
+    the scale/unscale rounding."""
+    lib = oracle_lib.load()
+    st = oracle_lib.Itpack()
+    lib.adco_itpack_init(st, oracle_lib.SEQUENTIAL)
+    rng = np.random.default_rng(5)
+    n = 40
+    J = np.zeros((n, 3), np.int32); J[:, 0] = np.arange(1, n + 1)
+    C = np.zeros((n, 3)); C[:, 0] = rng.uniform(0.5, 4.0, n)
+    b = rng.normal(size=n)
+    ier, iparm, _, u, _, co, _ = _jcg_call(lib, st, n, J, C, b, np.zeros(n), 50, 1e-10)
+    assert ier == 0 and iparm[0] <= 1
+    assert np.allclose(u, b / C[:, 0], rtol=1e-14, atol=0)
+    # 1-D Laplacian + mass, dense check
+    n = 60
+    J = np.zeros((n, 4), np.int32); C = np.zeros((n, 4))
+    A = np.zeros((n, n))
+    for i in range(n):
+        J[i, 0] = i + 1; C[i, 0] = 2.1; A[i, i] = 2.1
+        c = 1
+        for j in (i - 1, i + 1):
+            if 0 <= j < n:
+                J[i, c] = j + 1; C[i, c] = -1.0; A[i, j] = -1.0; c += 1
+    b = rng.normal(size=n)
+    ier, iparm, _, u, _, co, _ = _jcg_call(lib, st, n, J, C, b, np.zeros(n), 100, 1e-12)
+    x = np.linalg.solve(A, b)
+    assert ier == 0
+    assert np.linalg.norm(u - x) / np.linalg.norm(x) < 1e-9
+    assert np.allclose(co, C, rtol=4e-16 * 8, atol=0)
+
+
+def test_blocked_reduction_is_close_to_exact():
+    lib = oracle_lib.load()
+    rng = np.random.default_rng(1)
+    for n in (1, 7, 2048, 2049, 100000):
+        x = rng.normal(size=n)
+        y = rng.normal(size=n)
+        exact = math.fsum((x * y).tolist())
+        px, py = x.ctypes.data_as(oracle_lib._dp), y.ctypes.data_as(oracle_lib._dp)
+        blk = lib.adco_dot(oracle_lib.BLOCKED, n, px, py)
+        seq = lib.adco_dot(oracle_lib.SEQUENTIAL, n, px, py)
+        scale = float(np.sum(np.abs(x * y)))
+        assert abs(blk - exact) <= 1e-14 * scale * max(1, math.log2(n))
+        assert abs(seq - exact) <= 1e-15 * scale * n
+
+
+# --------------------------------------------------- physics pins (no reference)
+def _small_case_tables(case):
+    from adcirc_amd import prep
+    t = prep.build_tables(case.mesh, case.run.nolifa, case.run.h0, case.run.anginn)
+    npn = t["np"]
+    d = dict(t)
+    d["emo"] = np.asarray(case.emo, np.float64).ravel()
+    d["efa"] = (np.asarray(case.efa, np.float64) * (3.141592653589793 / 180.0)).ravel()
+    d["tau0var"] = np.full(npn, case.run.tau0)
+    d["evm"] = np.full(npn, case.run.eslm)
+    d["fric"] = np.full(npn, case.run.cf)
+    d["nodecode0"] = np.ones(npn, np.int32)
+    d["noff0"] = np.ones(t["ne"], np.int32)
+    d["eta2_0"] = np.zeros(npn)
+    d["prbckgrnd_mh2o"] = 100.0 * 1013.0 / (1000.0 * case.run.g)
+    return d
+
+
+def test_lake_at_rest_stays_at_rest():
+    """No forcing, flat water: every state array stays exactly zero."""
+    c = meshgen.basin(nx=21, ny=11, lx=20e3, ly=10e3, amp=0.0)
+    d = _small_case_tables(c)
+    m = oracle_lib.OracleModel(d, c.run)
+    for it in range(1, 30):
+        m.step(it)
+    for k in ("eta2", "uu2", "vv2", "etas"):
+        assert np.all(m.array(k) == 0.0), k
+
+
+def test_gwce_matrix_structure():
+    """Consistent GWCE matrix before boundary rows: symmetric, and with no
+    gravity term (A00 = 0) each interior row sums to 2/12+... i.e. the mass
+    matrix row sum (1/dt^2 + tau0/(2 dt)) * TotalArea/3."""
+    c = meshgen.basin(nx=15, ny=9, lx=14e3, ly=8e3, amp=0.0)
+    c.run.a00 = 0.0
+    c.run.b00 = 0.65
+    d = _small_case_tables(c)
+    d["nbd"] = np.zeros(0, np.int32)
+    d["neta"] = 0
+    m = oracle_lib.OracleModel(d, c.run)
+    m.step(1)
+    npn, mnei = int(d["np"]), int(d["mnei"])
+    coef = m.array("coef", npn * mnei).reshape(mnei, npn).T
+    nei = d["neitab"].reshape(mnei, npn).T
+    A = np.zeros((npn, npn))
+    for i in range(npn):
+        for j in range(int(d["nneigh"][i])):
+            A[i, nei[i, j] - 1] += coef[i, j]
+    assert np.allclose(A, A.T, rtol=1e-12, atol=0)
+    dt, tau0 = c.run.dtdp, c.run.tau0
+    ta = m.array("totalarea")
+    rows = A.sum(axis=1)
+    expect = (1.0 / dt + tau0 / 2.0) / dt * ta / 3.0
+    assert np.allclose(rows, expect, rtol=1e-12)
+
+
+def test_friction_closed_form():
+    """Quadratic friction: TK = min(Cf |u| / H, 2/dt) at every node."""
+    c = meshgen.basin(nx=11, ny=6, lx=10e3, ly=5e3)
+    d = _small_case_tables(c)
+    m = oracle_lib.OracleModel(d, c.run)
+    for it in range(1, 40):
+        m.step(it)
+    u, v = m.array("uu2"), m.array("vv2")
+    eta = m.array("eta2")
+    m.step(40)
+    tk = m.array("tk")
+    h = d["dp"] + eta
+    expect = np.minimum(c.run.cf * np.sqrt(u * u + v * v) / h, 2.0 / c.run.dtdp)
+    assert np.allclose(tk, expect, rtol=1e-14, atol=0)
