@@ -81,6 +81,7 @@ struct adcgpu_ctx {
   double *jpart = nullptr, *jsaved = nullptr;
   unsigned* jbar = nullptr;
   int jcg_grid = 0;
+  long long barrier_ticks = 0;  // 0.5 s of the device wall clock
   int64_t run_iters = 0;
   // optional timing of the JCG launches with HIP events on the context stream
   int profile = 0;
@@ -132,6 +133,9 @@ int setup_jcg_grid(adcgpu_ctx* c) {
                                        std::to_string(c->nch) + " chunks, " +
                                        std::to_string(cap) + " resident blocks)");
   c->jcg_grid = std::max(grid, 1);
+  int khz = 0;
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+  c->barrier_ticks = (long long)std::max(khz, 1000) * 500;
   return 0;
 }
 
@@ -164,6 +168,7 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   a.part = c->jpart;
   a.saved = c->jsaved;
   a.barrier = c->jbar;
+  a.barrier_ticks = c->barrier_ticks;
   a.flags = c->flags;
   a.etas = u;
   a.eta2 = eta2;
@@ -650,11 +655,18 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
     if (!rc) rc = adcgpu_wetdry(c, it);
     if (!rc) rc = adcgpu_momentum(c);
     if (rc) return rc;
+    if ((s & 63) == 63) {
+      int f[F_COUNT];
+      HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
+    }
   }
   int f[F_COUNT];
   HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->run_iters = f[F_ITSUM];
+  if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
   if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
   if (f[F_ERROR] & ERR_CAPACITY) return fail(ADCGPU_E_CAPACITY, "wetting candidate capacity");
   return 0;

@@ -100,6 +100,7 @@ struct JcgArgs {
   double* part;       // [nch*3*4] wave partials
   double* saved;      // ITPACK SAVE state: q1, gamold, rhoold, cmold, cme, stptst
   unsigned* barrier;
+  long long barrier_ticks;  // longest grid-barrier wait, in wall-clock ticks
   int* flags;
   double* etas;       // = u0 (kept for clarity)
   double* eta2;       // optional: ETA2 = NODECODE*(ETAS+0)+ETA1

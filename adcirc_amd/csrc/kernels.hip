@@ -40,26 +40,47 @@ __device__ __forceinline__ double block_tree(double v, double* lds) {
 // Grid barrier for cooperative launches (all blocks co-resident).  The
 // counter increases monotonically during one launch; `target` is this
 // block's running expectation.
-__device__ __forceinline__ void grid_barrier(unsigned* ctr, unsigned nblocks, unsigned& target,
-                                             int* err) {
-  __threadfence();  // every thread releases its own global writes
+// Every wave releases its own global stores with an agent-scope release fence
+// (write-back of the XCD's L2) before the workgroup barrier; thread 0 then
+// arrives on the counter and waits with relaxed loads, and only after the
+// wait does every wave acquire (invalidate) once.  An acquire load per poll
+// would invalidate the XCD's L2 on every poll, and with tens of waiting
+// blocks per XCD that starves the blocks that are still working.
+// The wait is bounded in wall-clock time (`ticks`), and a block that finds the
+// error flag raised by another block leaves at once, so a launch that has
+// gone wrong drains within one bound.  Returns false on time-out or error
+// (a logic error: the caller leaves).
+__device__ __forceinline__ bool grid_barrier(unsigned* ctr, unsigned nblocks, unsigned& target,
+                                             int* err, long long ticks) {
+  __shared__ int timed_out;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
     target += nblocks;
-    atomicAdd(ctr, 1u);
-    // bounded wait: a block that never arrives (a logic error) must not hang
-    // the device; the wait gives up after ~2^28 polls and flags the error
-    unsigned long long polls = 0;
-    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    timed_out = 0;
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long t0 = wall_clock64();
+    unsigned polls = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(2);
-      if (++polls > (1ull << 28)) {
-        atomicOr(err, ERR_BARRIER);
-        break;
+      if ((++polls & 63u) == 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER) {
+          timed_out = 1;
+          break;
+        }
+        if (wall_clock64() - t0 > ticks) {
+          atomicOr(err, ERR_BARRIER);
+          timed_out = 1;
+          break;
+        }
       }
     }
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)
+      timed_out = 1;
   }
   __syncthreads();
-  __threadfence();  // and acquires before reading other blocks' data
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return timed_out == 0;
 }
 
 // ------------------------------------------------------------ time levels
@@ -383,13 +404,20 @@ struct ItLocal {
   double dots[3];
 };
 
+// determ (src/itpackv.F:1029-1068): the backward three-term recurrence with
+// the two running values kept in registers (same operations, same order).
 __device__ double jcg_determ(int n, const double* tri1, const double* tri2, double* wk1,
                              double* wk2, double x) {
-  for (int i = 0; i < n; ++i) wk1[i] = tri1[i] - x;
-  wk2[n - 1] = wk1[n - 1];
-  wk2[n - 2] = wk1[n - 2] * wk2[n - 1] + tri2[n - 1];
-  for (int l = n - 3; l >= 0; --l) wk2[l] = wk1[l] * wk2[l + 1] + tri2[l + 1] * wk2[l + 2];
-  return wk2[0];
+  (void)wk1;
+  (void)wk2;
+  double a2 = tri1[n - 1] - x;                            // wk2(n)
+  double a1 = (tri1[n - 2] - x) * a2 + tri2[n - 1];       // wk2(n-1)
+  for (int l = n - 3; l >= 0; --l) {
+    const double w = (tri1[l] - x) * a1 + tri2[l + 1] * a2;  // wk2(l)
+    a2 = a1;
+    a1 = w;
+  }
+  return a1;
 }
 
 // zbrent (src/itpackv.F:2521-2781) with eps = 0; returns ier, result in *pb.
@@ -582,7 +610,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       }
     }
   }
-  grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR);
+  if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
 
   // ---- scal, part 2: scale off-diagonals; initial pseudo-residual
   // d0 = rhs - N u - u (src/itpackv.F:2394-2400, 346-353)
@@ -611,7 +639,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       }
     }
   }
-  grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR);
+  if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
 
   // ---- iterations
   for (int loop = 0; loop <= a.itmax; ++loop) {
@@ -661,7 +689,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         }
       }
     }
-    grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR);
+    if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
 
     // reduce the chunk partials (every block, identical tree)
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
@@ -701,7 +729,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         }
       }
     }
-    grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR);
+    if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
     if (t == 0) L.in = in + 1;
     __syncthreads();
   }

@@ -103,12 +103,26 @@ def main() -> int:
     log(f"elevation boundary table for {nsteps} steps")
     table = g.eta_bc_table(1, nsteps)
     log(f"warmup {a.warmup} steps")
-    g.run_resident(1, a.warmup, holland=run.holland, table=table[:a.warmup])
+    t0 = time.perf_counter()
+    done = 0
+    while done < a.warmup:
+        k = min(10 if done == 0 else 50, a.warmup - done)
+        g.run_resident(done + 1, k, holland=run.holland, table=table[done:done + k])
+        done += k
+        log(f"  warmup {done}/{a.warmup} steps, {time.perf_counter() - t0:.2f} s")
     g.synchronize()
     g.profile(True)
     log(f"timed {a.steps} steps")
+    # the timed steps are issued in chunks of up to 200 (each chunk ends with a
+    # stream synchronisation inside the library) so that progress is visible
     t0 = time.perf_counter()
-    iters = g.run_resident(a.warmup + 1, a.steps, holland=run.holland, table=table[a.warmup:])
+    iters, done = 0, 0
+    while done < a.steps:
+        k = min(200, a.steps - done)
+        s0 = a.warmup + done
+        iters += g.run_resident(s0 + 1, k, holland=run.holland, table=table[s0:s0 + k])
+        done += k
+        log(f"  {done}/{a.steps} steps, {time.perf_counter() - t0:.2f} s")
     g.synchronize()
     wall = time.perf_counter() - t0
     jcg_ms, launches = g.profile_read()

@@ -68,3 +68,24 @@ def compare(g, o, it, fields=STATE_F, ifields=STATE_I):
 def log(msg):
     print(msg, flush=True)
     sys.stdout.flush()
+
+
+def case_setup(case):
+    """Oracle-style setup dictionary for a generated case, built with the
+    product's own preprocessing (for meshes the reference harness did not run)."""
+    from adcirc_amd import prep
+    from adcirc_amd.model import cold_wetdry_state
+
+    r = case.run
+    t = prep.build_tables(case.mesh, r.nolifa, r.h0, r.anginn)
+    npn = int(t["np"])
+    d = dict(t)
+    d["emo"] = np.asarray(case.emo, np.float64).ravel()
+    d["efa"] = (np.asarray(case.efa, np.float64) * (3.141592653589793 / 180.0)).ravel()
+    d["tau0var"] = np.full(npn, r.tau0)
+    d["evm"] = np.full(npn, r.eslm)
+    d["fric"] = np.full(npn, r.cf)
+    e2, nc, nf = cold_wetdry_state(t, r)
+    d["nodecode0"], d["noff0"], d["eta2_0"] = nc, nf, e2
+    d["prbckgrnd_mh2o"] = 100.0 * 1013.0 / (1000.0 * r.g)
+    return d

@@ -45,3 +45,82 @@ def test_beach_hour_bitexact_and_reference(beach_golden):
     assert np.abs(g.get("eta2") - d[f"s{s}_eta2"]).max() <= 1e-6
     assert np.abs(g.get("uu2") - d[f"s{s}_uu2"]).max() <= 1e-6
     assert np.abs(g.get("vv2") - d[f"s{s}_vv2"]).max() <= 1e-6
+
+
+def test_coastal_mesh_bitexact():
+    """The product's own preprocessing on a generated 20k-node coastal mesh
+    (Delaunay, wet/dry, M2+K1, Holland storm): bit-exact vs oracle."""
+    from gpu_helpers import case_setup
+    case = meshgen.coastal(20000)
+    d = case_setup(case)
+    log(f"case coastal {int(d['np'])} nodes: 120 steps bit-exact vs oracle")
+    _run_bitexact(d, case.run, 120, check_every=10)
+
+
+def test_basin_full_day_resident_vs_reference(basin_golden):
+    """43,200 steps through the resident run (one launch sequence per step,
+    elevation forcing from a precomputed table): final state within the
+    reference's bar, and identical to stepping through the per-step API."""
+    d = basin_golden
+    run = meshgen.basin().run
+    nsteps = int(d["run_nsteps"])
+    log(f"case basin: resident run of {nsteps} steps")
+    g, o = build_pair(d, run)
+    iters = g.run_resident(1, nsteps)
+    s = nsteps
+    de = np.abs(g.get("eta2") - d[f"s{s}_eta2"]).max()
+    du = max(np.abs(g.get("uu2") - d[f"s{s}_uu2"]).max(),
+             np.abs(g.get("vv2") - d[f"s{s}_vv2"]).max())
+    log(f"  max |d eta| {de:.3e} m, max |d u| {du:.3e} m/s, JCG iterations {iters} "
+        f"(reference {int(d['step_numitr'].sum())})")
+    assert de <= 1e-6 and du <= 1e-6
+    assert np.array_equal(g.get("nodecode"), d[f"s{s}_nodecode"])
+    # the first 200 steps again, one API call per step, against the resident run
+    g2, _ = build_pair(d, run)
+    g3, _ = build_pair(d, run)
+    g2.run_resident(1, 200)
+    for it in range(1, 201):
+        g3.timestep(it)
+    for k in ("eta2", "uu2", "vv2", "etas", "tk"):
+        assert np.array_equal(g2.get(k), g3.get(k)), k
+
+
+def test_jcg_dropin_sequence(jcg_golden):
+    """adcgpu_jcg (the drop-in for CALL JCG) on the reference's fixture systems,
+    solved back to back: bit-exact vs the oracle's blocked order; against the
+    reference's own JCG the same iteration count (+-1) and solution to the
+    stopping tolerance."""
+    import oracle_lib
+    from adcirc_amd.model import jcg
+    from test_oracle import _jcg_call
+    g = jcg_golden
+    lib = oracle_lib.load()
+    st = oracle_lib.Itpack()
+    lib.adco_itpack_init(st, oracle_lib.BLOCKED)
+    for name in [str(s) for s in g["names"]]:
+        log(f"jcg system {name}")
+        J, Cm = g[f"{name}_in_jcoef"], g[f"{name}_in_coef"]
+        n, mz = J.shape
+        itmax, zeta = g[f"{name}_prm"]
+        ier_o, ip_o, rp_o, u_o, rhs_o, co_o, jc_o = _jcg_call(
+            lib, st, n, J, Cm, g[f"{name}_in_rhs"], g[f"{name}_in_u"], int(itmax), zeta)
+        jc = np.ascontiguousarray(J.astype(np.int32).T.ravel())
+        co = np.ascontiguousarray(np.asarray(Cm, np.float64).T.ravel())
+        rhs = np.array(g[f"{name}_in_rhs"], np.float64)
+        u = np.array(g[f"{name}_in_u"], np.float64)
+        iparm = np.zeros(12, np.int32)
+        rparm = np.zeros(12)
+        lib.adco_dfault(iparm.ctypes.data_as(oracle_lib._ip), rparm.ctypes.data_as(oracle_lib._dp))
+        iparm[0], iparm[1], iparm[3] = int(itmax), 0, 16
+        rparm[0] = zeta
+        ier = jcg(co, jc, rhs, u, iparm, rparm)
+        assert ier == ier_o
+        assert iparm[0] == ip_o[0], name
+        assert np.array_equal(u, u_o), name
+        assert np.array_equal(rhs, rhs_o), name
+        assert np.array_equal(co.reshape(mz, n).T, co_o), name
+        assert np.array_equal(jc.reshape(mz, n).T, jc_o), name
+        # against the reference JCG
+        assert abs(int(iparm[0]) - int(g[f"{name}_iparm1"])) <= 1, name
+        ur = g[f"{name}_u"]
+        assert np.linalg.norm(u - ur) <= 100 * zeta * max(np.linalg.norm(ur), 1e-300), name
