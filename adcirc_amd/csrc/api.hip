@@ -82,6 +82,7 @@ struct adcgpu_ctx {
   unsigned* jbar = nullptr;
   int jcg_grid = 0;
   long long barrier_ticks = 0;  // 0.5 s of the device wall clock
+  int jcg_cap = 0;              // resident k_jcg blocks on this device
   int64_t run_iters = 0;
   // optional timing of the JCG launches with HIP events on the context stream
   int profile = 0;
@@ -119,12 +120,34 @@ int check_launch(const char* what) {
   return 0;
 }
 
-int setup_jcg_grid(adcgpu_ctx* c) {
+// Blocks of k_jcg that are resident on one CU at once.  The occupancy query
+// counts wave slots per CU, but the dispatcher places the waves of a
+// workgroup on the four SIMDs in turn, so a block of W waves takes ceil(W/4)
+// slots of one SIMD: a 5-wave block with 164 VGPRs (3 waves per SIMD) runs one
+// per CU on gfx950 although the query reports two (measured: the second
+// block on each CU started only when the first ended).  A cooperative grid
+// larger than what is really resident never passes its first grid barrier.
+int jcg_blocks_per_cu(int device, int* per_cu_out, int* cus_out) {
   int per_cu = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jcg, JCG_BLOCK, 0));
+  hipFuncAttributes fa;
+  HIPCHK(hipFuncGetAttributes(&fa, (const void*)k_jcg));
+  const int vgpr = std::max((fa.numRegs + 7) & ~7, 8);  // allocation granule 8
+  const int per_simd = std::min(8, 512 / vgpr);
+  const int waves = (JCG_BLOCK + 63) / 64;
+  per_cu = std::min(per_cu, per_simd / ((waves + 3) / 4));
   hipDeviceProp_t prop;
-  HIPCHK(hipGetDeviceProperties(&prop, c->device));
-  int cap = per_cu * prop.multiProcessorCount;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  *per_cu_out = per_cu;
+  *cus_out = prop.multiProcessorCount;
+  return 0;
+}
+
+int setup_jcg_grid(adcgpu_ctx* c) {
+  int per_cu = 0, cus = 0;
+  if (int rc = jcg_blocks_per_cu(c->device, &per_cu, &cus)) return rc;
+  int cap = per_cu * cus;
+  c->jcg_cap = cap;
   if (cap <= 0) return fail(ADCGPU_E_HIP, "JCG kernel cannot be made resident");
   int need = (c->nch + JCG_MAXK - 1) / JCG_MAXK;
   int grid = std::min(c->nch, cap);
@@ -673,6 +696,25 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
 }
 
 int64_t adcgpu_run_iterations(adcgpu_ctx* c) { return c ? c->run_iters : -1; }
+
+int adcgpu_jcg_shape(adcgpu_ctx* c, int* shape3) {
+  if (!c || !shape3) return fail(ADCGPU_E_ARG, "null argument");
+  shape3[0] = c->jcg_grid;
+  shape3[1] = c->jcg_cap;
+  shape3[2] = c->nch;
+  return 0;
+}
+
+int adcgpu_set_jcg_blocks(adcgpu_ctx* c, int blocks) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  const int need = (c->nch + JCG_MAXK - 1) / JCG_MAXK;
+  if (blocks == 0) blocks = std::min(c->nch, c->jcg_cap);
+  if (blocks < need || blocks > c->jcg_cap || blocks > c->nch)
+    return fail(ADCGPU_E_ARG, "JCG blocks must lie in [" + std::to_string(need) + ", " +
+                                  std::to_string(std::min(c->nch, c->jcg_cap)) + "]");
+  c->jcg_grid = blocks;
+  return 0;
+}
 
 int adcgpu_profile(adcgpu_ctx* c, int enable) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");

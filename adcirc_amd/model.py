@@ -194,6 +194,15 @@ class Model:
         check(self.lib.adcgpu_profile_read(self.h, C.byref(ms), C.byref(n)), "profile_read")
         return ms.value, n.value
 
+    def jcg_shape(self):
+        """(blocks per JCG launch, resident-block limit, 2048-row chunks)."""
+        out = np.zeros(3, np.int32)
+        check(self.lib.adcgpu_jcg_shape(self.h, _p(out, ip_)), "jcg_shape")
+        return tuple(int(v) for v in out)
+
+    def set_jcg_blocks(self, blocks: int) -> None:
+        check(self.lib.adcgpu_set_jcg_blocks(self.h, int(blocks)), "set_jcg_blocks")
+
     def synchronize(self):
         check(self.lib.adcgpu_synchronize(self.h), "synchronize")
 

@@ -98,6 +98,7 @@ def main() -> int:
     eta2, nodecode, noff = cold_wetdry_state(tables, run)
     g = Model(tables, run, emo=case.emo, efa=case.efa * (3.141592653589793 / 180.0))
     g.cold_start(eta2, nodecode, noff)
+    log(f"JCG launch: {g.jcg_shape()} (blocks, resident-block limit, 2048-row chunks)")
     g.holland(0.0, run.holland)
     nsteps = a.warmup + a.steps
     log(f"elevation boundary table for {nsteps} steps")
@@ -126,6 +127,7 @@ def main() -> int:
     g.synchronize()
     wall = time.perf_counter() - t0
     jcg_ms, launches = g.profile_read()
+    shape = g.jcg_shape()
     g.profile(False)
     npn = int(tables["np"])
     Z = int(np.asarray(tables["nneigh"]).sum()) - npn
@@ -154,6 +156,7 @@ def main() -> int:
                         f"plain extrapolated start",
             "nodes": npn, "elements": int(tables["ne"]), "dt": run.dtdp, "convcr": run.convcr,
             "warm_start": "plain",
+            "jcg_blocks": shape[0], "jcg_resident_cap": shape[1], "jcg_chunks": shape[2],
         },
         "timesteps_per_s": a.steps / wall,
         "jcg_iterations_per_step": iters / a.steps,
