@@ -27,7 +27,7 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_set_forcing", "adcgpu_solve_gwce", "adcgpu_wetdry", "adcgpu_momentum",
            "adcgpu_timestep", "adcgpu_last_numitr", "adcgpu_synchronize", "adcgpu_run",
            "adcgpu_run_iterations", "adcgpu_elevation_bc", "adcgpu_jcg", "adcgpu_holland",
-           "adcgpu_profile", "adcgpu_profile_read", "adcgpu_jcg_shape", "adcgpu_set_jcg_blocks")
+           "adcgpu_profile", "adcgpu_profile_read", "adcgpu_jcg_shape")
 
 
 class Mesh(C.Structure):
@@ -86,7 +86,6 @@ def load() -> C.CDLL:
     lib.adcgpu_profile.argtypes = [vp, C.c_int]
     lib.adcgpu_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.adcgpu_jcg_shape.argtypes = [vp, ip_]
-    lib.adcgpu_set_jcg_blocks.argtypes = [vp, C.c_int]
     _lib = lib
     return lib
 

@@ -8,7 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libadcgpu.so")
-SOURCES = ["kernels.hip", "api.hip"]
+DRIVER = os.path.join(HERE, "adcgpu_run_case")
+SOURCES = ["kernels.hip", "jcg.hip", "api.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
@@ -41,6 +42,12 @@ def build(verbose: bool = False) -> str:
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the native driver (C ABI only), linked against the library beside it
+    cmd = [hipcc(), *HIPCC_FLAGS, os.path.join(CSRC, "run_case.hip"), "-L", HERE,
+           "-ladcgpu", "-Wl,-rpath,$ORIGIN", "-o", DRIVER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     return LIB
 
 

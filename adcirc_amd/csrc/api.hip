@@ -77,7 +77,7 @@ struct adcgpu_ctx {
   int cand_cap = 0;
   int* flags = nullptr;
   // JCG work
-  double *cs = nullptr, *ju1 = nullptr, *jd0 = nullptr, *jd1 = nullptr, *jw = nullptr;
+  double *cs = nullptr, *jd = nullptr, *jw = nullptr;
   double *jpart = nullptr, *jsaved = nullptr;
   unsigned* jbar = nullptr;
   int jcg_grid = 0;
@@ -123,10 +123,10 @@ int check_launch(const char* what) {
 // Blocks of k_jcg that are resident on one CU at once.  The occupancy query
 // counts wave slots per CU, but the dispatcher places the waves of a
 // workgroup on the four SIMDs in turn, so a block of W waves takes ceil(W/4)
-// slots of one SIMD: a 5-wave block with 164 VGPRs (3 waves per SIMD) runs one
-// per CU on gfx950 although the query reports two (measured: the second
-// block on each CU started only when the first ended).  A cooperative grid
-// larger than what is really resident never passes its first grid barrier.
+// slots of one SIMD (measured on gfx950: 5-wave blocks with 164 VGPRs run one
+// per CU although the query reports two; 4- and 8-wave blocks run as many as
+// it reports).  A cooperative grid larger than what is really resident never
+// passes its first grid barrier, so the smaller of the two counts is used.
 int jcg_blocks_per_cu(int device, int* per_cu_out, int* cus_out) {
   int per_cu = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jcg, JCG_BLOCK, 0));
@@ -143,23 +143,27 @@ int jcg_blocks_per_cu(int device, int* per_cu_out, int* cus_out) {
   return 0;
 }
 
+// One block per 2048-row chunk plus the eigenvalue block, all resident.
+int set_jcg_grid(adcgpu_ctx* c) {
+  const int grid = c->nch + 1;
+  if (grid > c->jcg_cap)
+    return fail(ADCGPU_E_CAPACITY, "system too large for the register-resident JCG (" +
+                                       std::to_string(c->nch) + " chunks of 2048 rows, " +
+                                       std::to_string(c->jcg_cap) + " resident blocks)");
+  c->jcg_grid = grid;
+  return 0;
+}
+
 int setup_jcg_grid(adcgpu_ctx* c) {
   int per_cu = 0, cus = 0;
   if (int rc = jcg_blocks_per_cu(c->device, &per_cu, &cus)) return rc;
-  int cap = per_cu * cus;
+  const int bar_cap = (JCG_BAR_WORDS / JCG_BAR_STRIDE - 2) * JCG_BAR_GROUP;
+  const int cap = std::min(per_cu * cus, bar_cap);
   c->jcg_cap = cap;
-  if (cap <= 0) return fail(ADCGPU_E_HIP, "JCG kernel cannot be made resident");
-  int need = (c->nch + JCG_MAXK - 1) / JCG_MAXK;
-  int grid = std::min(c->nch, cap);
-  if (grid < need)
-    return fail(ADCGPU_E_CAPACITY, "mesh too large for the register-resident JCG (" +
-                                       std::to_string(c->nch) + " chunks, " +
-                                       std::to_string(cap) + " resident blocks)");
-  c->jcg_grid = std::max(grid, 1);
   int khz = 0;
   HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
   c->barrier_ticks = (long long)std::max(khz, 1000) * 500;
-  return 0;
+  return set_jcg_grid(c);
 }
 
 int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
@@ -183,21 +187,20 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   a.col = col;
   a.nneigh = nneigh;
   a.rhs = rhs;
-  a.u0 = u;
-  a.u1 = c->ju1;
-  a.d0 = c->jd0;
-  a.d1 = c->jd1;
+  a.u = u;
+  a.d = c->jd;
   a.w = c->jw;
   a.part = c->jpart;
   a.saved = c->jsaved;
   a.barrier = c->jbar;
   a.barrier_ticks = c->barrier_ticks;
   a.flags = c->flags;
-  a.etas = u;
   a.eta2 = eta2;
   a.eta1 = eta1;
   a.nodecode = nodecode;
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->jbar, 0u, 1);
+  const int bar_words = ((c->jcg_grid + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE;
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(bar_words, 256)), dim3(256), 0, c->stream, c->jbar, 0u,
+                     bar_words);
   void* args[] = {&a};
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profile) {
@@ -365,7 +368,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   double** nodal[] = {&c->eta1, &c->eta2, &c->etas, &c->uu1, &c->vv1, &c->uu2, &c->vv2,
                       &c->qx1, &c->qy1, &c->qx2, &c->qy2, &c->h1, &c->h2, &c->tk,
                       &c->etamax, &c->totalarea, &c->wsx1, &c->wsy1, &c->wsx2, &c->wsy2,
-                      &c->pr1, &c->pr2, &c->gwce_lv, &c->ju1, &c->jd0, &c->jd1, &c->jw};
+                      &c->pr1, &c->pr2, &c->gwce_lv, &c->jd, &c->jw};
   for (auto p : nodal) A(p, np);
   A(&c->alphal, ne); A(&c->coef, (size_t)mnei * np); A(&c->cs, (size_t)mnei * np);
   A(&c->obc, (size_t)std::max(neta, 1) * mnei);
@@ -375,7 +378,8 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
   c->cand_cap = WD_SORT_CAP;
   A(&c->cand, c->cand_cap);
-  A(&c->flags, F_COUNT); A(&c->jpart, (size_t)c->nch * 12); A(&c->jsaved, 8); A(&c->jbar, 1);
+  A(&c->flags, F_COUNT); A(&c->jpart, (size_t)c->nch * 4); A(&c->jsaved, 8);
+  A(&c->jbar, JCG_BAR_WORDS);
   if (!rc) {
     hipError_t e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
@@ -705,17 +709,6 @@ int adcgpu_jcg_shape(adcgpu_ctx* c, int* shape3) {
   return 0;
 }
 
-int adcgpu_set_jcg_blocks(adcgpu_ctx* c, int blocks) {
-  if (!c) return fail(ADCGPU_E_ARG, "null context");
-  const int need = (c->nch + JCG_MAXK - 1) / JCG_MAXK;
-  if (blocks == 0) blocks = std::min(c->nch, c->jcg_cap);
-  if (blocks < need || blocks > c->jcg_cap || blocks > c->nch)
-    return fail(ADCGPU_E_ARG, "JCG blocks must lie in [" + std::to_string(need) + ", " +
-                                  std::to_string(std::min(c->nch, c->jcg_cap)) + "]");
-  c->jcg_grid = blocks;
-  return 0;
-}
-
 int adcgpu_profile(adcgpu_ctx* c, int enable) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -830,9 +823,9 @@ int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double*
       if (!rc) c->allocs.push_back((void*)*p);
     };
     A(&c->coef, (size_t)width * n); A(&c->cs, (size_t)width * n); A(&c->col, (size_t)width * n);
-    A(&c->nneigh, n); A(&c->gwce_lv, n); A(&c->etas, n); A(&c->ju1, n); A(&c->jd0, n);
-    A(&c->jd1, n); A(&c->jw, n); A(&c->jpart, (size_t)c->nch * 12); A(&c->jsaved, 8);
-    A(&c->jbar, 1); A(&c->flags, F_COUNT);
+    A(&c->nneigh, n); A(&c->gwce_lv, n); A(&c->etas, n); A(&c->jd, n); A(&c->jw, n);
+    A(&c->jpart, (size_t)c->nch * 4); A(&c->jsaved, 8); A(&c->jbar, JCG_BAR_WORDS);
+    A(&c->flags, F_COUNT);
     if (!rc) HIPCHK(hipStreamCreate(&c->stream));
     if (!rc) rc = setup_jcg_grid(c);
     if (rc) {
@@ -844,8 +837,9 @@ int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double*
     jc_n[dev] = n;
     jc_w[dev] = width;
   }
+  c->np = n;
   c->nch = (n + RED_CHUNK - 1) / RED_CHUNK;
-  if (c->jcg_grid * JCG_MAXK < c->nch) return fail(ADCGPU_E_CAPACITY, "n too large");
+  if (int rc = set_jcg_grid(c)) return rc;
   HIPCHK(hipMemcpyAsync(c->coef, hc.data(), hc.size() * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->col, hcol.data(), hcol.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->nneigh, nn.data(), n * 4, hipMemcpyHostToDevice, c->stream));

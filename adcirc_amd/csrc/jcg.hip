@@ -1,0 +1,498 @@
+// ITPACK jcg (src/itpackv.F:63-456) for one persistent cooperative launch.
+//
+// Launch: one 256-thread block per 2048-row chunk plus one eigenvalue block.
+// Row thread t of chunk c owns rows c*2048 + t + 256*r (r = 0..7) and keeps
+// their iterates (d, d_old, u, u_old) and the product N d in registers for
+// the whole solve; only d is published to memory, for the neighbours'
+// gathers.  The eigenvalue block runs chgcon/zbrent (the adaptive estimate of
+// the largest eigenvalue, a serial root search) while the other blocks form
+// the matrix-vector product, and publishes CME with the chunk partials.
+// After each grid barrier every block reduces the chunk partials with the
+// same fixed tree and evaluates the scalar recurrences (pstop, parcon)
+// redundantly, so all blocks take identical decisions and leave together.
+// Two grid barriers per iteration: after the product (partials and CME
+// complete) and after the update (d complete).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+#include "reduce.h"
+
+namespace adcgpu {
+
+// Grid barrier for cooperative launches whose blocks are all resident.
+// Arrival is counted in two levels, so that no counter sees more than a few
+// tens of atomic adds: blocks add to the counter of their group of
+// JCG_BAR_GROUP, the last arrival of a group adds to the top counter, and the
+// last arrival there publishes the barrier's generation number, which every
+// block polls with relaxed loads.
+// Memory ordering: the workgroup barrier orders every wave's stores before
+// thread 0's agent-scope release fence (write-back of the XCD's L2), and
+// thread 0's agent-scope acquire fence (invalidate) after the wait before
+// every wave's later loads; the group- and top-level last arrivals pass the
+// release on with acquire-release fences (fences synchronise through the
+// release sequence of the counters' atomic adds).  The L2 write-back and
+// invalidate are serialised per XCD, so they are issued once per block, not
+// once per wave (measured on MI355X with 512 blocks: 15 us per barrier
+// against 41 us), and never per poll.
+// The wait is bounded in wall-clock time and a block that sees the error
+// flag leaves at once, so a launch that went wrong drains quickly.  Returns
+// false on time-out or error (a logic error; the caller leaves the kernel).
+__device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& gen, int* err,
+                                          long long ticks) {
+  __shared__ int failed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    failed = 0;
+    ++gen;
+    const unsigned nb = gridDim.x;
+    const unsigned ng = (nb + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP;
+    const unsigned g = blockIdx.x / JCG_BAR_GROUP;
+    const unsigned gcount = (g == ng - 1) ? nb - g * JCG_BAR_GROUP : JCG_BAR_GROUP;
+    unsigned* genw = bar + JCG_BAR_STRIDE * (1 + ng);
+    const unsigned o1 = __hip_atomic_fetch_add(bar + JCG_BAR_STRIDE * (1 + g), 1u,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (o1 == gen * gcount - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      const unsigned o2 =
+          __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (o2 == gen * ng - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        __hip_atomic_store(genw, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    const long long t0 = wall_clock64();
+    unsigned polls = 0;
+    while (__hip_atomic_load(genw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
+      __builtin_amdgcn_s_sleep(1);
+      if ((++polls & 63u) == 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER) {
+          failed = 1;
+          break;
+        }
+        if (wall_clock64() - t0 > ticks) {
+          atomicOr(err, ERR_BARRIER);
+          failed = 1;
+          break;
+        }
+      }
+    }
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)
+      failed = 1;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return failed == 0;
+}
+
+// Element access through a 32-bit byte offset from a uniform base: the
+// addresses then take the scalar-base + 32-bit-offset form of the vector
+// memory instructions instead of one 64-bit VGPR pair per row.  Arrays here
+// stay below 4 GiB (checked on the host).
+template <class T>
+__device__ __forceinline__ T& at(T* base, unsigned i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ const T& at(const T* base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+
+struct ItLocal {
+  int in, adapt, halt, q1, maxfn, nsig, itmax, stop;
+  double zeta, cme, gamma, rho, delnnm, delsnm, udnm, stptst, gamold, rhoold, cmold;
+  double c1, c2, c3, c4, tz;
+};
+
+// determ (src/itpackv.F:1029-1068): the backward three-term recurrence with
+// the two running values kept in registers (same operations, same order).
+__device__ double jcg_determ(int n, const double* tri1, const double* tri2, double x) {
+  double a2 = tri1[n - 1] - x;                            // wk2(n)
+  double a1 = (tri1[n - 2] - x) * a2 + tri2[n - 1];       // wk2(n-1)
+  for (int l = n - 3; l >= 0; --l) {
+    const double w = (tri1[l] - x) * a1 + tri2[l + 1] * a2;  // wk2(l)
+    a2 = a1;
+    a1 = w;
+  }
+  return a1;
+}
+
+// zbrent (src/itpackv.F:2521-2781) with eps = 0; returns ier, result in *pb.
+__device__ int jcg_zbrent(int n, const double* tri1, const double* tri2, double t, int maxfn,
+                          double a, double* pb) {
+  double b = 1.0, c = 0, d = 0, e = 0, p, qq, r, ss, fa, fb, fc = 0, tol, rm, rone, temp;
+  int ic = 2;
+  const double eps = 0.0;
+  fa = jcg_determ(n, tri1, tri2, a);
+  fb = jcg_determ(n, tri1, tri2, b);
+  if ((fa > 0.0 && fb > 0.0) || (fa < 0.0 && fb < 0.0)) {
+    *pb = b;
+    return 502;
+  }
+  bool reset = true;
+  while (true) {
+    if (reset) {
+      c = a; fc = fa; d = b - c; e = d;
+    }
+    if (!(fabs(fc) >= fabs(fb))) {
+      a = b; b = c; c = a; fa = fb; fb = fc; fc = fa;
+    }
+    tol = t * fmax(fabs(b), 0.1);
+    rm = (c - b) * 0.5;
+    if (fabs(fb) <= eps || fabs(c - b) <= tol) { *pb = b; return 0; }
+    if (ic >= maxfn) { *pb = b; return 501; }
+    bool bisect = (fabs(e) < tol) || (fabs(fa) <= fabs(fb));
+    if (!bisect) {
+      ss = fb / fa;
+      if (a == c) {
+        p = (c - b) * ss;
+        qq = 1.0 - ss;
+      } else {
+        qq = fa / fc;
+        r = fb / fc;
+        rone = r - 1.0;
+        p = ss * ((c - b) * qq * (qq - r) - (b - a) * rone);
+        qq = (qq - 1.0) * rone * (ss - 1.0);
+      }
+      if (p > 0.0) qq = -qq;
+      if (p < 0.0) p = -p;
+      ss = e;
+      e = d;
+      if (p + p >= 3.0 * rm * qq || p + p >= fabs(ss * qq)) {
+        bisect = true;
+      } else {
+        d = p / qq;
+      }
+    }
+    if (bisect) { e = rm; d = e; }
+    a = b;
+    fa = fb;
+    temp = d;
+    if (fabs(temp) <= 0.5 * tol) temp = copysign(0.5 * tol, rm);
+    b = b + temp;
+    fb = jcg_determ(n, tri1, tri2, b);
+    ic = ic + 1;
+    if (fb == 0.0 || fc == 0.0) { reset = false; continue; }
+    if ((fb > 0.0 && fc < 0.0) || (fb < 0.0 && fc > 0.0)) { reset = false; continue; }
+    reset = true;
+  }
+}
+
+// chgcon(ibmth=1) (src/itpackv.F:893-1027); tri1/tri2 are the two columns.
+__device__ void jcg_chgcon(ItLocal& L, double* tri1, double* tri2) {
+  const int ip = L.in;
+  double start = L.cme, end1;
+  if (ip == 0) {
+    end1 = 0.0;
+    L.cmold = 0.0;
+  } else if (ip == 1) {
+    end1 = 1.0 - 1.0 / L.gamma;
+    tri1[0] = end1;
+    tri2[0] = 0.0;
+  } else {
+    if (fabs(start - L.cmold) <= L.zeta * start) {
+      L.adapt = 0;
+      return;
+    }
+    L.cmold = start;
+    tri1[ip - 1] = 1.0 - 1.0 / L.gamma;
+    tri2[ip - 1] = (1.0 - L.rho) / (L.rho * L.rhoold * L.gamma * L.gamold);
+    double b;
+    int ier = jcg_zbrent(ip, tri1, tri2, L.tz, L.maxfn, start, &b);
+    if (ier != 0) {
+      L.adapt = 0;
+      return;
+    }
+    end1 = b;
+  }
+  L.cme = end1;
+}
+
+// pstop_nrms(iflag=1) (src/itpackv.F:809-890)
+__device__ void jcg_pstop(ItLocal& L, double unrm, double dnrm) {
+  const double con = L.cme;
+  double uold, tr, tl;
+  L.halt = 0;
+  if (L.in < 1) {
+    L.q1 = 0;
+    L.udnm = 1.0;
+    L.stptst = 1000.0;
+  }
+  if (!(L.q1 || (L.in > 5 && (L.in % 5) != 0))) {
+    uold = L.udnm;
+    L.udnm = unrm;
+    if (L.udnm == 0.0) L.udnm = 1.0;
+    if (L.in > 5 && fabs(L.udnm - uold) <= L.udnm * L.zeta) L.q1 = 1;
+  }
+  tr = __builtin_sqrt(L.udnm);
+  tl = 1.0;
+  if (con != 1.0) {
+    tl = __builtin_sqrt(dnrm);
+    tr = tr * (1.0 - con);
+  }
+  L.stptst = tl / tr;
+  if (tl >= tr * L.zeta) return;
+  L.halt = 1;
+}
+
+// parcon(ibmth=1) (src/itpackv.F:1774-1851)
+__device__ void jcg_parcon(ItLocal& L, double dtnrm) {
+  const int ip = L.in;
+  const double rhoold = L.rho;
+  L.gamold = L.gamma;
+  L.gamma = 1.0 / (1.0 - dtnrm / L.delnnm);
+  L.rho = 1.0;
+  if (ip != 0) L.rho = 1.0 / (1.0 - L.gamma * L.delnnm / (L.gamold * L.delsnm * rhoold));
+  L.delsnm = L.delnnm;
+  L.rhoold = rhoold;
+  L.c1 = L.rho * L.gamma;
+  L.c2 = L.rho;
+  L.c3 = 1.0 - L.rho;
+  L.c4 = L.rho * (1.0 - L.gamma);
+}
+
+
+__global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
+  __shared__ ItLocal L;
+  __shared__ double lds[4];
+  __shared__ double tri1[JCG_MAX_ITMAX], tri2[JCG_MAX_ITMAX];
+  const int np = a.n;
+  const int t = threadIdx.x;
+  const int nch = (np + RED_CHUNK - 1) / RED_CHUNK;
+  const int c = blockIdx.x;
+  const bool eig_block = c == nch;  // the last block: no rows
+  const int base = c * RED_CHUNK + t;
+  unsigned gen = 0;
+  int* err = a.flags + F_ERROR;
+
+  if (t == 0) {
+    L.in = 0;
+    L.adapt = 1;
+    L.halt = 0;
+    L.q1 = a.saved[0] != 0.0;
+    L.gamold = a.saved[1];
+    L.rhoold = a.saved[2];
+    L.cmold = a.saved[3];
+    L.itmax = a.itmax;
+    L.zeta = a.zeta;
+    L.cme = 0.0;
+    L.gamma = 0.0;
+    L.rho = 0.0;
+    L.delnnm = 0.0;
+    L.delsnm = 0.0;
+    L.udnm = 0.0;
+    L.stptst = 0.0;
+    L.maxfn = a.maxfn;
+    L.tz = a.tz;
+    L.stop = 0;
+  }
+  __syncthreads();
+
+  // per-row iterates, register resident
+  double dc[RED_PER_THREAD], dp[RED_PER_THREAD], uc[RED_PER_THREAD], up[RED_PER_THREAD];
+  double nd[RED_PER_THREAD];
+  // neighbour counts of the 8 rows, one byte each (MNEI <= 255)
+  unsigned nnp[2] = {0x01010101u, 0x01010101u};
+#define NN(r) ((int)((nnp[(r) >> 2] >> (8 * ((r) & 3))) & 255u))
+
+  // ---- scal, part 1: diagonal scaling (src/itpackv.F:2374-2393)
+  if (!eig_block) {
+#pragma unroll
+    for (int r = 0; r < RED_PER_THREAD; ++r) {
+      const int i = base + RED_THREADS * r;
+      uc[r] = 0.0;
+      if (i < np) {
+        const double dg = at(a.coef, i);
+        if (!(dg > 0.0)) atomicOr(err, ERR_DIAG);
+        const double sq = __builtin_sqrt(dg);
+        const double u0 = at(a.u, i) * sq;
+        at(a.u, i) = u0;
+        uc[r] = u0;
+        const double w = 1.0 / sq;
+        at(a.w, i) = w;
+        at(a.rhs, i) = at(a.rhs, i) * w;
+        const unsigned sh = 8 * (r & 3);
+        nnp[r >> 2] = (nnp[r >> 2] & ~(255u << sh)) | ((unsigned)at(a.nneigh, i) << sh);
+      }
+    }
+  }
+  if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+
+  // ---- scal, part 2: scale off-diagonals; initial pseudo-residual
+  // d0 = rhs - N u - u (src/itpackv.F:2394-2400, 346-353)
+  if (!eig_block) {
+    double wi[RED_PER_THREAD];
+#pragma unroll
+    for (int r = 0; r < RED_PER_THREAD; ++r) {
+      const int i = base + RED_THREADS * r;
+      wi[r] = 0.0;
+      dc[r] = 0.0;
+      if (i < np) {
+        wi[r] = at(a.w, i);
+        dc[r] = at(a.rhs, i);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < RED_PER_THREAD; h += 4) {
+      int jm = 1;
+#pragma unroll
+      for (int r = h; r < h + 4; ++r) jm = NN(r) > jm ? NN(r) : jm;
+      for (int j = 1; j < jm; ++j) {
+#pragma unroll
+        for (int r = h; r < h + 4; ++r) {
+          const int i = base + RED_THREADS * r;
+          if (j < NN(r)) {
+            const unsigned q = (unsigned)(j * np + i);
+            const unsigned col = (unsigned)at(a.col, q);
+            const double cs = at(a.coef, q) * wi[r] * at(a.w, col);
+            at(a.cs, q) = cs;
+            dc[r] = dc[r] - cs * at(a.u, col);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RED_PER_THREAD; ++r) {
+      const int i = base + RED_THREADS * r;
+      dc[r] = dc[r] - uc[r];
+      dp[r] = 0.0;
+      up[r] = 0.0;
+      if (i < np) at(a.d, i) = dc[r];
+    }
+  }
+  if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+
+  // ---- iterations
+  for (int in = 0; in <= a.itmax; ++in) {
+    const bool need_u = !(L.q1 || (in > 5 && (in % 5) != 0));
+    if (eig_block) {
+      // adaptive eigenvalue estimate for this iteration
+      if (t == 0) {
+        L.in = in;
+        if (L.adapt) jcg_chgcon(L, tri1, tri2);
+        a.saved[6] = L.cme;
+        a.saved[7] = L.cmold;
+      }
+    } else {
+      // N d, rows in two groups of four (bounded register pressure); per row
+      // the neighbours are taken in slot order
+#pragma unroll
+      for (int h = 0; h < RED_PER_THREAD; h += 4) {
+        int jm = 1;
+#pragma unroll
+        for (int r = h; r < h + 4; ++r) {
+          nd[r] = 0.0;
+          jm = NN(r) > jm ? NN(r) : jm;
+        }
+        for (int j = 1; j < jm; ++j) {
+          const unsigned q0 = (unsigned)(j * np + base);
+#pragma unroll
+          for (int r = h; r < h + 4; ++r) {
+            const unsigned q = q0 + RED_THREADS * r;
+            if (j < NN(r)) nd[r] = nd[r] - at(a.cs, q) * at(a.d, (unsigned)at(a.col, q));
+          }
+        }
+      }
+      double add = 0.0, adt = 0.0, auu = 0.0;
+#pragma unroll
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        if (base + RED_THREADS * r < np) {
+          add = add + dc[r] * dc[r];
+          adt = adt + dc[r] * nd[r];
+          if (need_u) auu = auu + uc[r] * uc[r];
+        }
+      }
+      const double s0 = wave_sum(add), s1 = wave_sum(adt), s2 = wave_sum(auu);
+      __shared__ double wsum[12];
+      if ((t & 63) == 0) {
+        wsum[0 + (t >> 6)] = s0;
+        wsum[4 + (t >> 6)] = s1;
+        wsum[8 + (t >> 6)] = s2;
+      }
+      __syncthreads();
+      if (t < 3) {
+        const double* v = wsum + 4 * t;
+        a.part[(size_t)c * 4 + t] = (v[0] + v[1]) + (v[2] + v[3]);
+      }
+    }
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+
+    // reduce the chunk partials (every block, identical tree)
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    for (int cc = t; cc < nch; cc += RED_THREADS) {
+      const double* pc = a.part + (size_t)cc * 4;
+      r0 = r0 + pc[0];
+      r1 = r1 + pc[1];
+      r2 = r2 + pc[2];
+    }
+    const double delnnm = block_tree(r0, lds);
+    const double dtnrm = block_tree(r1, lds);
+    const double unorm_s = block_tree(r2, lds);
+    if (t == 0) {
+      L.in = in;
+      L.cme = a.saved[6];
+      L.cmold = a.saved[7];
+      L.delnnm = delnnm;
+      jcg_pstop(L, need_u ? unorm_s : 1.0, delnnm);
+      if (!L.halt) jcg_parcon(L, dtnrm);
+    }
+    __syncthreads();
+    if (L.halt) break;
+    const double c1 = L.c1, c2 = L.c2, c3 = L.c3, c4 = L.c4;
+    if (!eig_block) {
+#pragma unroll
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        const int i = base + RED_THREADS * r;
+        const double un = c1 * dc[r] + c2 * uc[r] + c3 * up[r];
+        const double dn = c1 * nd[r] + c4 * dc[r] + c3 * dp[r];
+        up[r] = uc[r];
+        uc[r] = un;
+        dp[r] = dc[r];
+        dc[r] = dn;
+        if (i < np) at(a.d, i) = dn;
+      }
+    }
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+  }
+  // the result is the iterate of the last iteration index: u at the halt,
+  // or the one before the final update when ITMAX ran out
+  const bool halted = L.halt != 0;
+
+  // ---- unscal (src/itpackv.F:2424-2479), ETAS and ETA2 (src/gwce.F:2006-2008)
+  if (!eig_block) {
+#pragma unroll
+    for (int r = 0; r < RED_PER_THREAD; ++r) {
+      const int i = base + RED_THREADS * r;
+      if (i < np) {
+        const double wi = at(a.w, i);
+        const double u = (halted ? uc[r] : up[r]) * wi;
+        const double ri = 1.0 / wi;
+        for (int j = 1; j < NN(r); ++j) {
+          const unsigned q = (unsigned)(j * np + i);
+          at(a.coef, q) = at(a.cs, q) * ri * (1.0 / at(a.w, (unsigned)at(a.col, q)));
+        }
+        at(a.coef, i) = ri * ri;
+        at(a.rhs, i) = at(a.rhs, i) * ri;
+        at(a.u, i) = u;
+        if (a.eta2)
+          at(a.eta2, i) = (double)at(a.nodecode, i) * (u + 0.0) + at(a.eta1, i);
+      }
+    }
+  }
+#undef NN
+  if (blockIdx.x == 0 && t == 0) {
+    const int in_final = halted ? L.in : a.itmax;
+    a.flags[F_NUMITR] = in_final;
+    a.flags[F_IER] = halted ? 0 : 13;
+    a.saved[0] = L.q1 ? 1.0 : 0.0;
+    a.saved[1] = L.gamold;
+    a.saved[2] = L.rhoold;
+    a.saved[3] = L.cmold;
+    a.saved[4] = L.cme;
+    a.saved[5] = L.stptst;
+    a.flags[F_ITSUM] += in_final;
+  }
+}
+
+}  // namespace adcgpu

@@ -10,10 +10,12 @@ constexpr int RED_THREADS = 256;
 constexpr int RED_PER_THREAD = 8;
 constexpr int RED_CHUNK = RED_THREADS * RED_PER_THREAD;
 
-// persistent JCG launch: 4 row waves + 1 wave for the eigenvalue estimate
-constexpr int JCG_BLOCK = RED_THREADS + 64;
-constexpr int JCG_MAXK = 2;          // chunks per block kept in registers
-constexpr int JCG_MAX_ITMAX = 256;   // tridiagonal history length (itmax)
+// persistent JCG launch: one block per chunk plus one eigenvalue block
+constexpr int JCG_BLOCK = RED_THREADS;  // one 2048-row chunk per block
+constexpr int JCG_BAR_GROUP = 16;        // blocks per first-level barrier counter
+constexpr int JCG_BAR_STRIDE = 64;       // words between barrier counters (256 B)
+constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BAR_STRIDE
+constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
 constexpr int WD_SORT_CAP = 4096;    // wetting candidates per step
 
 // device flag slots
@@ -88,21 +90,19 @@ struct JcgArgs {
   int n, itmax, maxfn;
   double zeta, tz;
   double* coef;       // [mnei*n] unscaled in/out (diag in slot 0)
-  double* cs;         // [mnei*n] scaled work copy
+  double* cs;         // [mnei*n] scaled off-diagonals
   const int* col;
   const int* nneigh;
-  double* rhs;
-  double* u0;         // initial guess in, ETAS out
-  double* u1;
-  double* d0;
-  double* d1;
-  double* w;
-  double* part;       // [nch*3*4] wave partials
-  double* saved;      // ITPACK SAVE state: q1, gamold, rhoold, cmold, cme, stptst
-  unsigned* barrier;
+  double* rhs;        // scaled in place, restored on exit
+  double* u;          // initial guess in (scaled in place), ETAS out
+  double* d;          // pseudo-residual of the current iteration
+  double* w;          // 1/sqrt(diag)
+  double* part;       // [nch*4] chunk partials (d.d, d.Nd, u.u)
+  double* saved;      // [8] ITPACK SAVE state q1, gamold, rhoold, cmold, cme,
+                      // stptst; [6..7] = cme, cmold of the running iteration
+  unsigned* barrier;  // [JCG_BAR_WORDS] grid barrier counters
   long long barrier_ticks;  // longest grid-barrier wait, in wall-clock ticks
   int* flags;
-  double* etas;       // = u0 (kept for clarity)
   double* eta2;       // optional: ETA2 = NODECODE*(ETAS+0)+ETA1
   const double* eta1;
   const int* nodecode;

@@ -13,75 +13,9 @@
 #include <stdint.h>
 
 #include "kernels.h"
+#include "reduce.h"
 
 namespace adcgpu {
-
-// ------------------------------------------------------------------ helpers
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Reduces one value per thread of the first 256 threads of the block with the
-// fixed tree; result valid in thread 0.  `lds` holds 4 doubles.
-__device__ __forceinline__ double block_tree(double v, double* lds) {
-  const int t = threadIdx.x;
-  double w = 0.0;
-  if (t < 256) w = wave_sum(v);
-  if (t < 256 && (t & 63) == 0) lds[t >> 6] = w;
-  __syncthreads();
-  double r = 0.0;
-  if (t == 0) r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-  __syncthreads();
-  return r;
-}
-
-// Grid barrier for cooperative launches (all blocks co-resident).  The
-// counter increases monotonically during one launch; `target` is this
-// block's running expectation.
-// Every wave releases its own global stores with an agent-scope release fence
-// (write-back of the XCD's L2) before the workgroup barrier; thread 0 then
-// arrives on the counter and waits with relaxed loads, and only after the
-// wait does every wave acquire (invalidate) once.  An acquire load per poll
-// would invalidate the XCD's L2 on every poll, and with tens of waiting
-// blocks per XCD that starves the blocks that are still working.
-// The wait is bounded in wall-clock time (`ticks`), and a block that finds the
-// error flag raised by another block leaves at once, so a launch that has
-// gone wrong drains within one bound.  Returns false on time-out or error
-// (a logic error: the caller leaves).
-__device__ __forceinline__ bool grid_barrier(unsigned* ctr, unsigned nblocks, unsigned& target,
-                                             int* err, long long ticks) {
-  __shared__ int timed_out;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    target += nblocks;
-    timed_out = 0;
-    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const long long t0 = wall_clock64();
-    unsigned polls = 0;
-    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if ((++polls & 63u) == 0) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER) {
-          timed_out = 1;
-          break;
-        }
-        if (wall_clock64() - t0 > ticks) {
-          atomicOr(err, ERR_BARRIER);
-          timed_out = 1;
-          break;
-        }
-      }
-    }
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)
-      timed_out = 1;
-  }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return timed_out == 0;
-}
 
 // ------------------------------------------------------------ time levels
 __global__ void k_init_h(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
@@ -387,390 +321,6 @@ __global__ void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ con
   }
   s.gwce_lv[i] = acc;
   s.eta2[i] = 0.0;
-}
-
-// ----------------------------------------------------------------- JCG
-// ITPACK jcg restated for one persistent cooperative launch (src/itpackv.F:
-// 63-456).  Block layout: RED_THREADS (256) row threads in 4 waves plus one
-// extra wave whose first lane runs chgcon/zbrent (the adaptive eigenvalue
-// estimate) concurrently with the matrix-vector product.  Every block
-// evaluates the scalar recurrences redundantly from identical data, so all
-// blocks take identical decisions and leave the loop together.
-
-struct ItLocal {
-  int in, adapt, halt, q1, maxfn, nsig, itmax, stop;
-  double zeta, cme, gamma, rho, delnnm, delsnm, udnm, stptst, gamold, rhoold, cmold;
-  double c1, c2, c3, c4, tz;
-  double dots[3];
-};
-
-// determ (src/itpackv.F:1029-1068): the backward three-term recurrence with
-// the two running values kept in registers (same operations, same order).
-__device__ double jcg_determ(int n, const double* tri1, const double* tri2, double* wk1,
-                             double* wk2, double x) {
-  (void)wk1;
-  (void)wk2;
-  double a2 = tri1[n - 1] - x;                            // wk2(n)
-  double a1 = (tri1[n - 2] - x) * a2 + tri2[n - 1];       // wk2(n-1)
-  for (int l = n - 3; l >= 0; --l) {
-    const double w = (tri1[l] - x) * a1 + tri2[l + 1] * a2;  // wk2(l)
-    a2 = a1;
-    a1 = w;
-  }
-  return a1;
-}
-
-// zbrent (src/itpackv.F:2521-2781) with eps = 0; returns ier, result in *pb.
-__device__ int jcg_zbrent(int n, const double* tri1, const double* tri2, double* wk1, double* wk2,
-                          double t, int maxfn, double a, double* pb) {
-  double b = 1.0, c = 0, d = 0, e = 0, p, qq, r, ss, fa, fb, fc = 0, tol, rm, rone, temp;
-  int ic = 2;
-  const double eps = 0.0;
-  fa = jcg_determ(n, tri1, tri2, wk1, wk2, a);
-  fb = jcg_determ(n, tri1, tri2, wk1, wk2, b);
-  if ((fa > 0.0 && fb > 0.0) || (fa < 0.0 && fb < 0.0)) {
-    *pb = b;
-    return 502;
-  }
-  bool reset = true;
-  while (true) {
-    if (reset) {
-      c = a; fc = fa; d = b - c; e = d;
-    }
-    if (!(fabs(fc) >= fabs(fb))) {
-      a = b; b = c; c = a; fa = fb; fb = fc; fc = fa;
-    }
-    tol = t * fmax(fabs(b), 0.1);
-    rm = (c - b) * 0.5;
-    if (fabs(fb) <= eps || fabs(c - b) <= tol) { *pb = b; return 0; }
-    if (ic >= maxfn) { *pb = b; return 501; }
-    bool bisect = (fabs(e) < tol) || (fabs(fa) <= fabs(fb));
-    if (!bisect) {
-      ss = fb / fa;
-      if (a == c) {
-        p = (c - b) * ss;
-        qq = 1.0 - ss;
-      } else {
-        qq = fa / fc;
-        r = fb / fc;
-        rone = r - 1.0;
-        p = ss * ((c - b) * qq * (qq - r) - (b - a) * rone);
-        qq = (qq - 1.0) * rone * (ss - 1.0);
-      }
-      if (p > 0.0) qq = -qq;
-      if (p < 0.0) p = -p;
-      ss = e;
-      e = d;
-      if (p + p >= 3.0 * rm * qq || p + p >= fabs(ss * qq)) {
-        bisect = true;
-      } else {
-        d = p / qq;
-      }
-    }
-    if (bisect) { e = rm; d = e; }
-    a = b;
-    fa = fb;
-    temp = d;
-    if (fabs(temp) <= 0.5 * tol) temp = copysign(0.5 * tol, rm);
-    b = b + temp;
-    fb = jcg_determ(n, tri1, tri2, wk1, wk2, b);
-    ic = ic + 1;
-    if (fb == 0.0 || fc == 0.0) { reset = false; continue; }
-    if ((fb > 0.0 && fc < 0.0) || (fb < 0.0 && fc > 0.0)) { reset = false; continue; }
-    reset = true;
-  }
-}
-
-// chgcon(ibmth=1) (src/itpackv.F:893-1027); tri1/tri2 are the two columns.
-__device__ void jcg_chgcon(ItLocal& L, double* tri1, double* tri2, double* wk1, double* wk2) {
-  const int ip = L.in;
-  double start = L.cme, end1;
-  if (ip == 0) {
-    end1 = 0.0;
-    L.cmold = 0.0;
-  } else if (ip == 1) {
-    end1 = 1.0 - 1.0 / L.gamma;
-    tri1[0] = end1;
-    tri2[0] = 0.0;
-  } else {
-    if (fabs(start - L.cmold) <= L.zeta * start) {
-      L.adapt = 0;
-      return;
-    }
-    L.cmold = start;
-    tri1[ip - 1] = 1.0 - 1.0 / L.gamma;
-    tri2[ip - 1] = (1.0 - L.rho) / (L.rho * L.rhoold * L.gamma * L.gamold);
-    double b;
-    int ier = jcg_zbrent(ip, tri1, tri2, wk1, wk2, L.tz, L.maxfn, start, &b);
-    if (ier != 0) {
-      L.adapt = 0;
-      return;
-    }
-    end1 = b;
-  }
-  L.cme = end1;
-}
-
-// pstop_nrms(iflag=1) (src/itpackv.F:809-890)
-__device__ void jcg_pstop(ItLocal& L, double unrm, double dnrm) {
-  const double con = L.cme;
-  double uold, tr, tl;
-  L.halt = 0;
-  if (L.in < 1) {
-    L.q1 = 0;
-    L.udnm = 1.0;
-    L.stptst = 1000.0;
-  }
-  if (!(L.q1 || (L.in > 5 && (L.in % 5) != 0))) {
-    uold = L.udnm;
-    L.udnm = unrm;
-    if (L.udnm == 0.0) L.udnm = 1.0;
-    if (L.in > 5 && fabs(L.udnm - uold) <= L.udnm * L.zeta) L.q1 = 1;
-  }
-  tr = __builtin_sqrt(L.udnm);
-  tl = 1.0;
-  if (con != 1.0) {
-    tl = __builtin_sqrt(dnrm);
-    tr = tr * (1.0 - con);
-  }
-  L.stptst = tl / tr;
-  if (tl >= tr * L.zeta) return;
-  L.halt = 1;
-}
-
-// parcon(ibmth=1) (src/itpackv.F:1774-1851)
-__device__ void jcg_parcon(ItLocal& L, double dtnrm) {
-  const int ip = L.in;
-  const double rhoold = L.rho;
-  L.gamold = L.gamma;
-  L.gamma = 1.0 / (1.0 - dtnrm / L.delnnm);
-  L.rho = 1.0;
-  if (ip != 0) L.rho = 1.0 / (1.0 - L.gamma * L.delnnm / (L.gamold * L.delsnm * rhoold));
-  L.delsnm = L.delnnm;
-  L.rhoold = rhoold;
-  L.c1 = L.rho * L.gamma;
-  L.c2 = L.rho;
-  L.c3 = 1.0 - L.rho;
-  L.c4 = L.rho * (1.0 - L.gamma);
-}
-
-__global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
-  __shared__ ItLocal L;
-  __shared__ double lds[4];
-  __shared__ double tri1[JCG_MAX_ITMAX], tri2[JCG_MAX_ITMAX], wk1[JCG_MAX_ITMAX],
-      wk2[JCG_MAX_ITMAX];
-  const int np = a.n;
-  const int t = threadIdx.x;
-  const bool row_thread = t < RED_THREADS;
-  const unsigned nblk = gridDim.x;
-  unsigned target = 0;
-  const int nch = (np + RED_CHUNK - 1) / RED_CHUNK;
-
-  if (t == 0) {
-    L.in = 0;
-    L.adapt = 1;
-    L.halt = 0;
-    L.q1 = a.saved[0] != 0.0;
-    L.gamold = a.saved[1];
-    L.rhoold = a.saved[2];
-    L.cmold = a.saved[3];
-    L.itmax = a.itmax;
-    L.zeta = a.zeta;
-    L.cme = 0.0;
-    L.gamma = 0.0;
-    L.rho = 0.0;
-    L.delnnm = 0.0;
-    L.delsnm = 0.0;
-    L.udnm = 0.0;
-    L.stptst = 0.0;
-    L.maxfn = a.maxfn;
-    L.tz = a.tz;
-    L.stop = 0;
-  }
-  __syncthreads();
-
-  // owned rows: chunks c = blockIdx.x + k*gridDim.x, rows c*2048 + t + 256*r
-  double dreg[JCG_MAXK][RED_PER_THREAD];
-
-  // ---- scal, part 1: diagonal scaling (src/itpackv.F:2374-2393)
-  if (row_thread) {
-    for (int k = 0; k < JCG_MAXK; ++k) {
-      const int c = blockIdx.x + k * nblk;
-      if (c >= nch) break;
-      for (int r = 0; r < RED_PER_THREAD; ++r) {
-        const int i = c * RED_CHUNK + t + RED_THREADS * r;
-        if (i >= np) break;
-        const double dg = a.coef[i];
-        if (!(dg > 0.0)) atomicOr(&a.flags[F_ERROR], ERR_DIAG);
-        const double sq = __builtin_sqrt(dg);
-        a.u0[i] = a.u0[i] * sq;
-        const double w = 1.0 / sq;
-        a.w[i] = w;
-        a.rhs[i] = a.rhs[i] * w;
-      }
-    }
-  }
-  if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
-
-  // ---- scal, part 2: scale off-diagonals; initial pseudo-residual
-  // d0 = rhs - N u - u (src/itpackv.F:2394-2400, 346-353)
-  if (row_thread) {
-    for (int k = 0; k < JCG_MAXK; ++k) {
-      const int c = blockIdx.x + k * nblk;
-      if (c >= nch) break;
-      for (int r = 0; r < RED_PER_THREAD; ++r) {
-        const int i = c * RED_CHUNK + t + RED_THREADS * r;
-        if (i >= np) break;
-        const double wi = a.w[i];
-        double d = a.rhs[i];
-        const int nn = a.nneigh[i];
-        for (int j = 1; j < nn; ++j) {
-          const size_t q = (size_t)j * np + i;
-          const int col = a.col[q];
-          const double cs = a.coef[q] * wi * a.w[col];
-          a.cs[q] = cs;
-          d = d - cs * a.u0[col];
-        }
-        d = d - a.u0[i];
-        a.d0[i] = d;
-        a.u1[i] = 0.0;
-        a.d1[i] = 0.0;
-        dreg[k][r] = d;
-      }
-    }
-  }
-  if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
-
-  // ---- iterations
-  for (int loop = 0; loop <= a.itmax; ++loop) {
-    const int in = loop;
-    double* U = (in & 1) ? a.u1 : a.u0;
-    double* U1 = (in & 1) ? a.u0 : a.u1;
-    double* D = (in & 1) ? a.d1 : a.d0;
-    double* D1 = (in & 1) ? a.d0 : a.d1;
-    const bool need_u = !(L.q1 || (in > 5 && (in % 5) != 0));
-    double dtw[JCG_MAXK][RED_PER_THREAD];
-    if (t == RED_THREADS) {
-      // extra wave: adaptive eigenvalue estimate for this iteration
-      L.in = in;
-      if (L.adapt) jcg_chgcon(L, tri1, tri2, wk1, wk2);
-    } else if (row_thread) {
-      for (int k = 0; k < JCG_MAXK; ++k) {
-        const int c = blockIdx.x + k * nblk;
-        if (c >= nch) break;
-        double add = 0.0, adt = 0.0, auu = 0.0;
-        for (int r = 0; r < RED_PER_THREAD; ++r) {
-          const int i = c * RED_CHUNK + t + RED_THREADS * r;
-          if (i >= np) break;
-          double acc = 0.0;
-          const int nn = a.nneigh[i];
-          for (int j = 1; j < nn; ++j) {
-            const size_t q = (size_t)j * np + i;
-            acc = acc - a.cs[q] * D[a.col[q]];
-          }
-          dtw[k][r] = acc;
-          const double d = dreg[k][r];
-          add = add + d * d;
-          adt = adt + d * acc;
-          if (need_u) {
-            const double uu = U[i];
-            auu = auu + uu * uu;
-          }
-        }
-        // wave sums of the chunk; the 4 waves are combined as (w0+w1)+(w2+w3)
-        // when the chunk sums are reduced, without a block barrier here (the
-        // eigenvalue wave is still busy)
-        const double s0 = wave_sum(add), s1 = wave_sum(adt), s2 = wave_sum(auu);
-        if ((t & 63) == 0) {
-          double* pc = a.part + (size_t)c * 12;
-          pc[0 * 4 + (t >> 6)] = s0;
-          pc[1 * 4 + (t >> 6)] = s1;
-          pc[2 * 4 + (t >> 6)] = s2;
-        }
-      }
-    }
-    if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
-
-    // reduce the chunk partials (every block, identical tree)
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-    if (row_thread) {
-      for (int c = t; c < nch; c += RED_THREADS) {
-        const double* pc = a.part + (size_t)c * 12;
-        r0 = r0 + ((pc[0] + pc[1]) + (pc[2] + pc[3]));
-        r1 = r1 + ((pc[4] + pc[5]) + (pc[6] + pc[7]));
-        r2 = r2 + ((pc[8] + pc[9]) + (pc[10] + pc[11]));
-      }
-    }
-    const double delnnm = block_tree(r0, lds);
-    const double dtnrm = block_tree(r1, lds);
-    const double unorm_s = block_tree(r2, lds);
-    if (t == 0) {
-      L.in = in;
-      L.delnnm = delnnm;
-      jcg_pstop(L, need_u ? unorm_s : 1.0, delnnm);
-      if (!L.halt) jcg_parcon(L, dtnrm);
-    }
-    __syncthreads();
-    if (L.halt) break;
-    const double c1 = L.c1, c2 = L.c2, c3 = L.c3, c4 = L.c4;
-    if (row_thread) {
-      for (int k = 0; k < JCG_MAXK; ++k) {
-        const int c = blockIdx.x + k * nblk;
-        if (c >= nch) break;
-        for (int r = 0; r < RED_PER_THREAD; ++r) {
-          const int i = c * RED_CHUNK + t + RED_THREADS * r;
-          if (i >= np) break;
-          const double d = dreg[k][r];
-          const double un = c1 * d + c2 * U[i] + c3 * U1[i];
-          const double dn = c1 * dtw[k][r] + c4 * d + c3 * D1[i];
-          U1[i] = un;
-          D1[i] = dn;
-          dreg[k][r] = dn;
-        }
-      }
-    }
-    if (!grid_barrier(a.barrier, nblk, target, a.flags + F_ERROR, a.barrier_ticks)) return;
-    if (t == 0) L.in = in + 1;
-    __syncthreads();
-  }
-  // L.in is the iteration at exit (halt at in, or itmax after the last one)
-  const int in_final = L.halt ? L.in : a.itmax;
-  const double* Ufin = (in_final & 1) ? a.u1 : a.u0;
-
-  // ---- unscal (src/itpackv.F:2424-2479), ETAS and ETA2 (src/gwce.F:2006-2008)
-  if (row_thread) {
-    for (int k = 0; k < JCG_MAXK; ++k) {
-      const int c = blockIdx.x + k * nblk;
-      if (c >= nch) break;
-      for (int r = 0; r < RED_PER_THREAD; ++r) {
-        const int i = c * RED_CHUNK + t + RED_THREADS * r;
-        if (i >= np) break;
-        const double wi = a.w[i];
-        const double u = Ufin[i] * wi;
-        const double ri = 1.0 / wi;
-        const int nn = a.nneigh[i];
-        for (int j = 1; j < nn; ++j) {
-          const size_t q = (size_t)j * np + i;
-          a.coef[q] = a.cs[q] * ri * (1.0 / a.w[a.col[q]]);
-        }
-        a.coef[i] = ri * ri;
-        a.rhs[i] = a.rhs[i] * ri;
-        a.etas[i] = u;
-        if (a.eta2) a.eta2[i] = (double)a.nodecode[i] * (u + 0.0) + a.eta1[i];
-      }
-    }
-  }
-  if (blockIdx.x == 0 && t == 0) {
-    a.flags[F_NUMITR] = in_final;
-    a.flags[F_IER] = L.halt ? 0 : 13;
-    a.saved[0] = L.q1 ? 1.0 : 0.0;
-    a.saved[1] = L.gamold;
-    a.saved[2] = L.rhoold;
-    a.saved[3] = L.cmold;
-    a.saved[4] = L.cme;
-    a.saved[5] = L.stptst;
-    a.flags[F_ITSUM] += in_final;
-  }
 }
 
 // ---------------------------------------------------------------- wet/dry

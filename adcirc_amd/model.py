@@ -18,11 +18,119 @@ from . import _lib
 from ._lib import check, dp_, ip_
 
 DEG2RAD = 3.141592653589793 / 180.0
-PRBCKGRND_MH2O_G = 100.0 * 1013.0 / 1000.0  # divided by g below (src/wind.F:353-354)
 
 
 def _p(a, t):
     return a.ctypes.data_as(t)
+
+
+def c_layout(tables: Dict[str, np.ndarray], run, tau0var=None, evm=None, fric=None,
+             corif=None):
+    """The adcgpu_mesh / adcgpu_params contents in the C ABI's layout: a dict
+    of scalars and a dict of contiguous arrays (int32 / float64), keyed by the
+    struct field names."""
+    t = tables
+    npn, ne = int(t["np"]), int(t["ne"])
+    neta = int(t["neta"])
+    sc = {"np": npn, "ne": ne, "mnei": int(t["mnei"]), "neta": neta,
+          "nvel": int(t["nvel"]), "nvelme": int(t["nvelme"]),
+          "dtdp": run.dtdp, "g": run.g, "h0": run.h0, "velmin": run.velmin,
+          "a00": run.a00, "b00": run.b00, "c00": run.c00, "convcr": run.convcr,
+          # background pressure in m of water, PRBCKGRND*100/(RHOWAT0*G) (src/wind.F:353-354)
+          "prbckgrnd_mh2o": 100.0 * 1013.0 / (1000.0 * run.g),
+          "nolifa": run.nolifa, "nolica": run.nolica, "nolicat": run.nolicat,
+          "nolibf": run.nolibf, "itmax": run.itmax, "nws": run.nws}
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)  # noqa: E731
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+    ar = {
+        "nm": i32(np.asarray(t["nm"], np.int32).reshape(ne, 3).T.ravel()),      # NM(NE,3)
+        "x": f64(t["x"]), "y": f64(t["y"]), "dp": f64(t["dp"]),
+        "fdxe": f64(np.asarray(t["fdxe"], np.float64).reshape(ne, 3).ravel()),  # FDXE(3,NE)
+        "fdye": f64(np.asarray(t["fdye"], np.float64).reshape(ne, 3).ravel()),
+        "areas": f64(t["areas"]), "nneigh": i32(t["nneigh"]), "neitab": i32(t["neitab"]),
+        "csr_position": i32(np.asarray(t["csr"], np.int32).reshape(ne, 3, 2)
+                            .transpose(2, 1, 0).ravel()),                         # (NE,3,2)
+        "nbd": i32(t["nbd"] if neta else np.zeros(1)), "nbv": i32(t["nbv"]),
+        "lbcodei": i32(t["lbcodei"]), "csii": f64(t["csii"]), "siii": f64(t["siii"]),
+        "me2gw": i32(t["me2gw"]),
+        "tau0var": f64(tau0var if tau0var is not None else np.full(npn, run.tau0)),
+        "evm": f64(evm if evm is not None else np.full(npn, run.eslm)),
+        "fric": f64(fric if fric is not None else np.full(npn, run.cf)),
+        "corif": f64(corif if corif is not None else np.full(npn, run.cori)),
+    }
+    return sc, ar
+
+
+_MESH_I = ("np", "ne", "mnei", "neta", "nvel", "nvelme")
+_MESH_A = ("nm", "x", "y", "dp", "fdxe", "fdye", "areas", "nneigh", "neitab", "csr_position",
+           "nbd", "nbv", "lbcodei", "csii", "siii", "me2gw")
+_PRM_D = ("dtdp", "g", "h0", "velmin", "a00", "b00", "c00", "convcr", "prbckgrnd_mh2o")
+_PRM_I = ("nolifa", "nolica", "nolicat", "nolibf", "itmax", "nws")
+_PRM_A = ("tau0var", "evm", "fric", "corif")
+
+
+def write_case(path: str, tables, run, eta2, nodecode, noff, eta_bc_table, holland=None,
+               **nodal) -> None:
+    """Writes a case for the native driver (adcgpu_run_case): the C-ABI
+    arrays, the cold-start state, the elevation-boundary table and the
+    Holland parameters.  Format: magic "ADCGCASE", then records of
+    (16-byte name, int32 kind 0=float64 1=int32, int32 0, int64 count, data)."""
+    sc, ar = c_layout(tables, run, **nodal)
+    recs = [("precision", np.array([_lib.FP64], np.int32))]
+    recs += [(k, np.array([sc[k]], np.int32)) for k in _MESH_I + _PRM_I]
+    recs += [(k, np.array([sc[k]], np.float64)) for k in _PRM_D]
+    recs += [(k, ar[k]) for k in _MESH_A + _PRM_A]
+    recs += [("eta2_0", np.asarray(eta2, np.float64)),
+             ("nodecode_0", np.asarray(nodecode, np.int32)),
+             ("noff_0", np.asarray(noff, np.int32)),
+             ("eta_bc_table", np.asarray(eta_bc_table, np.float64).ravel())]
+    if holland is not None:
+        recs.append(("holland", np.asarray(list(holland) + [0.0] * (13 - len(holland)),
+                                           np.float64)))
+    with open(path, "wb") as f:
+        f.write(b"ADCGCASE")
+        for name, a in recs:
+            a = np.ascontiguousarray(a)
+            kind = 1 if a.dtype == np.int32 else 0
+            assert a.dtype in (np.int32, np.float64), name
+            f.write(name.encode().ljust(16, b"\0"))
+            f.write(np.array([kind, 0], np.int32).tobytes())
+            f.write(np.array([a.size], np.int64).tobytes())
+            f.write(a.tobytes())
+
+
+def run_times(run, it: int):
+    """(timeloc, timeh, ramp) of step it: TimeLoc (src/timestep.F:257), the
+    harmonic time TIMEH of the elevation forcing, and RampElev, the tanh ramp
+    on DRampElev (src/timestep.F:296-304, 350)."""
+    timeloc = it * run.dtdp + run.statim * 86400.0
+    timeh = it * run.dtdp + (run.statim - run.reftim) * 86400.0
+    if run.nramp == 0 or run.drampelev < 1.0e-6:
+        ramp = 1.0
+    else:
+        ramp = math.tanh((2.0 * timeloc / 86400.0) / run.drampelev)
+    return timeloc, timeh, ramp
+
+
+def elevation_table(run, emo, efa, it0: int, nsteps: int) -> np.ndarray:
+    """Elevation-boundary values for steps it0 .. it0+nsteps-1 through
+    adcgpu_elevation_bc (host computation, no device context); emo/efa are
+    (NETA, NBFR), efa in radians."""
+    lib = _lib.load()
+    emo = np.ascontiguousarray(emo, np.float64)
+    efa = np.ascontiguousarray(efa, np.float64)
+    neta, nbfr = emo.shape
+    amig = np.array(run.amig, np.float64)
+    ff = np.array(run.ff, np.float64)
+    face = np.array([v * DEG2RAD for v in run.face], np.float64)
+    tab = np.empty((nsteps, max(neta, 1)))
+    for s in range(nsteps):
+        _, timeh, ramp = run_times(run, it0 + s)
+        row = tab[s]
+        check(lib.adcgpu_elevation_bc(neta, nbfr, _p(amig, dp_), _p(ff, dp_), _p(face, dp_),
+                                      _p(emo, dp_), _p(efa, dp_), timeh, ramp, _p(row, dp_)),
+              "elevation_bc")
+    return tab
 
 
 class Model:
@@ -34,49 +142,23 @@ class Model:
         t = tables
         self.np_, self.ne = int(t["np"]), int(t["ne"])
         self.neta, self.mnei = int(t["neta"]), int(t["mnei"])
-        self._keep = []
-
-        def ia(a):
-            a = np.ascontiguousarray(a, np.int32)
-            self._keep.append(a)
-            return _p(a, ip_)
-
-        def da(a):
-            a = np.ascontiguousarray(a, np.float64)
-            self._keep.append(a)
-            return _p(a, dp_)
-
-        ne = self.ne
-        nm_cm = np.asarray(t["nm"], np.int32).reshape(ne, 3).T.ravel()     # NM(NE,3)
-        fdxe = np.asarray(t["fdxe"], np.float64).reshape(ne, 3).ravel()    # FDXE(3,NE)
-        fdye = np.asarray(t["fdye"], np.float64).reshape(ne, 3).ravel()
-        csr_cm = np.asarray(t["csr"], np.int32).reshape(ne, 3, 2).transpose(2, 1, 0).ravel()
+        sc, ar = c_layout(tables, run, tau0var, evm, fric, corif)
         m = _lib.Mesh()
-        m.np, m.ne, m.mnei, m.neta = self.np_, ne, self.mnei, self.neta
-        m.nvel, m.nvelme = int(t["nvel"]), int(t["nvelme"])
-        m.nm = ia(nm_cm); m.x = da(t["x"]); m.y = da(t["y"]); m.dp = da(t["dp"])
-        m.fdxe = da(fdxe); m.fdye = da(fdye); m.areas = da(t["areas"])
-        m.nneigh = ia(t["nneigh"]); m.neitab = ia(t["neitab"]); m.csr_position = ia(csr_cm)
-        m.nbd = ia(t["nbd"] if self.neta else np.zeros(1)); m.nbv = ia(t["nbv"])
-        m.lbcodei = ia(t["lbcodei"]); m.csii = da(t["csii"]); m.siii = da(t["siii"])
-        m.me2gw = ia(t["me2gw"])
+        for k in _MESH_I:
+            setattr(m, k, sc[k])
+        for k in _MESH_A:
+            setattr(m, k, _p(ar[k], ip_ if ar[k].dtype == np.int32 else dp_))
         p = _lib.Params()
-        p.dtdp, p.g, p.h0, p.velmin = run.dtdp, run.g, run.h0, run.velmin
-        p.a00, p.b00, p.c00, p.convcr = run.a00, run.b00, run.c00, run.convcr
-        p.prbckgrnd_mh2o = PRBCKGRND_MH2O_G / (1000.0 * run.g) * 1000.0 if False else \
-            100.0 * 1013.0 / (1000.0 * run.g)
-        p.nolifa, p.nolica, p.nolicat, p.nolibf = run.nolifa, run.nolica, run.nolicat, run.nolibf
-        p.itmax, p.nws, p.precision = run.itmax, run.nws, precision
-        npn = self.np_
-        p.tau0var = da(tau0var if tau0var is not None else np.full(npn, run.tau0))
-        p.evm = da(evm if evm is not None else np.full(npn, run.eslm))
-        p.fric = da(fric if fric is not None else np.full(npn, run.cf))
-        p.corif = da(corif if corif is not None else np.full(npn, run.cori))
+        for k in _PRM_D + _PRM_I:
+            setattr(p, k, sc[k])
+        p.precision = precision
+        for k in _PRM_A:
+            setattr(p, k, _p(ar[k], dp_))
         self.prbckgrnd_mh2o = p.prbckgrnd_mh2o
         h = C.c_void_p()
         check(lib.adcgpu_create(C.byref(m), C.byref(p), device, C.byref(h)), "adcgpu_create")
         self.h = h
-        self._keep = []  # the library copied everything
+        del ar  # the library copied everything
         # elevation forcing tables (radians), (NBFR, NETA) column-major = [neta][nbfr]
         self.nbfr = run.nbfr
         self.amig = np.array(run.amig, np.float64)
@@ -129,14 +211,7 @@ class Model:
 
     # ------------------------------------------------------------- forcing
     def times(self, it: int):
-        r = self.run
-        timeloc = it * r.dtdp + r.statim * 86400.0
-        timeh = it * r.dtdp + (r.statim - r.reftim) * 86400.0
-        if r.nramp == 0 or r.drampelev < 1.0e-6:
-            ramp = 1.0
-        else:
-            ramp = math.tanh((2.0 * timeloc / 86400.0) / r.drampelev)
-        return timeloc, timeh, ramp
+        return run_times(self.run, it)
 
     def eta_bc(self, it: int, out: Optional[np.ndarray] = None) -> np.ndarray:
         _, timeh, ramp = self.times(it)
@@ -199,9 +274,6 @@ class Model:
         out = np.zeros(3, np.int32)
         check(self.lib.adcgpu_jcg_shape(self.h, _p(out, ip_)), "jcg_shape")
         return tuple(int(v) for v in out)
-
-    def set_jcg_blocks(self, blocks: int) -> None:
-        check(self.lib.adcgpu_set_jcg_blocks(self.h, int(blocks)), "set_jcg_blocks")
 
     def synchronize(self):
         check(self.lib.adcgpu_synchronize(self.h), "synchronize")

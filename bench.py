@@ -84,6 +84,9 @@ def main() -> int:
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dump", metavar="PATH",
+                    help="write the case for the native driver adcirc_amd/adcgpu_run_case "
+                         "(warmup + steps steps of forcing) and exit without using the GPU")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -91,11 +94,17 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    from adcirc_amd.model import Model, cold_wetdry_state
+    from adcirc_amd.model import Model, cold_wetdry_state, elevation_table, write_case
 
     case, tables = build_case(a.nodes)
     run = case.run
     eta2, nodecode, noff = cold_wetdry_state(tables, run)
+    if a.dump:
+        efa = np.asarray(case.efa, np.float64) * (3.141592653589793 / 180.0)
+        table = elevation_table(run, case.emo, efa, 1, a.warmup + a.steps)
+        write_case(a.dump, tables, run, eta2, nodecode, noff, table, run.holland)
+        log(f"case written to {a.dump}")
+        return 0
     g = Model(tables, run, emo=case.emo, efa=case.efa * (3.141592653589793 / 180.0))
     g.cold_start(eta2, nodecode, noff)
     log(f"JCG launch: {g.jcg_shape()} (blocks, resident-block limit, 2048-row chunks)")

@@ -148,13 +148,11 @@ int64_t adcgpu_run_iterations(adcgpu_ctx *ctx);
 int adcgpu_profile(adcgpu_ctx *ctx, int enable);
 int adcgpu_profile_read(adcgpu_ctx *ctx, double *jcg_ms, int64_t *launches);
 
-/* Launch shape of the JCG solver (no counterpart in the reference; for tuning
- * and tests): shape[0] = blocks per launch, shape[1] = the most blocks the
- * device keeps resident at once, shape[2] = 2048-row chunks of the system.
- * adcgpu_set_jcg_blocks picks the block count (0 = default); each block owns
- * at most 2 chunks.  Results do not depend on it. */
+/* Launch shape of the JCG solver (no counterpart in the reference; for
+ * tuning): shape[0] = blocks per launch (one per 2048-row chunk plus one),
+ * shape[1] = the most blocks the device keeps resident at once (the size
+ * limit), shape[2] = 2048-row chunks of the system. */
 int adcgpu_jcg_shape(adcgpu_ctx *ctx, int *shape3);
-int adcgpu_set_jcg_blocks(adcgpu_ctx *ctx, int blocks);
 
 /* Elevation-boundary values at the NBD nodes for one time (src/gwce.F:
  * 1638-1651): sum over constituents J of EMO(J,I)*FF(J)*RampElev*cos(AMIG(J)*

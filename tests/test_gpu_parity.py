@@ -15,10 +15,8 @@ from adcirc_amd import meshgen
 pytestmark = pytest.mark.gpu
 
 
-def _run_bitexact(d, run, nsteps, check_every=1, jcg_blocks=None):
+def _run_bitexact(d, run, nsteps, check_every=1):
     g, o = build_pair(d, run)
-    if jcg_blocks is not None:
-        g.set_jcg_blocks(jcg_blocks)
     for it in range(1, nsteps + 1):
         step_pair(g, o, it)
         if it % check_every == 0 or it <= 3 or it == nsteps:
@@ -57,20 +55,6 @@ def test_coastal_mesh_bitexact():
     d = case_setup(case)
     log(f"case coastal {int(d['np'])} nodes: 120 steps bit-exact vs oracle")
     _run_bitexact(d, case.run, 120, check_every=10)
-
-
-def test_coastal_mesh_two_chunks_per_block():
-    """The JCG launch with the fewest blocks (each block owns two 2048-row
-    chunks, the shape used when a mesh has more chunks than resident blocks):
-    still bit-exact vs oracle, since the reduction tree is fixed by chunks."""
-    from gpu_helpers import case_setup
-    case = meshgen.coastal(20000)
-    d = case_setup(case)
-    nch = (int(d["np"]) + 2047) // 2048
-    need = (nch + 1) // 2
-    log(f"case coastal {int(d['np'])} nodes, JCG on {need} blocks for {nch} chunks: 60 steps")
-    g, _ = _run_bitexact(d, case.run, 60, check_every=10, jcg_blocks=need)
-    assert g.jcg_shape()[0] == need
 
 
 def test_basin_full_day_resident_vs_reference(basin_golden):
