@@ -48,7 +48,41 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+    build_fortran(verbose)
     return LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_CHECK = os.path.join(FORTRAN, "jcg_dropin_check")
+
+
+def flang() -> str:
+    for c in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang"):
+        if c and os.path.exists(c):
+            return c
+    return ""
+
+
+def build_fortran(verbose: bool = False) -> None:
+    """The ISO_C_BINDING interface module and the program that calls the
+    drop-in JCG through it (skipped, with a message, when no flang exists)."""
+    fc = flang()
+    if not fc:
+        print("flang not found: Fortran interface check not built", file=sys.stderr)
+        return
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    iface_o = os.path.join(bdir, "adcgpu_iface.o")
+    cmds = [
+        [fc, "-O2", "-c", os.path.join(FORTRAN, "adcgpu_iface.F90"), "-module-dir", bdir,
+         "-o", iface_o],
+        [fc, "-O2", os.path.join(FORTRAN, "jcg_dropin_check.F90"), iface_o, "-I", bdir,
+         "-L", HERE, "-ladcgpu", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_CHECK],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

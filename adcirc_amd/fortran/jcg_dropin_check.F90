@@ -27,10 +27,12 @@ program jcg_dropin_check
   end do
   rhs = b
   etas = 0.d0
-  ! dfault + ADCIRC overrides (src/itpackv.F:1137-1167, src/gwce.F:146-150)
+  ! dfault (src/itpackv.F:1135-1160) with ADCIRC's overrides (src/gwce.F:146-150):
+  ! IPARM(1)=ITMAX, IPARM(2)=ISLDIA, IPARM(4)=log unit, RPARM(1)=CONVCR
   iparm = (/ 100, 0, 0, 16, 0, 1, 1, 0, -2, 0, 0, 3 /)
   rparm = 0.d0
   rparm(1) = 1.d-10; rparm(4) = 0.75d0; rparm(5) = 1.d0; rparm(7) = 0.25d0
+  rparm(8) = 128.d0 * epsilon(1.d0) / 2.d0
   ier = adcgpu_jcg(n, n, mnei, neitab, coef, rhs, etas, iparm, rparm)
   r = maxval(abs(etas - x))
   print '(a,i0,a,i0,a,es10.3)', 'jcg ier=', ier, ' NUMITR=', iparm(1), ' max|x-x*|=', r
