@@ -86,7 +86,8 @@ struct adcgpu_ctx {
   int64_t run_iters = 0;
   // optional timing of the JCG launches with HIP events on the context stream
   int profile = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> jcg_events;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> jcg_events;  // pool, created ahead
+  size_t ev_used = 0;
   std::vector<void*> allocs;
 
   DevMesh mesh() const {
@@ -143,9 +144,9 @@ int jcg_blocks_per_cu(int device, int* per_cu_out, int* cus_out) {
   return 0;
 }
 
-// One block per 2048-row chunk plus the eigenvalue block, all resident.
+// JCG_CPB chunks of 2048 rows per block plus the eigenvalue block, all resident.
 int set_jcg_grid(adcgpu_ctx* c) {
-  const int grid = c->nch + 1;
+  const int grid = (c->nch + JCG_CPB - 1) / JCG_CPB + 1;
   if (grid > c->jcg_cap)
     return fail(ADCGPU_E_CAPACITY, "system too large for the register-resident JCG (" +
                                        std::to_string(c->nch) + " chunks of 2048 rows, " +
@@ -164,6 +165,20 @@ int setup_jcg_grid(adcgpu_ctx* c) {
   HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
   c->barrier_ticks = (long long)std::max(khz, 1000) * 500;
   return set_jcg_grid(c);
+}
+
+// Event pairs for timing JCG launches, created outside the timed region.
+int grow_events(adcgpu_ctx* c, size_t n) {
+  while (c->jcg_events.size() < n) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return fail(ADCGPU_E_HIP, "hipEventCreate");
+    }
+    c->jcg_events.emplace_back(e0, e1);
+  }
+  return 0;
 }
 
 int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
@@ -202,18 +217,14 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   hipLaunchKernelGGL(k_fill_u32, dim3(nblk(bar_words, 256)), dim3(256), 0, c->stream, c->jbar, 0u,
                      bar_words);
   void* args[] = {&a};
-  hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profile) {
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, c->stream));
+    if (c->ev_used == c->jcg_events.size())
+      if (int rc = grow_events(c, c->jcg_events.size() + 256)) return rc;
+    HIPCHK(hipEventRecord(c->jcg_events[c->ev_used].first, c->stream));
   }
   HIPCHK(hipLaunchCooperativeKernel((const void*)k_jcg, dim3(c->jcg_grid), dim3(JCG_BLOCK), args,
                                     0, c->stream));
-  if (c->profile) {
-    HIPCHK(hipEventRecord(e1, c->stream));
-    c->jcg_events.emplace_back(e0, e1);
-  }
+  if (c->profile) HIPCHK(hipEventRecord(c->jcg_events[c->ev_used++].second, c->stream));
   return 0;
 }
 
@@ -412,6 +423,10 @@ void adcgpu_destroy(adcgpu_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& ev : c->jcg_events) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
   for (void* p : c->allocs) hipFree(p);
   if (c->eta_bc_table) hipFree(c->eta_bc_table);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -553,7 +568,7 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc) {
   hipLaunchKernelGGL(k_gwce_flags, dim3(1), dim3(1), 0, c->stream, c->flags);
   hipLaunchKernelGGL(k_lhs_rows, dim3(c->nch), dim3(RED_THREADS), 0, c->stream, c->mesh(),
                      c->state(), c->flags, c->ep_part);
-  hipLaunchKernelGGL(k_ep, dim3(1), dim3(RED_THREADS), 0, c->stream, c->nch, np, c->ep_part,
+  hipLaunchKernelGGL(k_ep, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->nch, np, c->ep_part,
                      c->flags, c->scal);
   hipLaunchKernelGGL(k_lhs_bc, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->flags, c->scal);
@@ -712,12 +727,9 @@ int adcgpu_jcg_shape(adcgpu_ctx* c, int* shape3) {
 int adcgpu_profile(adcgpu_ctx* c, int enable) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (auto& ev : c->jcg_events) {
-    (void)hipEventDestroy(ev.first);
-    (void)hipEventDestroy(ev.second);
-  }
-  c->jcg_events.clear();
+  c->ev_used = 0;
   c->profile = enable;
+  if (enable) return grow_events(c, std::max<size_t>(c->jcg_events.size(), 1024));
   return 0;
 }
 
@@ -725,13 +737,13 @@ int adcgpu_profile_read(adcgpu_ctx* c, double* jcg_ms, int64_t* launches) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
   double tot = 0.0;
-  for (auto& ev : c->jcg_events) {
+  for (size_t k = 0; k < c->ev_used; ++k) {
     float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, ev.first, ev.second));
+    HIPCHK(hipEventElapsedTime(&ms, c->jcg_events[k].first, c->jcg_events[k].second));
     tot += ms;
   }
   if (jcg_ms) *jcg_ms = tot;
-  if (launches) *launches = (int64_t)c->jcg_events.size();
+  if (launches) *launches = (int64_t)c->ev_used;
   return 0;
 }
 

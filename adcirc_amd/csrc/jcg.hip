@@ -1,7 +1,8 @@
 // ITPACK jcg (src/itpackv.F:63-456) for one persistent cooperative launch.
 //
-// Launch: one 256-thread block per 2048-row chunk plus one eigenvalue block.
-// Row thread t of chunk c owns rows c*2048 + t + 256*r (r = 0..7) and keeps
+// Launch: blocks of JCG_CPB x 512 threads, each 512-thread group owning one
+// 2048-row chunk, plus one eigenvalue block.
+// Row thread t of chunk c owns rows c*2048 + t + 512*r (r = 0..3) and keeps
 // their iterates (d, d_old, u, u_old) and the product N d in registers for
 // the whole solve; only d is published to memory, for the neighbours'
 // gathers.  The eigenvalue block runs chgcon/zbrent (the adaptive estimate of
@@ -260,9 +261,12 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   const int np = a.n;
   const int t = threadIdx.x;
   const int nch = (np + RED_CHUNK - 1) / RED_CHUNK;
-  const int c = blockIdx.x;
-  const bool eig_block = c == nch;  // the last block: no rows
-  const int base = c * RED_CHUNK + t;
+  const int nrb = (nch + JCG_CPB - 1) / JCG_CPB;  // blocks with rows
+  const bool eig_block = (int)blockIdx.x == nrb;  // the last block: no rows
+  // each group of RED_THREADS threads owns one chunk; rows past NP (including
+  // a whole missing chunk in the last block) are skipped like padding
+  const int c = blockIdx.x * JCG_CPB + t / RED_THREADS;
+  const int base = c * RED_CHUNK + t % RED_THREADS;
   unsigned gen = 0;
   int* err = a.flags + F_ERROR;
 
@@ -292,8 +296,10 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   // per-row iterates, register resident
   double dc[RED_PER_THREAD], dp[RED_PER_THREAD], uc[RED_PER_THREAD], up[RED_PER_THREAD];
   double nd[RED_PER_THREAD];
-  // neighbour counts of the 8 rows, one byte each (MNEI <= 255)
-  unsigned nnp[2] = {0x01010101u, 0x01010101u};
+  // neighbour counts of the rows, one byte each (MNEI <= 255)
+  unsigned nnp[(RED_PER_THREAD + 3) / 4];
+#pragma unroll
+  for (int k = 0; k < (RED_PER_THREAD + 3) / 4; ++k) nnp[k] = 0x01010101u;
 #define NN(r) ((int)((nnp[(r) >> 2] >> (8 * ((r) & 3))) & 255u))
 
   // ---- scal, part 1: diagonal scaling (src/itpackv.F:2374-2393)
@@ -375,23 +381,44 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         a.saved[7] = L.cmold;
       }
     } else {
-      // N d, rows in two groups of four (bounded register pressure); per row
-      // the neighbours are taken in slot order
+      // N d, rows in groups of four (bounded register pressure); per row
+      // the neighbours are taken in slot order.  The next slot's column
+      // index and coefficient are loaded while this slot's d is gathered, so
+      // each slot costs one memory round trip instead of two.
 #pragma unroll
       for (int h = 0; h < RED_PER_THREAD; h += 4) {
         int jm = 1;
+        unsigned cn[4];
+        double sn[4];
 #pragma unroll
         for (int r = h; r < h + 4; ++r) {
           nd[r] = 0.0;
           jm = NN(r) > jm ? NN(r) : jm;
+          const unsigned q = (unsigned)(np + base + RED_THREADS * r);
+          cn[r - h] = 1 < NN(r) ? (unsigned)at(a.col, q) : 0u;
+          sn[r - h] = 1 < NN(r) ? at(a.cs, q) : 0.0;
         }
         for (int j = 1; j < jm; ++j) {
-          const unsigned q0 = (unsigned)(j * np + base);
+          unsigned cc[4];
+          double sc[4];
 #pragma unroll
-          for (int r = h; r < h + 4; ++r) {
-            const unsigned q = q0 + RED_THREADS * r;
-            if (j < NN(r)) nd[r] = nd[r] - at(a.cs, q) * at(a.d, (unsigned)at(a.col, q));
+          for (int k = 0; k < 4; ++k) {
+            cc[k] = cn[k];
+            sc[k] = sn[k];
           }
+          if (j + 1 < jm) {
+            const unsigned q0 = (unsigned)((j + 1) * np + base);
+#pragma unroll
+            for (int r = h; r < h + 4; ++r) {
+              if (j + 1 < NN(r)) {
+                cn[r - h] = (unsigned)at(a.col, q0 + RED_THREADS * r);
+                sn[r - h] = at(a.cs, q0 + RED_THREADS * r);
+              }
+            }
+          }
+#pragma unroll
+          for (int r = h; r < h + 4; ++r)
+            if (j < NN(r)) nd[r] = nd[r] - sc[r - h] * at(a.d, cc[r - h]);
         }
       }
       double add = 0.0, adt = 0.0, auu = 0.0;
@@ -404,23 +431,27 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         }
       }
       const double s0 = wave_sum(add), s1 = wave_sum(adt), s2 = wave_sum(auu);
-      __shared__ double wsum[12];
-      if ((t & 63) == 0) {
-        wsum[0 + (t >> 6)] = s0;
-        wsum[4 + (t >> 6)] = s1;
-        wsum[8 + (t >> 6)] = s2;
+      // chunk value: ((w0+w1)+(w2+w3))+((w4+w5)+(w6+w7)) over the 8 waves
+      __shared__ double wsum[JCG_CPB][24];
+      const int tt = t % RED_THREADS;
+      double* ws = wsum[t / RED_THREADS];
+      if ((tt & 63) == 0) {
+        ws[0 + (tt >> 6)] = s0;
+        ws[8 + (tt >> 6)] = s1;
+        ws[16 + (tt >> 6)] = s2;
       }
       __syncthreads();
-      if (t < 3) {
-        const double* v = wsum + 4 * t;
-        a.part[(size_t)c * 4 + t] = (v[0] + v[1]) + (v[2] + v[3]);
+      if (tt < 3 && c < nch) {
+        const double* v = ws + 8 * tt;
+        a.part[(size_t)c * 4 + tt] =
+            ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
       }
     }
     if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
 
     // reduce the chunk partials (every block, identical tree)
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-    for (int cc = t; cc < nch; cc += RED_THREADS) {
+    for (int cc = t; t < RED_XTHREADS && cc < nch; cc += RED_XTHREADS) {
       const double* pc = a.part + (size_t)cc * 4;
       r0 = r0 + pc[0];
       r1 = r1 + pc[1];

@@ -5,13 +5,16 @@
 
 namespace adcgpu {
 
-// fixed blocked reduction: 2048-row chunks, 256 threads x 8 rows
-constexpr int RED_THREADS = 256;
-constexpr int RED_PER_THREAD = 8;
+// fixed blocked reduction: 2048-row chunks of 512 threads x 4 rows; chunk
+// values combined by 256 threads
+constexpr int RED_THREADS = 512;
+constexpr int RED_PER_THREAD = 4;
 constexpr int RED_CHUNK = RED_THREADS * RED_PER_THREAD;
+constexpr int RED_XTHREADS = 256;
 
 // persistent JCG launch: one block per chunk plus one eigenvalue block
-constexpr int JCG_BLOCK = RED_THREADS;  // one 2048-row chunk per block
+constexpr int JCG_CPB = 2;                      // 2048-row chunks per block
+constexpr int JCG_BLOCK = RED_THREADS * JCG_CPB;  // threads per block
 constexpr int JCG_BAR_GROUP = 16;        // blocks per first-level barrier counter
 constexpr int JCG_BAR_STRIDE = 64;       // words between barrier counters (256 B)
 constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BAR_STRIDE

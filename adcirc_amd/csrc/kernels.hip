@@ -18,7 +18,7 @@
 namespace adcgpu {
 
 // ------------------------------------------------------------ time levels
-__global__ void k_init_h(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
+__global__ __launch_bounds__(256) void k_init_h(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
                          double ifnlfa, double* __restrict__ h1, double* __restrict__ h2) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
@@ -30,7 +30,7 @@ __global__ void k_init_h(int np, const double* __restrict__ dp, const double* __
 // Apply2DBottomFriction (src/nodalattr.F:2400-2411) for IFHYBF = 0, and the
 // TKM = (TK, TK, 0) of Apply2DInternalWaveDrag without wave drag; TKM is
 // represented by TK alone.
-__global__ void k_friction(int np, const double* __restrict__ uu1, const double* __restrict__ vv1,
+__global__ __launch_bounds__(256) void k_friction(int np, const double* __restrict__ uu1, const double* __restrict__ vv1,
                            const double* __restrict__ dp, const double* __restrict__ eta2,
                            const double* __restrict__ fric, double ifnlfa, double iflinbf,
                            double ifnlbf, double tkmax, double* __restrict__ tk) {
@@ -47,7 +47,7 @@ __global__ void k_friction(int np, const double* __restrict__ uu1, const double*
 // Holland (1980) vortex forcing on the device (synthetic storms for the
 // resident benchmark run).  Same formulas as the oracle's host version; device
 // exp/pow/sqrt differ from host libm by a few ulp.
-__global__ void k_holland(int np, const double* __restrict__ x, const double* __restrict__ y,
+__global__ __launch_bounds__(256) void k_holland(int np, const double* __restrict__ x, const double* __restrict__ y,
                           const double* __restrict__ tptr, int tstep, HollandPrm p,
                           double* __restrict__ wsx, double* __restrict__ wsy,
                           double* __restrict__ pr) {
@@ -83,7 +83,7 @@ __global__ void k_holland(int np, const double* __restrict__ x, const double* __
 }
 
 // --------------------------------------------------------------- GWCE LHS
-__global__ void k_gwce_flags(int* flags) {
+__global__ __launch_bounds__(256) void k_gwce_flags(int* flags) {
   // rebuild = NCChange > 0; NCChange = 0   (src/gwce.F:427-428)
   flags[F_REBUILD] = flags[F_NCCHANGE] > 0 ? 1 : 0;
   flags[F_NCCHANGE] = 0;
@@ -92,9 +92,9 @@ __global__ void k_gwce_flags(int* flags) {
 // Row assembly of COEF (src/gwce.F:456-592), one thread per row, incident
 // elements in ascending element order; also the EP chunk partials of
 // sum COEF(i,1)^2 (src/gwce.F:646-650) in the fixed blocked tree.
-__global__ __launch_bounds__(256) void k_lhs_rows(DevMesh m, DevState s, const int* __restrict__ flags,
+__global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s, const int* __restrict__ flags,
                                                   double* __restrict__ ep_part) {
-  __shared__ double lds[4];
+  __shared__ double lds[8];
   if (flags[F_REBUILD] == 0) return;
   const int np = m.np;
   const int base = blockIdx.x * RED_CHUNK + threadIdx.x;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_lhs_rows(DevMesh m, DevState s, const i
     row[i] = diag;
     acc = acc + diag * diag;
   }
-  double tot = block_tree(acc, lds);
+  double tot = chunk_tree(acc, lds);
   if (threadIdx.x == 0) ep_part[blockIdx.x] = tot;
 }
 
@@ -152,13 +152,13 @@ __global__ __launch_bounds__(256) void k_ep(int nch, int np, const double* __res
   __shared__ double lds[4];
   if (flags[F_REBUILD] == 0) return;
   double acc = 0.0;
-  for (int c = threadIdx.x; c < nch; c += RED_THREADS) acc = acc + part[c];
+  for (int c = threadIdx.x; c < nch; c += RED_XTHREADS) acc = acc + part[c];
   double tot = block_tree(acc, lds);
   if (threadIdx.x == 0) scal[S_EP] = __builtin_sqrt(tot / (double)np);
 }
 
 // Boundary rows/columns and diagonal checks (src/gwce.F:654-707).
-__global__ void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* __restrict__ scal) {
+__global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* __restrict__ scal) {
   if (flags[F_REBUILD] == 0) return;
   const int np = m.np;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -192,7 +192,7 @@ __global__ void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* __rest
 // the reference multiplies by identically zero quantities in this
 // configuration (sponge layer, dispersion, baroclinic, tidal potential, water
 // level offset) are omitted; they only add signed zeros.
-__global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, double* __restrict__ contrib) {
+__global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, double* __restrict__ contrib) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
@@ -297,7 +297,7 @@ __global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, double* __restrict__
 // Node gather of GWCE_LV, the time-level shift of ETA and the elevation
 // boundary conditions (src/gwce.F:1613-1631, 1638-1651, 1846-1855).  `eta1`
 // already points at the previous ETA2; `eta2` is rewritten.
-__global__ void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ contrib,
+__global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ contrib,
                            const double* __restrict__ eta_bc, const double* __restrict__ scal) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
@@ -325,7 +325,7 @@ __global__ void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ con
 
 // ---------------------------------------------------------------- wet/dry
 // D1 (src/wetdry.F:234-285); also NNODECODE := NODECODE for the W1 writes.
-__global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin) {
+__global__ __launch_bounds__(256) void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   int nc = s.nodecode[i];
@@ -345,7 +345,7 @@ __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double hab
 // Candidate elements whose steady-flow velocity exceeds VELMIN wet their dry
 // node here (an idempotent store); their friction updates are order
 // dependent and are appended for the ordered scan below.
-__global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdCand* __restrict__ cand,
+__global__ __launch_bounds__(256) void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdCand* __restrict__ cand,
                           int* __restrict__ flags) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(1024) void k_wd_scan(const WdCand* __restrict__ can
 
 // D2: MJU, TotalArea, landlocked nodes, NODECODE update, NCCHANGE, and
 // H2 = DP + IFNLFA*ETA2 (src/wetdry.F:1361-1430, src/timestep.F:1097-1099).
-__global__ void k_wd_d2(DevMesh m, DevState s, int* flags, double ifnlfa) {
+__global__ __launch_bounds__(256) void k_wd_d2(DevMesh m, DevState s, int* flags, double ifnlfa) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   int mju = 0;
@@ -475,7 +475,7 @@ __global__ void k_wd_d2(DevMesh m, DevState s, int* flags, double ifnlfa) {
 
 // NOFF changes count toward NCCHANGE (src/wetdry.F:1428-1430), and
 // computeAlpha (src/wetdry.F:1561-1652).
-__global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim) {
+__global__ __launch_bounds__(256) void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   if (s.noff[e] != s.noffold[e]) atomicOr(&flags[F_NCCHANGE], 1);
@@ -509,7 +509,7 @@ __global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim) {
   s.alphal[e] = al;
 }
 
-__global__ void k_h2(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
+__global__ __launch_bounds__(256) void k_h2(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
                      double ifnlfa, double* __restrict__ h2) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
@@ -519,7 +519,7 @@ __global__ void k_h2(int np, const double* __restrict__ dp, const double* __rest
 // --------------------------------------------------------------- momentum
 // Element loop of Mom_Eqs_New_NC (src/momentum.F:298-699): advection,
 // barotropic pressure gradient, IBPV lateral stress; contributions x NCEle.
-__global__ void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* __restrict__ cx,
+__global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* __restrict__ cx,
                            double* __restrict__ cy) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
@@ -579,7 +579,7 @@ __global__ void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* __restrict__
 // Nodal part of Mom_Eqs_New_NC (src/momentum.F:713-1182): gather, /TotalArea,
 // wind, friction/Coriolis 2x2, land-boundary constraints, solve, fluxes;
 // then the peak-surge maximum (src/write_output.F:6357-6366).
-__global__ void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* __restrict__ cx,
+__global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* __restrict__ cx,
                            const double* __restrict__ cy) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
@@ -645,18 +645,18 @@ __global__ void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* __rest
   if (nci == 1 && e2 > s.etamax[i]) s.etamax[i] = e2;
 }
 
-__global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
+__global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, unsigned v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-__global__ void k_fill_i32(int* p, int v, int n) {
+__global__ __launch_bounds__(256) void k_fill_i32(int* p, int v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
 // TotalArea / MJU at cold start (src/mesh.F:3360-3400), node gather.
-__global__ void k_total_area(DevMesh m, DevState s) {
+__global__ __launch_bounds__(256) void k_total_area(DevMesh m, DevState s) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   int mju = 0;

@@ -1,9 +1,10 @@
 // Fixed-order reductions shared by the device kernels.  Every inner product
-// of the hot path is formed with one blocked tree: 2048-row chunks, 8 rows per
-// thread summed in row order, an xor butterfly over the 64 lanes of a wave,
-// and (w0+w1)+(w2+w3) over the four waves; chunk values are then summed by
-// thread t over chunks t, t+256, ... and reduced with the same tree.  The
-// tree is a property of the data layout, not of the launch shape.
+// of the hot path is formed with one blocked tree: 2048-row chunks of 512
+// threads, 4 rows per thread summed in row order (rows t + 512 r), an xor
+// butterfly over the 64 lanes of a wave, and ((w0+w1)+(w2+w3)) +
+// ((w4+w5)+(w6+w7)) over the eight waves; the chunk values are then summed by
+// thread t of 256 over chunks t, t+256, ..., the butterfly, and (w0+w1)+(w2+w3).
+// The tree is a property of the data layout, not of the launch shape.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -15,8 +16,8 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Reduces one value per thread of the first 256 threads of the block with the
-// fixed tree; result valid in thread 0.  `lds` holds 4 doubles.
+// Cross-chunk stage: reduces one value per thread of the first 256 threads
+// of the block; result valid in thread 0.  `lds` holds 4 doubles.
 __device__ __forceinline__ double block_tree(double v, double* lds) {
   const int t = threadIdx.x;
   double w = 0.0;
@@ -25,6 +26,24 @@ __device__ __forceinline__ double block_tree(double v, double* lds) {
   __syncthreads();
   double r = 0.0;
   if (t == 0) r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return r;
+}
+
+// Chunk stage: reduces one value per thread of a 512-thread group (threads
+// g*512 .. g*512+511 of the block, g = group); result valid in the group's
+// first thread.  `lds` holds 8 doubles per group; every thread of the block
+// must call it.
+__device__ __forceinline__ double chunk_tree(double v, double* lds) {
+  const int t = threadIdx.x;
+  const double w = wave_sum(v);
+  if ((t & 63) == 0) lds[t >> 6] = w;
+  __syncthreads();
+  double r = 0.0;
+  if ((t & 511) == 0) {
+    const double* x = lds + 8 * (t >> 9);
+    r = ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+  }
   __syncthreads();
   return r;
 }

@@ -5,11 +5,12 @@
 //   adcgpu_create -> adcgpu_cold_start -> adcgpu_holland (time 0)
 //   -> adcgpu_run (warm-up steps) -> adcgpu_run (timed steps)
 //
-//   adcgpu_run_case CASE [--warmup W] [--steps K] [--chunk C]
+//   adcgpu_run_case CASE [--warmup W] [--steps K] [--chunk C] [--copy-mib N]
 //   adcgpu_run_case --copy-mib N
 //
 // Prints one line of timing: steps, ms per step, JCG iterations per step.
-// --copy-mib runs a plain device-to-device copy kernel over N MiB ten times:
+// --copy-mib runs a plain device-to-device copy kernel over N MiB ten times
+// (after the case, when one is given):
 // the reference reading of the HBM traffic counters (FETCH_SIZE, WRITE_SIZE)
 // for a kernel whose bytes are known.
 #include <algorithm>
@@ -116,11 +117,12 @@ int main(int argc, char** argv) {
     return 2;
   }
   if (argc == 3 && !std::strcmp(argv[1], "--copy-mib")) return copy_probe(std::atoi(argv[2]));
-  int warmup = 10, steps = 40, chunk = 200;
+  int warmup = 10, steps = 40, chunk = 200, copy_mib = 0;
   for (int a = 2; a + 1 < argc; a += 2) {
     if (!std::strcmp(argv[a], "--warmup")) warmup = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--steps")) steps = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--chunk")) chunk = std::atoi(argv[a + 1]);
+    else if (!std::strcmp(argv[a], "--copy-mib")) copy_mib = std::atoi(argv[a + 1]);
   }
   std::map<std::string, Record> c;
   if (!read_case(argv[1], c)) {
@@ -183,5 +185,5 @@ int main(int argc, char** argv) {
   std::printf("steps %d ms_per_step %.4f jcg_iterations_per_step %.3f\n", steps, ms / steps,
               steps ? (double)iters / steps : 0.0);
   adcgpu_destroy(ctx);
-  return 0;
+  return copy_mib > 0 ? copy_probe(copy_mib) : 0;
 }

@@ -38,9 +38,17 @@ module adcgpu_iface
   public :: adcgpu_create, adcgpu_destroy, adcgpu_jcg, adcgpu_cold_start, adcgpu_begin_step, &
        adcgpu_friction, adcgpu_set_forcing, adcgpu_solve_gwce, adcgpu_wetdry, &
        adcgpu_momentum, adcgpu_get_field, adcgpu_set_field, adcgpu_last_numitr, &
-       adcgpu_elevation_bc
+       adcgpu_elevation_bc, adcgpu_last_error, adcgpu_error_message
 
   interface
+     !> message of the last failed call on this thread (NUL-terminated C string)
+     type(c_ptr) function adcgpu_last_error() bind(C, name="adcgpu_last_error")
+       import :: c_ptr
+     end function
+     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+       import :: c_ptr, c_size_t
+       type(c_ptr), value :: s
+     end function
      integer(c_int) function adcgpu_create(mesh, prm, device, out) bind(C, name="adcgpu_create")
        import :: c_int, c_ptr, adcgpu_mesh, adcgpu_params
        type(adcgpu_mesh), intent(in) :: mesh
@@ -128,4 +136,25 @@ module adcgpu_iface
        real(c_double), intent(out) :: out(*)
      end function
   end interface
+
+contains
+
+  !> adcgpu_last_error as a Fortran string
+  function adcgpu_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: k, n
+    p = adcgpu_last_error()
+    if (.not. c_associated(p)) then
+       msg = ''
+       return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: msg)
+    do k = 1, n
+       msg(k:k) = chars(k)
+    end do
+  end function
 end module adcgpu_iface

@@ -36,5 +36,6 @@ program jcg_dropin_check
   ier = adcgpu_jcg(n, n, mnei, neitab, coef, rhs, etas, iparm, rparm)
   r = maxval(abs(etas - x))
   print '(a,i0,a,i0,a,es10.3)', 'jcg ier=', ier, ' NUMITR=', iparm(1), ' max|x-x*|=', r
+  if (ier < 0) print '(2a)', 'adcgpu_last_error: ', adcgpu_error_message()
   if (ier /= 0 .or. r > 1.d-6) stop 1
 end program

@@ -32,10 +32,12 @@ extern "C" {
 
 enum { ADCO_ORDER_SEQUENTIAL = 0, ADCO_ORDER_BLOCKED = 1 };
 
-/* Blocked reduction shape shared with the GPU kernels. */
-#define ADCO_RED_THREADS 256
-#define ADCO_RED_PER_THREAD 8
+/* Blocked reduction shape shared with the GPU kernels: 2048-element chunks
+ * of 512 threads x 4 elements; the chunk values are combined by 256 threads. */
+#define ADCO_RED_THREADS 512
+#define ADCO_RED_PER_THREAD 4
 #define ADCO_RED_CHUNK (ADCO_RED_THREADS * ADCO_RED_PER_THREAD)
+#define ADCO_RED_XTHREADS 256
 
 /* ITPACK's COMMON /itcom1-3/ plus the SAVEd locals of itjcg and chgcon. */
 typedef struct {

@@ -16,12 +16,13 @@
 #define J(i, j) jcoef[(size_t)((j)-1) * ndim + ((i)-1)]
 
 /* Blocked tree reduction; the GPU kernels use exactly this shape:
- * chunks of 2048 elements, thread t of a chunk sums elements t + 256*r
- * (r = 0..7) sequentially, the 64 lanes of a wave combine with an xor
- * butterfly (offsets 32..1), the 4 wave sums combine as (w0+w1)+(w2+w3);
- * the chunk sums are then reduced with the same per-thread / butterfly /
- * wave pattern (thread t sums chunk sums t, t+256, ...). */
-static double wave_butterfly(double *v) {
+ * chunks of 2048 elements, thread t (0..511) of a chunk sums elements
+ * t + 512*r (r = 0..3) sequentially, the 64 lanes of a wave combine with an
+ * xor butterfly (offsets 32..1), the 8 wave sums of a chunk combine as
+ * ((w0+w1)+(w2+w3))+((w4+w5)+(w6+w7)); the chunk sums are then reduced by
+ * 256 threads (thread t sums chunk sums t, t+256, ...), the butterfly, and
+ * (w0+w1)+(w2+w3) over those 4 waves. */
+static double wave_butterfly(const double *v) {
   double a[64], b[64];
   memcpy(a, v, sizeof(a));
   for (int off = 32; off >= 1; off >>= 1) {
@@ -31,10 +32,14 @@ static double wave_butterfly(double *v) {
   return a[0];
 }
 
-static double block_combine(double *thr /* [256] */) {
+static double combine4(const double *thr /* [256] */) {
   double w0 = wave_butterfly(thr), w1 = wave_butterfly(thr + 64);
   double w2 = wave_butterfly(thr + 128), w3 = wave_butterfly(thr + 192);
   return (w0 + w1) + (w2 + w3);
+}
+
+static double combine8(const double *thr /* [512] */) {
+  return combine4(thr) + combine4(thr + 256);
 }
 
 static double dot_blocked(int n, const double *x, const double *y) {
@@ -52,14 +57,14 @@ static double dot_blocked(int n, const double *x, const double *y) {
       }
       thr[t] = acc;
     }
-    part[c] = block_combine(thr);
+    part[c] = combine8(thr);
   }
-  for (int t = 0; t < ADCO_RED_THREADS; ++t) {
+  for (int t = 0; t < ADCO_RED_XTHREADS; ++t) {
     double acc = 0.0;
-    for (int c = t; c < nch; c += ADCO_RED_THREADS) acc = acc + part[c];
+    for (int c = t; c < nch; c += ADCO_RED_XTHREADS) acc = acc + part[c];
     thr[t] = acc;
   }
-  double r = block_combine(thr);
+  double r = combine4(thr);
   if (part != stackbuf) free(part);
   return r;
 }
