@@ -26,7 +26,7 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_get_field", "adcgpu_cold_start", "adcgpu_begin_step", "adcgpu_friction",
            "adcgpu_set_forcing", "adcgpu_solve_gwce", "adcgpu_wetdry", "adcgpu_momentum",
            "adcgpu_timestep", "adcgpu_last_numitr", "adcgpu_synchronize", "adcgpu_run",
-           "adcgpu_run_iterations", "adcgpu_elevation_bc", "adcgpu_jcg", "adcgpu_holland",
+           "adcgpu_run_iterations", "adcgpu_run_rebuilds", "adcgpu_elevation_bc", "adcgpu_jcg", "adcgpu_holland",
            "adcgpu_profile", "adcgpu_profile_read", "adcgpu_jcg_shape")
 
 
@@ -79,6 +79,8 @@ def load() -> C.CDLL:
     lib.adcgpu_run.argtypes = [vp, C.c_int, C.c_int, dp_, dp_, C.c_int]
     lib.adcgpu_run_iterations.argtypes = [vp]
     lib.adcgpu_run_iterations.restype = C.c_int64
+    lib.adcgpu_run_rebuilds.argtypes = [vp]
+    lib.adcgpu_run_rebuilds.restype = C.c_int64
     lib.adcgpu_elevation_bc.argtypes = [C.c_int, C.c_int, dp_, dp_, dp_, dp_, dp_, C.c_double,
                                         C.c_double, dp_]
     lib.adcgpu_jcg.argtypes = [C.c_int, C.c_int, C.c_int, ip_, dp_, dp_, dp_, ip_, dp_]

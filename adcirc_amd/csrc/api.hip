@@ -59,7 +59,8 @@ struct adcgpu_ctx {
   double ifnlfa = 1, ifnlct = 1, ifnlcat = 1, iflinbf = 0, ifnlbf = 1, tkmax = 1;
   double habsmin = 0, hoff = 0;
   // mesh
-  int *nm = nullptr, *inc_ptr = nullptr, *inc = nullptr, *nneigh = nullptr, *col = nullptr;
+  int *nm = nullptr, *inc_ptr = nullptr, *inc = nullptr, *slot = nullptr, *nneigh = nullptr;
+  int *col = nullptr, *slot_cols = nullptr;
   int *csrpos = nullptr, *bnd_index = nullptr, *nbd = nullptr, *obc_slot = nullptr;
   int *obcg_ptr = nullptr, *obcg = nullptr, *vbc_ptr = nullptr, *vbc = nullptr, *lbcodei = nullptr;
   double *fdx = nullptr, *fdy = nullptr, *areas = nullptr, *x = nullptr, *y = nullptr, *dp = nullptr;
@@ -83,7 +84,7 @@ struct adcgpu_ctx {
   int jcg_grid = 0;
   long long barrier_ticks = 0;  // 0.5 s of the device wall clock
   int jcg_cap = 0;              // resident k_jcg blocks on this device
-  int64_t run_iters = 0;
+  int64_t run_iters = 0, run_rebuilds = 0;
   // optional timing of the JCG launches with HIP events on the context stream
   int profile = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> jcg_events;  // pool, created ahead
@@ -96,6 +97,7 @@ struct adcgpu_ctx {
     m.dt = prm.dtdp; m.g = prm.g; m.a00 = prm.a00; m.b00 = prm.b00; m.c00 = prm.c00;
     m.nm = nm; m.fdx = fdx; m.fdy = fdy; m.areas = areas; m.x = x; m.y = y; m.dp = dp;
     m.tau0 = tau0; m.evm = evm; m.corif = corif; m.inc_ptr = inc_ptr; m.inc = inc;
+    m.slot = slot; m.slot_cols = slot_cols;
     m.nneigh = nneigh; m.col = col; m.csrpos = csrpos; m.bnd_index = bnd_index; m.nbd = nbd;
     m.obc_slot = obc_slot; m.obcg_ptr = obcg_ptr; m.obcg = obcg; m.vbc_ptr = vbc_ptr;
     m.vbc = vbc; m.lbcodei = lbcodei; m.csii = csii; m.siii = siii;
@@ -240,6 +242,8 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   const int np = M->np, ne = M->ne, mnei = M->mnei, neta = M->neta;
   if (np <= 0 || ne <= 0 || mnei < 2 || neta < 0 || M->nvel < 0 || M->nvelme < 0)
     return fail(ADCGPU_E_ARG, "invalid sizes");
+  if (mnei > LHS_MAXW)
+    return fail(ADCGPU_E_UNSUPPORTED, "MNEI above " + std::to_string(LHS_MAXW));
   if (P->nolibf == 2) return fail(ADCGPU_E_UNSUPPORTED, "hybrid bottom friction (NOLIBF=2)");
   if (P->nolibf < 0 || P->nolibf > 3) return fail(ADCGPU_E_ARG, "NOLIBF");
   if (P->itmax <= 0 || P->itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
@@ -262,8 +266,16 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     for (int k = 0; k < 3; ++k) cnt[nm0[3 * (size_t)e + k] + 1]++;
   for (int i = 0; i < np; ++i) cnt[i + 1] += cnt[i];
   std::vector<int> inc(3 * (size_t)ne), fillp(cnt.begin(), cnt.end() - 1);
-  for (int e = 0; e < ne; ++e)  // ascending element order per node
-    for (int k = 0; k < 3; ++k) inc[fillp[nm0[3 * (size_t)e + k]]++] = 3 * e + k;
+  // ascending element order per node; slot is the inverse map, so element
+  // kernels write each (element, vertex) contribution where its node's
+  // ordered sum reads it
+  std::vector<int> slot(3 * (size_t)ne);
+  for (int e = 0; e < ne; ++e)
+    for (int k = 0; k < 3; ++k) {
+      const int p = fillp[nm0[3 * (size_t)e + k]]++;
+      inc[p] = 3 * e + k;
+      slot[3 * (size_t)e + k] = p;
+    }
   std::vector<int> nneigh(M->nneigh, M->nneigh + np);
   std::vector<int> col((size_t)mnei * np, -1);
   for (int i = 0; i < np; ++i) {
@@ -284,6 +296,12 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
         if (J < 2 || J > nneigh[node]) return fail(ADCGPU_E_ARG, "CSR_POSITION out of range");
         csrpos[(3 * (size_t)e + k) * 2 + l] = J - 1;
       }
+  // per incidence slot, the two CSR_POSITION columns of its row
+  std::vector<int> slot_cols(3 * (size_t)ne);
+  for (size_t p = 0; p < 3 * (size_t)ne; ++p) {
+    const int code = inc[p];
+    slot_cols[p] = csrpos[(size_t)code * 2] | (csrpos[(size_t)code * 2 + 1] << 16);
+  }
   std::vector<double> fdx(3 * (size_t)ne), fdy(3 * (size_t)ne);
   for (size_t q = 0; q < 3 * (size_t)ne; ++q) {
     fdx[q] = M->fdxe[q];
@@ -341,6 +359,11 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     vbc.insert(vbc.end(), vbl[i].begin(), vbl[i].end());
   }
 
+  // NCEle is recovered from NCEle*AREAS/2 in the wet/dry D2 gather
+  for (int e = 0; e < ne; ++e)
+    if (!(M->areas[e] > 0.0))
+      return fail(ADCGPU_E_ARG, "element " + std::to_string(e + 1) + " has a non-positive AREAS");
+
   adcgpu_ctx* c = new adcgpu_ctx();
   c->device = device;
   c->np = np; c->ne = ne; c->mnei = mnei; c->neta = neta;
@@ -370,7 +393,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   std::vector<double> vt(P->tau0var, P->tau0var + np), ve(P->evm, P->evm + np);
   std::vector<double> vcor(P->corif, P->corif + np), vfric(P->fric, P->fric + np);
   std::vector<double> var(M->areas, M->areas + ne);
-  U(&c->nm, nm0); U(&c->inc_ptr, cnt); U(&c->inc, inc); U(&c->nneigh, nneigh); U(&c->col, col);
+  U(&c->nm, nm0); U(&c->inc_ptr, cnt); U(&c->inc, inc); U(&c->slot, slot); U(&c->slot_cols, slot_cols); U(&c->nneigh, nneigh); U(&c->col, col);
   U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_slot, obc_slot);
   U(&c->obcg_ptr, obcg_ptr); U(&c->obcg, obcg); U(&c->vbc_ptr, vbc_ptr); U(&c->vbc, vbc);
   U(&c->lbcodei, lbc); U(&c->fdx, fdx); U(&c->fdy, fdy); U(&c->areas, var); U(&c->x, vx);
@@ -396,6 +419,14 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
   }
   if (!rc) rc = setup_jcg_grid(c);
+  if (!rc) {
+    // the LHS row kernel keeps MNEI x 512 doubles of LDS per block
+    hipError_t e = hipFuncSetAttribute((const void*)k_lhs_rows,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       mnei * RED_THREADS * (int)sizeof(double));
+    if (e != hipSuccess) rc = fail(ADCGPU_E_UNSUPPORTED, std::string("LDS for MNEI: ") +
+                                                         hipGetErrorString(e));
+  }
   if (!rc) {
     std::vector<double> ones(ne, 1.0), pr(np, P->prbckgrnd_mh2o), emax(np, -99999.0);
     (void)hipMemcpy(c->alphal, ones.data(), ne * sizeof(double), hipMemcpyHostToDevice);
@@ -566,8 +597,12 @@ int adcgpu_set_forcing(adcgpu_ctx* c, const double* wsx2, const double* wsy2, co
 static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc) {
   const int np = c->np, ne = c->ne;
   hipLaunchKernelGGL(k_gwce_flags, dim3(1), dim3(1), 0, c->stream, c->flags);
-  hipLaunchKernelGGL(k_lhs_rows, dim3(c->nch), dim3(RED_THREADS), 0, c->stream, c->mesh(),
-                     c->state(), c->flags, c->ep_part);
+  // contrib, cx and cy are free until the GWCE right-hand side
+  hipLaunchKernelGGL(k_lhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->flags, c->contrib, c->cx, c->cy);
+  hipLaunchKernelGGL(k_lhs_rows, dim3(c->nch), dim3(RED_THREADS),
+                     (size_t)c->mnei * RED_THREADS * sizeof(double), c->stream, c->mesh(),
+                     c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part);
   hipLaunchKernelGGL(k_ep, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->nch, np, c->ep_part,
                      c->flags, c->scal);
   hipLaunchKernelGGL(k_lhs_bc, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
@@ -613,8 +648,11 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
   hipLaunchKernelGGL(k_wd_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), q, c->cand, c->flags);
   hipLaunchKernelGGL(k_wd_scan, dim3(1), dim3(1024), 0, c->stream, c->cand, c->flags, c->tk);
+  // cx is free until the momentum step: it holds NCEle*AREAS/2 per incidence slot
+  hipLaunchKernelGGL(k_wd_area, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->cx);
   hipLaunchKernelGGL(k_wd_d2, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), c->flags, c->ifnlfa);
+                     c->state(), c->cx, c->flags, c->ifnlfa);
   hipLaunchKernelGGL(k_wd_alpha, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->flags, 1.0e9);
   return check_launch("wetdry");
@@ -687,7 +725,8 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
     HIPCHK(hipMemcpyAsync(c->eta_bc_table, eta_bc_table, (size_t)nsteps * c->neta * 8,
                           hipMemcpyHostToDevice, c->stream));
   }
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_ITSUM, 0, 1);
+  // F_ITSUM and F_NREBUILD are adjacent: one fill clears both
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_ITSUM, 0, 2);
   for (int s = 0; s < nsteps; ++s) {
     const int it = it0 + s;
     int rc = adcgpu_begin_step(c, it);
@@ -708,6 +747,7 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->run_iters = f[F_ITSUM];
+  c->run_rebuilds = f[F_NREBUILD];
   if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
   if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
   if (f[F_ERROR] & ERR_CAPACITY) return fail(ADCGPU_E_CAPACITY, "wetting candidate capacity");
@@ -715,6 +755,8 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
 }
 
 int64_t adcgpu_run_iterations(adcgpu_ctx* c) { return c ? c->run_iters : -1; }
+
+int64_t adcgpu_run_rebuilds(adcgpu_ctx* c) { return c ? c->run_rebuilds : -1; }
 
 int adcgpu_jcg_shape(adcgpu_ctx* c, int* shape3) {
   if (!c || !shape3) return fail(ADCGPU_E_ARG, "null argument");

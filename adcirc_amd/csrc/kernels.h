@@ -19,10 +19,12 @@ constexpr int JCG_BAR_GROUP = 16;        // blocks per first-level barrier count
 constexpr int JCG_BAR_STRIDE = 64;       // words between barrier counters (256 B)
 constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BAR_STRIDE
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
+constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
 constexpr int WD_SORT_CAP = 4096;    // wetting candidates per step
 
 // device flag slots
-enum { F_NCCHANGE = 0, F_REBUILD, F_NUMITR, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_COUNT = 16 };
+enum { F_NCCHANGE = 0, F_REBUILD, F_NUMITR, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_NREBUILD,
+       F_COUNT = 16 };
 enum { ERR_DIAG = 1, ERR_CAPACITY = 2, ERR_BARRIER = 4 };
 // device scalar slots
 enum { S_EP = 0, S_COUNT = 8 };
@@ -42,6 +44,8 @@ struct DevMesh {
   const double* corif;
   const int* inc_ptr;   // [np+1] node -> incident (element, local) codes 3e+k
   const int* inc;       //        sorted by element id
+  const int* slot;      // [ne*3] position p of code 3e+k in inc (inc[slot[c]] == c)
+  const int* slot_cols; // [ne*3] per slot p: CSR_POSITION slots ja | jb << 16 (0-based)
   const int* nneigh;    // [np]
   const int* col;       // [mnei*np] column-major, 0-based (slot 0 = self)
   const int* csrpos;    // [ne*3*2] 0-based column slots
@@ -119,7 +123,10 @@ __global__ void k_friction(int np, const double* uu1, const double* vv1, const d
 __global__ void k_holland(int np, const double* x, const double* y, const double* tptr, int tstep,
                           HollandPrm p, double* wsx, double* wsy, double* pr);
 __global__ void k_gwce_flags(int* flags);
-__global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, double* ep_part);
+__global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
+                           double* sb);
+__global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
+                           const double* sa, const double* sb, double* ep_part);
 __global__ void k_ep(int nch, int np, const double* part, const int* flags, double* scal);
 __global__ void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* scal);
 __global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, double* contrib);
@@ -129,7 +136,8 @@ __global__ void k_jcg(JcgArgs a);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdCand* cand, int* flags);
 __global__ void k_wd_scan(const WdCand* cand, int* flags, double* tk);
-__global__ void k_wd_d2(DevMesh m, DevState s, int* flags, double ifnlfa);
+__global__ void k_wd_area(DevMesh m, DevState s, double* area_slot);
+__global__ void k_wd_d2(DevMesh m, DevState s, const double* area_slot, int* flags, double ifnlfa);
 __global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim);
 __global__ void k_h2(int np, const double* dp, const double* eta2, double ifnlfa, double* h2);
 __global__ void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* cx, double* cy);

@@ -86,64 +86,85 @@ __global__ __launch_bounds__(256) void k_holland(int np, const double* __restric
 __global__ __launch_bounds__(256) void k_gwce_flags(int* flags) {
   // rebuild = NCChange > 0; NCChange = 0   (src/gwce.F:427-428)
   flags[F_REBUILD] = flags[F_NCCHANGE] > 0 ? 1 : 0;
+  flags[F_NREBUILD] += flags[F_REBUILD];
   flags[F_NCCHANGE] = 0;
 }
 
-// Row assembly of COEF (src/gwce.F:456-592), one thread per row, incident
-// elements in ascending element order; also the EP chunk partials of
-// sum COEF(i,1)^2 (src/gwce.F:646-650) in the fixed blocked tree.
-__global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s, const int* __restrict__ flags,
-                                                  double* __restrict__ ep_part) {
+// GWCE matrix, element part (src/gwce.F:456-592): for element e and its local
+// vertex k, the diagonal term and the two off-diagonal terms of row NM(e,k),
+// written to the row's incidence slot p = slot[3e+k] (sd, sa, sb).
+__global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
+                                                  const int* __restrict__ flags,
+                                                  double* __restrict__ sd, double* __restrict__ sa,
+                                                  double* __restrict__ sb) {
+  if (flags[F_REBUILD] == 0) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const double dt = m.dt;
+  const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
+  const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
+  const double NCELE = (double)ncele;
+  const double fx[3] = {m.fdx[3 * e], m.fdx[3 * e + 1], m.fdx[3 * e + 2]};
+  const double fy[3] = {m.fdy[3 * e], m.fdy[3 * e + 1], m.fdy[3 * e + 2]};
+  const double area2 = m.areas[e];
+  const double area = area2 * 0.5;
+  const double area4 = area2 * 2.0;
+  const double dpavg = (m.dp[n1] + m.dp[n2] + m.dp[n3]) / 3.0;
+  const double ga = s.alphal[e] * m.a00 * m.g * dpavg * 1.0 / area4;
+  const double t0avg = (m.tau0[n1] + m.tau0[n2] + m.tau0[n3]) / 3.0;
+  const double mson = (2.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
+  const double msoff = (1.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    // diagonal: (MsOn + GA*(FDXk*FDXk + FDYk*FDYk))*NCELE; off-diagonals in
+    // the reference's local order (k+1, k+2 mod 3 map to CSR_POSITION(e,k,1)
+    // and (e,k,2))
+    const int ka = (k == 0) ? 1 : 0, kb = (k == 2) ? 1 : 2;
+    const int p = m.slot[3 * e + k];
+    sd[p] = (mson * 1.0 + ga * (1.0 * fx[k] * fx[k] + 1.0 * fy[k] * fy[k])) * NCELE;
+    sa[p] = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[ka] + 1.0 * fy[k] * fy[ka])) * NCELE;
+    sb[p] = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[kb] + 1.0 * fy[k] * fy[kb])) * NCELE;
+  }
+}
+
+// GWCE matrix, row part: row i of COEF is the sum of its incidence slots in
+// ascending element order, each row held in LDS while it is formed (the
+// reference adds into COEF(i,j) element by element, src/gwce.F:538-592);
+// also the EP chunk partials of sum COEF(i,1)^2 (src/gwce.F:646-650) in the
+// fixed blocked tree.  Dynamic LDS: MNEI x RED_THREADS doubles, slot-major.
+__global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
+                                                          const int* __restrict__ flags,
+                                                          const double* __restrict__ sd,
+                                                          const double* __restrict__ sa,
+                                                          const double* __restrict__ sb,
+                                                          double* __restrict__ ep_part) {
   __shared__ double lds[8];
+  extern __shared__ double racc_lds[];
+  double(*racc)[RED_THREADS] = reinterpret_cast<double(*)[RED_THREADS]>(racc_lds);
   if (flags[F_REBUILD] == 0) return;
   const int np = m.np;
-  const int base = blockIdx.x * RED_CHUNK + threadIdx.x;
-  const double dt = m.dt, a00g = 0.0;
-  (void)a00g;
+  const int t = threadIdx.x;
+  const int base = blockIdx.x * RED_CHUNK + t;
   double acc = 0.0;
   for (int r = 0; r < RED_PER_THREAD; ++r) {
-    int i = base + RED_THREADS * r;
+    const int i = base + RED_THREADS * r;
     if (i >= np) break;
     const int nn = m.nneigh[i];
-    double* row = s.coef;
-    for (int j = 0; j < nn; ++j) row[(size_t)j * np + i] = 0.0;
-    const int p0 = m.inc_ptr[i], p1 = m.inc_ptr[i + 1];
+    for (int j = 1; j < nn; ++j) racc[j][t] = 0.0;
     double diag = 0.0;
-    for (int p = p0; p < p1; ++p) {
-      const int code = m.inc[p];
-      const int e = code / 3, k = code - 3 * e;
-      const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
-      const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
-      const double NCELE = (double)ncele;
-      const double fx[3] = {m.fdx[3 * e], m.fdx[3 * e + 1], m.fdx[3 * e + 2]};
-      const double fy[3] = {m.fdy[3 * e], m.fdy[3 * e + 1], m.fdy[3 * e + 2]};
-      const double area2 = m.areas[e];
-      const double area = area2 * 0.5;
-      const double area4 = area2 * 2.0;
-      const double dpavg = (m.dp[n1] + m.dp[n2] + m.dp[n3]) / 3.0;
-      const double ga = s.alphal[e] * m.a00 * m.g * dpavg * 1.0 / area4;
-      const double t0avg = (m.tau0[n1] + m.tau0[n2] + m.tau0[n3]) / 3.0;
-      const double mfac = area * (1.0 / dt + t0avg / 2.0);
-      const double mson = (2.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
-      const double msoff = (1.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
-      (void)mfac;
-      // diagonal: (MsOn + GA*(FDXk*FDXk + FDYk*FDYk))*NCELE
-      const double cd = (mson * 1.0 + ga * (1.0 * fx[k] * fx[k] + 1.0 * fy[k] * fy[k])) * NCELE;
-      diag = diag + cd;
-      // the two off-diagonals of this element in row i, in the reference's
-      // local order (k+1, k+2 mod 3 map to CSR_POSITION(e,k,1) and (e,k,2))
-      const int ka = (k == 0) ? 1 : 0, kb = (k == 2) ? 1 : 2;
-      const int ja = m.csrpos[(3 * e + k) * 2 + 0], jb = m.csrpos[(3 * e + k) * 2 + 1];
-      const double ca = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[ka] + 1.0 * fy[k] * fy[ka])) * NCELE;
-      const double cb = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[kb] + 1.0 * fy[k] * fy[kb])) * NCELE;
-      row[(size_t)ja * np + i] = row[(size_t)ja * np + i] + ca;
-      row[(size_t)jb * np + i] = row[(size_t)jb * np + i] + cb;
+    for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
+      const int jab = m.slot_cols[p];
+      const int ja = jab & 0xffff, jb = jab >> 16;
+      diag = diag + sd[p];
+      racc[ja][t] = racc[ja][t] + sa[p];
+      racc[jb][t] = racc[jb][t] + sb[p];
     }
-    row[i] = diag;
+    s.coef[i] = diag;
+    for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = racc[j][t];
     acc = acc + diag * diag;
   }
   double tot = chunk_tree(acc, lds);
-  if (threadIdx.x == 0) ep_part[blockIdx.x] = tot;
+  if (t == 0) ep_part[blockIdx.x] = tot;
 }
 
 // EP = sqrt(sum/NP) from the chunk partials (one block).
@@ -289,9 +310,9 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
                     GDP * (A00pB00 * (1.0 * FDX3 * E1XGrad2A + 1.0 * FDY3 * E1YGrad2A)) +
                     (1.0 * JXAvg * FDX3 + 1.0 * JYAvg * FDY3) * 0.5 + Tau0SpaVar;
   const double nce = (double)ncele;
-  contrib[3 * e] = A1 * nce;
-  contrib[3 * e + 1] = A2 * nce;
-  contrib[3 * e + 2] = A3 * nce;
+  contrib[m.slot[3 * e]] = A1 * nce;
+  contrib[m.slot[3 * e + 1]] = A2 * nce;
+  contrib[m.slot[3 * e + 2]] = A3 * nce;
 }
 
 // Node gather of GWCE_LV, the time-level shift of ETA and the elevation
@@ -311,7 +332,7 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
     return;
   }
   double acc = 0.0;
-  for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[m.inc[p]];
+  for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
   for (int p = m.obcg_ptr[i]; p < m.obcg_ptr[i + 1]; ++p) {
     const int slot = m.obcg[p];
     const int I = slot / m.mnei;
@@ -447,19 +468,35 @@ __global__ __launch_bounds__(1024) void k_wd_scan(const WdCand* __restrict__ can
   }
 }
 
+// D2, element part: NCEle (NNODECODE of the three nodes times NOFF) and its
+// area term NCEle*AREAS/2, written to the incidence slots of the element's
+// three nodes (src/wetdry.F:1361-1380).
+__global__ __launch_bounds__(256) void k_wd_area(DevMesh m, DevState s, double* __restrict__ area_slot) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const int ncele = s.nnodecode[m.nm[3 * e]] * s.nnodecode[m.nm[3 * e + 1]] *
+                    s.nnodecode[m.nm[3 * e + 2]] * s.noff[e];
+  const double a = (double)ncele * m.areas[e] / 2.0;
+  area_slot[m.slot[3 * e]] = a;
+  area_slot[m.slot[3 * e + 1]] = a;
+  area_slot[m.slot[3 * e + 2]] = a;
+}
+
 // D2: MJU, TotalArea, landlocked nodes, NODECODE update, NCCHANGE, and
 // H2 = DP + IFNLFA*ETA2 (src/wetdry.F:1361-1430, src/timestep.F:1097-1099).
-__global__ __launch_bounds__(256) void k_wd_d2(DevMesh m, DevState s, int* flags, double ifnlfa) {
+// MJU counts the wet incident elements: AREAS > 0, so NCEle = 1 exactly when
+// the slot's area term is non-zero.
+__global__ __launch_bounds__(256) void k_wd_d2(DevMesh m, DevState s,
+                                               const double* __restrict__ area_slot, int* flags,
+                                               double ifnlfa) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   int mju = 0;
   double ta = 0.0;
   for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
-    const int e = m.inc[p] / 3;
-    const int ncele = s.nnodecode[m.nm[3 * e]] * s.nnodecode[m.nm[3 * e + 1]] *
-                      s.nnodecode[m.nm[3 * e + 2]] * s.noff[e];
-    mju += ncele;
-    ta = ta + (double)ncele * m.areas[e] / 2.0;
+    const double a = area_slot[p];
+    mju += a != 0.0 ? 1 : 0;
+    ta = ta + a;
   }
   int nn = s.nnodecode[i];
   if (nn == 1 && mju == 0) nn = 0;
@@ -571,8 +608,9 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
     // TANPHI = 0: the AreaIE*TANPHI*(...) terms are signed zeros
     const double ta = NCDT * (advx - (1.5 * (lxx * FX[k] + lxy * FY[k]) - AreaIE * 0.0 * (lxy + lyx)) / Hs[k]);
     const double tb = NCDT * (advy - (1.5 * (lyx * FX[k] + lyy * FY[k]) + AreaIE * 0.0 * (lxx - lyy)) / Hs[k]);
-    cx[3 * e + k] = ta;
-    cy[3 * e + k] = tb;
+    const int sl = m.slot[3 * e + k];
+    cx[sl] = ta;
+    cy[sl] = tb;
   }
 }
 
@@ -587,9 +625,8 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
   const int nci = s.nodecode[i];
   double mx = 0.0, my = 0.0;
   for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
-    const int c = m.inc[p];
-    mx = mx + cx[c];
-    my = my + cy[c];
+    mx = mx + cx[p];
+    my = my + cy[p];
   }
   const double ta = s.totalarea[i];
   if (ta != 0.0) {

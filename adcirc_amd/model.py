@@ -155,6 +155,7 @@ class Model:
         for k in _PRM_A:
             setattr(p, k, _p(ar[k], dp_))
         self.prbckgrnd_mh2o = p.prbckgrnd_mh2o
+        self.last_rebuilds = 0
         h = C.c_void_p()
         check(lib.adcgpu_create(C.byref(m), C.byref(p), device, C.byref(h)), "adcgpu_create")
         self.h = h
@@ -254,6 +255,7 @@ class Model:
         rc = self.lib.adcgpu_run(self.h, it0, nsteps, _p(tab, dp_),
                                  _p(hol, dp_) if hol is not None else None, int(use_graph))
         check(rc, "run")
+        self.last_rebuilds = int(self.lib.adcgpu_run_rebuilds(self.h))
         return int(self.lib.adcgpu_run_iterations(self.h))
 
     def holland(self, t: float, prm) -> None:

@@ -126,11 +126,12 @@ def main() -> int:
     # the timed steps are issued in chunks of up to 200 (each chunk ends with a
     # stream synchronisation inside the library) so that progress is visible
     t0 = time.perf_counter()
-    iters, done = 0, 0
+    iters, done, rebuilds = 0, 0, 0
     while done < a.steps:
         k = min(200, a.steps - done)
         s0 = a.warmup + done
         iters += g.run_resident(s0 + 1, k, holland=run.holland, table=table[s0:s0 + k])
+        rebuilds += g.last_rebuilds
         done += k
         log(f"  {done}/{a.steps} steps, {time.perf_counter() - t0:.2f} s")
     g.synchronize()
@@ -169,6 +170,7 @@ def main() -> int:
         },
         "timesteps_per_s": a.steps / wall,
         "jcg_iterations_per_step": iters / a.steps,
+        "gwce_matrix_rebuilds": rebuilds,
         "wet_nodes_end": wet,
         "peak_surge_max_m": float(etamax.max()),
         "roofline": {

@@ -141,6 +141,9 @@ int adcgpu_run(adcgpu_ctx *ctx, int it0, int nsteps, const double *eta_bc_table,
                const double *holland, int use_graph);
 /* Sum of JCG iterations over the last adcgpu_run. */
 int64_t adcgpu_run_iterations(adcgpu_ctx *ctx);
+/* Steps of the last adcgpu_run that rebuilt the GWCE matrix (NCChange > 0,
+ * src/gwce.F:427). */
+int64_t adcgpu_run_rebuilds(adcgpu_ctx *ctx);
 
 /* Timing of the JCG launches with HIP events on the context's stream:
  * adcgpu_profile(ctx, 1) clears and starts recording, adcgpu_profile_read
