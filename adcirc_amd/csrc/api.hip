@@ -420,11 +420,15 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   }
   if (!rc) rc = setup_jcg_grid(c);
   if (!rc) {
-    // the LHS row kernel keeps MNEI x 512 doubles of LDS per block
+    // the LHS row kernel keeps MNEI x 512 doubles of LDS per block; the
+    // wetting scan keeps its candidates in LDS
     hipError_t e = hipFuncSetAttribute((const void*)k_lhs_rows,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        mnei * RED_THREADS * (int)sizeof(double));
-    if (e != hipSuccess) rc = fail(ADCGPU_E_UNSUPPORTED, std::string("LDS for MNEI: ") +
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_wd_scan, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              WD_SCAN_LDS);
+    if (e != hipSuccess) rc = fail(ADCGPU_E_UNSUPPORTED, std::string("LDS request: ") +
                                                          hipGetErrorString(e));
   }
   if (!rc) {
@@ -596,7 +600,6 @@ int adcgpu_set_forcing(adcgpu_ctx* c, const double* wsx2, const double* wsy2, co
 
 static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc) {
   const int np = c->np, ne = c->ne;
-  hipLaunchKernelGGL(k_gwce_flags, dim3(1), dim3(1), 0, c->stream, c->flags);
   // contrib, cx and cy are free until the GWCE right-hand side
   hipLaunchKernelGGL(k_lhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->flags, c->contrib, c->cx, c->cy);
@@ -609,7 +612,7 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc) {
                      c->state(), c->flags, c->scal);
   RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, c->prm.nws};
   hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), q, c->contrib);
+                     c->state(), q, c->flags, c->contrib);
   // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
   std::swap(c->eta1, c->eta2);
   std::swap(c->h1, c->h2);
@@ -641,13 +644,13 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
                        c->ifnlfa, c->h2);
     return check_launch("h2");
   }
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_NCAND, 0, 1);
   hipLaunchKernelGGL(k_wd_d1, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->flags, c->prm.h0, c->habsmin);
   WdPrm q{c->hoff, c->prm.velmin, c->iflinbf, c->ifnlbf, c->tkmax, c->cand_cap};
   hipLaunchKernelGGL(k_wd_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), q, c->cand, c->flags);
-  hipLaunchKernelGGL(k_wd_scan, dim3(1), dim3(1024), 0, c->stream, c->cand, c->flags, c->tk);
+  hipLaunchKernelGGL(k_wd_scan, dim3(1), dim3(1024), WD_SCAN_LDS, c->stream, c->cand, c->flags,
+                     c->tk);
   // cx is free until the momentum step: it holds NCEle*AREAS/2 per incidence slot
   hipLaunchKernelGGL(k_wd_area, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->cx);

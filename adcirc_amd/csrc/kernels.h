@@ -21,9 +21,10 @@ constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BA
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
 constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
 constexpr int WD_SORT_CAP = 4096;    // wetting candidates per step
+constexpr int WD_SCAN_LDS = WD_SORT_CAP * (4 * 4 + 8);  // dynamic LDS of k_wd_scan
 
 // device flag slots
-enum { F_NCCHANGE = 0, F_REBUILD, F_NUMITR, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_NREBUILD,
+enum { F_NCCHANGE = 0, F_NUMITR = 2, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_NREBUILD,
        F_COUNT = 16 };
 enum { ERR_DIAG = 1, ERR_CAPACITY = 2, ERR_BARRIER = 4 };
 // device scalar slots
@@ -122,14 +123,13 @@ __global__ void k_friction(int np, const double* uu1, const double* vv1, const d
                            double ifnlbf, double tkmax, double* tk);
 __global__ void k_holland(int np, const double* x, const double* y, const double* tptr, int tstep,
                           HollandPrm p, double* wsx, double* wsy, double* pr);
-__global__ void k_gwce_flags(int* flags);
 __global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
                            double* sb);
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
                            const double* sa, const double* sb, double* ep_part);
 __global__ void k_ep(int nch, int np, const double* part, const int* flags, double* scal);
 __global__ void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* scal);
-__global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, double* contrib);
+__global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags, double* contrib);
 __global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const double* eta_bc,
                            const double* scal);
 __global__ void k_jcg(JcgArgs a);

@@ -83,12 +83,10 @@ __global__ __launch_bounds__(256) void k_holland(int np, const double* __restric
 }
 
 // --------------------------------------------------------------- GWCE LHS
-__global__ __launch_bounds__(256) void k_gwce_flags(int* flags) {
-  // rebuild = NCChange > 0; NCChange = 0   (src/gwce.F:427-428)
-  flags[F_REBUILD] = flags[F_NCCHANGE] > 0 ? 1 : 0;
-  flags[F_NREBUILD] += flags[F_REBUILD];
-  flags[F_NCCHANGE] = 0;
-}
+// The matrix is rebuilt when NCChange > 0 and NCChange is then cleared
+// (src/gwce.F:427-428): the LHS kernels test F_NCCHANGE, and the first thread
+// of the right-hand-side element kernel, which runs after them and before
+// wet/dry sets the flag again, clears it.
 
 // GWCE matrix, element part (src/gwce.F:456-592): for element e and its local
 // vertex k, the diagonal term and the two off-diagonal terms of row NM(e,k),
@@ -97,7 +95,7 @@ __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
                                                   const int* __restrict__ flags,
                                                   double* __restrict__ sd, double* __restrict__ sa,
                                                   double* __restrict__ sb) {
-  if (flags[F_REBUILD] == 0) return;
+  if (flags[F_NCCHANGE] == 0) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const double dt = m.dt;
@@ -141,7 +139,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
   __shared__ double lds[8];
   extern __shared__ double racc_lds[];
   double(*racc)[RED_THREADS] = reinterpret_cast<double(*)[RED_THREADS]>(racc_lds);
-  if (flags[F_REBUILD] == 0) return;
+  if (flags[F_NCCHANGE] == 0) return;
   const int np = m.np;
   const int t = threadIdx.x;
   const int base = blockIdx.x * RED_CHUNK + t;
@@ -171,7 +169,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
 __global__ __launch_bounds__(256) void k_ep(int nch, int np, const double* __restrict__ part,
                                             const int* __restrict__ flags, double* __restrict__ scal) {
   __shared__ double lds[4];
-  if (flags[F_REBUILD] == 0) return;
+  if (flags[F_NCCHANGE] == 0) return;
   double acc = 0.0;
   for (int c = threadIdx.x; c < nch; c += RED_XTHREADS) acc = acc + part[c];
   double tot = block_tree(acc, lds);
@@ -180,7 +178,7 @@ __global__ __launch_bounds__(256) void k_ep(int nch, int np, const double* __res
 
 // Boundary rows/columns and diagonal checks (src/gwce.F:654-707).
 __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* __restrict__ scal) {
-  if (flags[F_REBUILD] == 0) return;
+  if (flags[F_NCCHANGE] == 0) return;
   const int np = m.np;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
@@ -213,8 +211,13 @@ __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, int* flag
 // the reference multiplies by identically zero quantities in this
 // configuration (sponge layer, dispersion, baroclinic, tidal potential, water
 // level offset) are omitted; they only add signed zeros.
-__global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, double* __restrict__ contrib) {
+__global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags,
+                                                  double* __restrict__ contrib) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    if (flags[F_NCCHANGE] > 0) flags[F_NREBUILD] += 1;
+    flags[F_NCCHANGE] = 0;
+  }
   if (e >= m.ne) return;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
@@ -348,6 +351,7 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
 // D1 (src/wetdry.F:234-285); also NNODECODE := NODECODE for the W1 writes.
 __global__ __launch_bounds__(256) void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) flags[F_NCAND] = 0;  // wetting candidates of this step (k_wd_elem)
   if (i >= m.np) return;
   int nc = s.nodecode[i];
   if (nc == 1) {
@@ -418,53 +422,102 @@ __global__ __launch_bounds__(256) void k_wd_elem(DevMesh m, DevState s, WdPrm q,
   s.noff[e] = noff;
 }
 
-// Ordered W1 friction updates: the candidates are sorted by element id in
-// LDS (bitonic sort) and applied in that order by one lane, which is the
-// reference's element order (src/wetdry.F:447-472).
+// Ordered W1 friction updates (src/wetdry.F:447-472).  The reference visits
+// the wetting candidates in ascending element order; candidate k sets
+// TK(NM123) = MAX(TK_TMP, TK(NM123)), gives the partner the larger of the two
+// and takes it back if the partner was larger, so afterwards both of its
+// nodes hold M_k = max(TK_TMP_k, their values before k).  A node's value
+// before k is M_j of the last earlier candidate j on that node (values only
+// grow), or its value on entry.  So: sort the candidates by element id
+// (bitonic, LDS), find each one's predecessors on its two nodes (parallel),
+// form M_k in element order (one lane, LDS only), and give each node the M
+// of its last candidate (parallel).  TK values are non-negative and finite,
+// so max is exact and order-free.  One block of 1024; dynamic LDS of
+// WD_SCAN_LDS bytes.
 __global__ __launch_bounds__(1024) void k_wd_scan(const WdCand* __restrict__ cand, int* flags,
                                                   double* __restrict__ tk) {
-  __shared__ int key[WD_SORT_CAP];
-  __shared__ int idx[WD_SORT_CAP];
+  extern __shared__ int wd_lds[];
+  int* pa = wd_lds;                    // sort key, then predecessor on node a
+  int* pp = pa + WD_SORT_CAP;          // sort index, then predecessor on node p
+  int* na = pp + WD_SORT_CAP;          // node a of sorted candidate k
+  int* np_ = na + WD_SORT_CAP;         // node p of sorted candidate k
+  double* M = reinterpret_cast<double*>(np_ + WD_SORT_CAP);
   const int n = flags[F_NCAND];
+  const int t = threadIdx.x, nt = blockDim.x;
   if (n == 0) return;
   if (n > WD_SORT_CAP) {
-    if (threadIdx.x == 0) atomicOr(&flags[F_ERROR], ERR_CAPACITY);
+    if (t == 0) atomicOr(&flags[F_ERROR], ERR_CAPACITY);
     return;
   }
   int P = 1;
   while (P < n) P <<= 1;
-  for (int i = threadIdx.x; i < P; i += blockDim.x) {
-    key[i] = i < n ? cand[i].e : 0x7fffffff;
-    idx[i] = i;
+  for (int i = t; i < P; i += nt) {
+    pa[i] = i < n ? cand[i].e : 0x7fffffff;
+    pp[i] = i;
   }
   __syncthreads();
   for (int k = 2; k <= P; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+      for (int i = t; i < P; i += nt) {
         const int l = i ^ j;
         if (l > i) {
           const bool up = (i & k) == 0;
-          const int ki = key[i], kl = key[l];
+          const int ki = pa[i], kl = pa[l];
           if ((ki > kl) == up) {
-            key[i] = kl; key[l] = ki;
-            const int t = idx[i]; idx[i] = idx[l]; idx[l] = t;
+            pa[i] = kl; pa[l] = ki;
+            const int x = pp[i]; pp[i] = pp[l]; pp[l] = x;
           }
         }
       }
       __syncthreads();
     }
   }
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < n; ++k) {
-      const WdCand c = cand[idx[k]];
-      // TK(NM123) = MAX(TK_TMP, TK(NM123)); partner takes the max; if the
-      // partner was larger NM123 takes it back (src/wetdry.F:447-471)
-      const double ta = fmax(c.tk, tk[c.a]);
-      tk[c.a] = ta;
-      const double tp = fmax(ta, tk[c.p]);
-      tk[c.p] = tp;
-      if (ta < tp) tk[c.a] = tp;
+  for (int k = t; k < n; k += nt) {
+    const WdCand c = cand[pp[k]];
+    na[k] = c.a;
+    np_[k] = c.p;
+    M[k] = c.tk;
+  }
+  __syncthreads();
+  // predecessors: the nearest earlier sorted candidate on the same node
+  for (int k = t; k < n; k += nt) {
+    const int a = na[k], p = np_[k];
+    int qa = -1, qp = -1;
+    for (int j = k - 1; j >= 0 && (qa < 0 || qp < 0); --j) {
+      const int x = na[j], y = np_[j];
+      if (qa < 0 && (x == a || y == a)) qa = j;
+      if (qp < 0 && (x == p || y == p)) qp = j;
     }
+    pa[k] = qa;
+    pp[k] = qp;
+    // entry values of nodes with no earlier candidate
+    double m = M[k];
+    if (qa < 0) m = fmax(m, tk[a]);
+    if (qp < 0) m = fmax(m, tk[p]);
+    M[k] = m;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 0; k < n; ++k) {
+      double m = M[k];
+      const int qa = pa[k], qp = pp[k];
+      if (qa >= 0) m = fmax(m, M[qa]);
+      if (qp >= 0) m = fmax(m, M[qp]);
+      M[k] = m;
+    }
+  }
+  __syncthreads();
+  // each node takes the M of its last candidate
+  for (int k = t; k < n; k += nt) {
+    const int a = na[k], p = np_[k];
+    bool la = true, lp = true;
+    for (int j = k + 1; j < n && (la || lp); ++j) {
+      const int x = na[j], y = np_[j];
+      if (x == a || y == a) la = false;
+      if (x == p || y == p) lp = false;
+    }
+    if (la) tk[a] = M[k];
+    if (lp) tk[p] = M[k];
   }
 }
 
