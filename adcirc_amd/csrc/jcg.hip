@@ -265,7 +265,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   const bool eig_block = (int)blockIdx.x == nrb;  // the last block: no rows
   // each group of RED_THREADS threads owns one chunk; rows past NP (including
   // a whole missing chunk in the last block) are skipped like padding
-  const int c = blockIdx.x * JCG_CPB + t / RED_THREADS;
+  const int c = (eig_block ? nrb : xcd_tile(blockIdx.x, nrb)) * JCG_CPB + t / RED_THREADS;
   const int base = c * RED_CHUNK + t % RED_THREADS;
   unsigned gen = 0;
   int* err = a.flags + F_ERROR;

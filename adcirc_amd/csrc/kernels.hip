@@ -20,7 +20,7 @@ namespace adcgpu {
 // ------------------------------------------------------------ time levels
 __global__ __launch_bounds__(256) void k_init_h(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
                          double ifnlfa, double* __restrict__ h1, double* __restrict__ h2) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= np) return;
   double h = dp[i] + ifnlfa * eta2[i];
   h2[i] = h;
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_friction(int np, const double* __restri
                            const double* __restrict__ dp, const double* __restrict__ eta2,
                            const double* __restrict__ fric, double ifnlfa, double iflinbf,
                            double ifnlbf, double tkmax, double* __restrict__ tk) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= np) return;
   double u = uu1[i], v = vv1[i];
   double uv1 = __builtin_sqrt(u * u + v * v);
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_holland(int np, const double* __restric
                           const double* __restrict__ tptr, int tstep, HollandPrm p,
                           double* __restrict__ wsx, double* __restrict__ wsy,
                           double* __restrict__ pr) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= np) return;
   double t = tptr ? tptr[0] : p.t;
   const double pc = p.v[0] * 100.0, pn = p.v[1] * 100.0, rmax = p.v[2], B = p.v[3];
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
                                                   double* __restrict__ sd, double* __restrict__ sa,
                                                   double* __restrict__ sb) {
   if (flags[F_NCCHANGE] == 0) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const double dt = m.dt;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
@@ -142,7 +142,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
   if (flags[F_NCCHANGE] == 0) return;
   const int np = m.np;
   const int t = threadIdx.x;
-  const int base = blockIdx.x * RED_CHUNK + t;
+  const int chunk = xcd_tile(blockIdx.x, gridDim.x);
+  const int base = chunk * RED_CHUNK + t;
   double acc = 0.0;
   for (int r = 0; r < RED_PER_THREAD; ++r) {
     const int i = base + RED_THREADS * r;
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
     acc = acc + diag * diag;
   }
   double tot = chunk_tree(acc, lds);
-  if (t == 0) ep_part[blockIdx.x] = tot;
+  if (t == 0) ep_part[chunk] = tot;
 }
 
 // EP = sqrt(sum/NP) from the chunk partials (one block).
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(256) void k_ep(int nch, int np, const double* __res
 __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* __restrict__ scal) {
   if (flags[F_NCCHANGE] == 0) return;
   const int np = m.np;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const double ep = scal[S_EP];
   const int nn = m.nneigh[i];
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, int* flag
 // level offset) are omitted; they only add signed zeros.
 __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags,
                                                   double* __restrict__ contrib) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e == 0) {
     if (flags[F_NCCHANGE] > 0) flags[F_NREBUILD] += 1;
     flags[F_NCCHANGE] = 0;
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
 // already points at the previous ETA2; `eta2` is rewritten.
 __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ contrib,
                            const double* __restrict__ eta_bc, const double* __restrict__ scal) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   const int bi = m.bnd_index[i];
   if (bi >= 0) {
@@ -350,7 +351,7 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
 // ---------------------------------------------------------------- wet/dry
 // D1 (src/wetdry.F:234-285); also NNODECODE := NODECODE for the W1 writes.
 __global__ __launch_bounds__(256) void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i == 0) flags[F_NCAND] = 0;  // wetting candidates of this step (k_wd_elem)
   if (i >= m.np) return;
   int nc = s.nodecode[i];
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(256) void k_wd_d1(DevMesh m, DevState s, int* flags
 // dependent and are appended for the ordered scan below.
 __global__ __launch_bounds__(256) void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdCand* __restrict__ cand,
                           int* __restrict__ flags) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const int n[3] = {m.nm[3 * e], m.nm[3 * e + 1], m.nm[3 * e + 2]};
   s.noffold[e] = s.noff[e];
@@ -525,7 +526,7 @@ __global__ __launch_bounds__(1024) void k_wd_scan(const WdCand* __restrict__ can
 // area term NCEle*AREAS/2, written to the incidence slots of the element's
 // three nodes (src/wetdry.F:1361-1380).
 __global__ __launch_bounds__(256) void k_wd_area(DevMesh m, DevState s, double* __restrict__ area_slot) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const int ncele = s.nnodecode[m.nm[3 * e]] * s.nnodecode[m.nm[3 * e + 1]] *
                     s.nnodecode[m.nm[3 * e + 2]] * s.noff[e];
@@ -542,7 +543,7 @@ __global__ __launch_bounds__(256) void k_wd_area(DevMesh m, DevState s, double* 
 __global__ __launch_bounds__(256) void k_wd_d2(DevMesh m, DevState s,
                                                const double* __restrict__ area_slot, int* flags,
                                                double ifnlfa) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   int mju = 0;
   double ta = 0.0;
@@ -566,7 +567,7 @@ __global__ __launch_bounds__(256) void k_wd_d2(DevMesh m, DevState s,
 // NOFF changes count toward NCCHANGE (src/wetdry.F:1428-1430), and
 // computeAlpha (src/wetdry.F:1561-1652).
 __global__ __launch_bounds__(256) void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   if (s.noff[e] != s.noffold[e]) atomicOr(&flags[F_NCCHANGE], 1);
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(256) void k_wd_alpha(DevMesh m, DevState s, int* fl
 
 __global__ __launch_bounds__(256) void k_h2(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
                      double ifnlfa, double* __restrict__ h2) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= np) return;
   h2[i] = dp[i] + ifnlfa * eta2[i];
 }
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(256) void k_h2(int np, const double* __restrict__ d
 // barotropic pressure gradient, IBPV lateral stress; contributions x NCEle.
 __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* __restrict__ cx,
                            double* __restrict__ cy) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const double THIRD = 1.0 / 3.0, DT = m.dt, GO2 = m.g / 2.0;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
@@ -672,7 +673,7 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
 // then the peak-surge maximum (src/write_output.F:6357-6366).
 __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* __restrict__ cx,
                            const double* __restrict__ cy) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   const double DTO2 = m.dt / 2.0;
   const int nci = s.nodecode[i];
@@ -736,18 +737,18 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, unsigned v, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int* p, int v, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
 // TotalArea / MJU at cold start (src/mesh.F:3360-3400), node gather.
 __global__ __launch_bounds__(256) void k_total_area(DevMesh m, DevState s) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
   int mju = 0;
   double ta = 0.0;

@@ -48,4 +48,16 @@ __device__ __forceinline__ double chunk_tree(double v, double* lds) {
   return r;
 }
 
+// XCD-aware tile order.  The dispatcher hands workgroup b to XCD b mod 8,
+// so consecutive blocks land in different L2 caches; renumbering blocks so
+// that each XCD works through one contiguous range of tiles keeps the halo of
+// neighbouring tiles (shared nodes, gathered pseudo-residuals) in one L2.
+// A permutation of 0..nb-1; only locality depends on it.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int x = b % NUM_XCD, k = b / NUM_XCD;
+  const int q = nb / NUM_XCD, r = nb % NUM_XCD;
+  return x * q + (x < r ? x : r) + k;
+}
+
 }  // namespace adcgpu
