@@ -256,7 +256,7 @@ __device__ void jcg_parcon(ItLocal& L, double dtnrm) {
 
 __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   __shared__ ItLocal L;
-  __shared__ double lds[4];
+  __shared__ double lds[12];
   __shared__ double tri1[JCG_MAX_ITMAX], tri2[JCG_MAX_ITMAX];
   const int np = a.n;
   const int t = threadIdx.x;
@@ -339,21 +339,45 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         dc[r] = at(a.rhs, i);
       }
     }
+    // the next slot's column index and coefficient are loaded while this
+    // slot's w and u are gathered
 #pragma unroll
     for (int h = 0; h < RED_PER_THREAD; h += 4) {
       int jm = 1;
+      unsigned cn[4];
+      double kn[4];
 #pragma unroll
-      for (int r = h; r < h + 4; ++r) jm = NN(r) > jm ? NN(r) : jm;
+      for (int r = h; r < h + 4; ++r) {
+        jm = NN(r) > jm ? NN(r) : jm;
+        const unsigned q = (unsigned)(np + base + RED_THREADS * r);
+        cn[r - h] = 1 < NN(r) ? (unsigned)at(a.col, q) : 0u;
+        kn[r - h] = 1 < NN(r) ? at(a.coef, q) : 0.0;
+      }
       for (int j = 1; j < jm; ++j) {
+        unsigned cc[4];
+        double kc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          cc[k] = cn[k];
+          kc[k] = kn[k];
+        }
+        if (j + 1 < jm) {
+          const unsigned q0 = (unsigned)((j + 1) * np + base);
+#pragma unroll
+          for (int r = h; r < h + 4; ++r) {
+            if (j + 1 < NN(r)) {
+              cn[r - h] = (unsigned)at(a.col, q0 + RED_THREADS * r);
+              kn[r - h] = at(a.coef, q0 + RED_THREADS * r);
+            }
+          }
+        }
+        const unsigned q0 = (unsigned)(j * np + base);
 #pragma unroll
         for (int r = h; r < h + 4; ++r) {
-          const int i = base + RED_THREADS * r;
           if (j < NN(r)) {
-            const unsigned q = (unsigned)(j * np + i);
-            const unsigned col = (unsigned)at(a.col, q);
-            const double cs = at(a.coef, q) * wi[r] * at(a.w, col);
-            at(a.cs, q) = cs;
-            dc[r] = dc[r] - cs * at(a.u, col);
+            const double cs = kc[r - h] * wi[r] * at(a.w, cc[r - h]);
+            at(a.cs, q0 + RED_THREADS * r) = cs;
+            dc[r] = dc[r] - cs * at(a.u, cc[r - h]);
           }
         }
       }
@@ -457,9 +481,8 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       r1 = r1 + pc[1];
       r2 = r2 + pc[2];
     }
-    const double delnnm = block_tree(r0, lds);
-    const double dtnrm = block_tree(r1, lds);
-    const double unorm_s = block_tree(r2, lds);
+    block_tree3(r0, r1, r2, lds);
+    const double delnnm = r0, dtnrm = r1, unorm_s = r2;
     if (t == 0) {
       L.in = in;
       L.cme = a.saved[6];
@@ -492,22 +515,33 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
 
   // ---- unscal (src/itpackv.F:2424-2479), ETAS and ETA2 (src/gwce.F:2006-2008)
   if (!eig_block) {
+    double ri[RED_PER_THREAD];
+    int jm = 1;
 #pragma unroll
     for (int r = 0; r < RED_PER_THREAD; ++r) {
       const int i = base + RED_THREADS * r;
+      ri[r] = 1.0;
       if (i < np) {
         const double wi = at(a.w, i);
         const double u = (halted ? uc[r] : up[r]) * wi;
-        const double ri = 1.0 / wi;
-        for (int j = 1; j < NN(r); ++j) {
-          const unsigned q = (unsigned)(j * np + i);
-          at(a.coef, q) = at(a.cs, q) * ri * (1.0 / at(a.w, (unsigned)at(a.col, q)));
-        }
-        at(a.coef, i) = ri * ri;
-        at(a.rhs, i) = at(a.rhs, i) * ri;
+        ri[r] = 1.0 / wi;
+        at(a.coef, i) = ri[r] * ri[r];
+        at(a.rhs, i) = at(a.rhs, i) * ri[r];
         at(a.u, i) = u;
         if (a.eta2)
           at(a.eta2, i) = (double)at(a.nodecode, i) * (u + 0.0) + at(a.eta1, i);
+      }
+      jm = NN(r) > jm ? NN(r) : jm;
+    }
+    // off-diagonals, four rows per slot so that their gathers overlap
+    for (int j = 1; j < jm; ++j) {
+      const unsigned q0 = (unsigned)(j * np + base);
+#pragma unroll
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        if (j < NN(r)) {
+          const unsigned q = q0 + RED_THREADS * r;
+          at(a.coef, q) = at(a.cs, q) * ri[r] * (1.0 / at(a.w, (unsigned)at(a.col, q)));
+        }
       }
     }
   }

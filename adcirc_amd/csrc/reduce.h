@@ -30,6 +30,26 @@ __device__ __forceinline__ double block_tree(double v, double* lds) {
   return r;
 }
 
+// Cross-chunk stage for three values at once (the same tree as block_tree
+// applied to each); results valid in thread 0.  `lds` holds 12 doubles.
+__device__ __forceinline__ void block_tree3(double& a, double& b, double& c, double* lds) {
+  const int t = threadIdx.x;
+  if (t < 256) {
+    const double wa = wave_sum(a), wb = wave_sum(b), wc = wave_sum(c);
+    if ((t & 63) == 0) {
+      lds[t >> 6] = wa;
+      lds[4 + (t >> 6)] = wb;
+      lds[8 + (t >> 6)] = wc;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    a = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    b = (lds[4] + lds[5]) + (lds[6] + lds[7]);
+    c = (lds[8] + lds[9]) + (lds[10] + lds[11]);
+  }
+}
+
 // Chunk stage: reduces one value per thread of a 512-thread group (threads
 // g*512 .. g*512+511 of the block, g = group); result valid in the group's
 // first thread.  `lds` holds 8 doubles per group; every thread of the block
