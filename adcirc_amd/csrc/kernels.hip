@@ -499,12 +499,17 @@ __global__ __launch_bounds__(1024) void k_wd_scan(const WdCand* __restrict__ can
   }
   __syncthreads();
   if (t == 0) {
+    // the previous candidate's M stays in a register: along a shoreline most
+    // predecessors are the candidate just before
+    double mprev = 0.0;
+#pragma unroll 4
     for (int k = 0; k < n; ++k) {
       double m = M[k];
       const int qa = pa[k], qp = pp[k];
-      if (qa >= 0) m = fmax(m, M[qa]);
-      if (qp >= 0) m = fmax(m, M[qp]);
+      if (qa >= 0) m = fmax(m, qa == k - 1 ? mprev : M[qa]);
+      if (qp >= 0) m = fmax(m, qp == k - 1 ? mprev : M[qp]);
       M[k] = m;
+      mprev = m;
     }
   }
   __syncthreads();

@@ -84,6 +84,9 @@ def main() -> int:
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="time the steps without HIP events around the JCG launches "
+                         "(no roofline entry)")
     ap.add_argument("--dump", metavar="PATH",
                     help="write the case for the native driver adcirc_amd/adcgpu_run_case "
                          "(warmup + steps steps of forcing) and exit without using the GPU")
@@ -121,7 +124,7 @@ def main() -> int:
         done += k
         log(f"  warmup {done}/{a.warmup} steps, {time.perf_counter() - t0:.2f} s")
     g.synchronize()
-    g.profile(True)
+    g.profile(not a.no_kernel_timing)
     log(f"timed {a.steps} steps")
     # the timed steps are issued in chunks of up to 200 (each chunk ends with a
     # stream synchronisation inside the library) so that progress is visible

@@ -57,6 +57,17 @@ def test_coastal_mesh_bitexact():
     _run_bitexact(d, case.run, 120, check_every=10)
 
 
+def test_coastal_large_mesh_bitexact():
+    """A 400k-node coastal mesh (about 200 JCG chunks, so every XCD range
+    holds several blocks and the wetting front carries many candidates):
+    bit-exact vs oracle over the first steps."""
+    from gpu_helpers import case_setup
+    case = meshgen.coastal(400000)
+    d = case_setup(case)
+    log(f"case coastal {int(d['np'])} nodes: 12 steps bit-exact vs oracle")
+    _run_bitexact(d, case.run, 12, check_every=4)
+
+
 def test_basin_full_day_resident_vs_reference(basin_golden):
     """43,200 steps through the resident run (one launch sequence per step,
     elevation forcing from a precomputed table): final state within the
