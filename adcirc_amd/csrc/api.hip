@@ -74,8 +74,7 @@ struct adcgpu_ctx {
   double *contrib = nullptr, *cx = nullptr, *cy = nullptr, *ep_part = nullptr, *scal = nullptr;
   double *eta_bc = nullptr, *eta_bc_table = nullptr;
   int eta_bc_table_steps = 0;
-  WdCand* cand = nullptr;
-  int cand_cap = 0;
+  WdWork wd{};
   int* flags = nullptr;
   // JCG work
   double *cs = nullptr, *jd = nullptr, *jw = nullptr;
@@ -410,8 +409,12 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   A(&c->mju, np);
   A(&c->contrib, 3 * (size_t)ne); A(&c->cx, 3 * (size_t)ne); A(&c->cy, 3 * (size_t)ne);
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
-  c->cand_cap = WD_SORT_CAP;
-  A(&c->cand, c->cand_cap);
+  {
+    const size_t nb = (size_t)(ne + 255) / 256;
+    A(&c->wd.mask, nb * 4); A(&c->wd.bcnt, nb); A(&c->wd.bofs, nb);
+    A(&c->wd.tkc, ne); A(&c->wd.lc, ne); A(&c->wd.m0, ne); A(&c->wd.pred, ne);
+    A(&c->wd.wnode, ne); A(&c->wd.M, ne);
+  }
   A(&c->flags, F_COUNT); A(&c->jpart, (size_t)c->nch * 4); A(&c->jsaved, 8);
   A(&c->jbar, JCG_BAR_WORDS);
   if (!rc) {
@@ -421,13 +424,13 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   if (!rc) rc = setup_jcg_grid(c);
   if (!rc) {
     // the LHS row kernel keeps MNEI x 512 doubles of LDS per block; the
-    // wetting scan keeps its candidates in LDS
+    // wetting propagation stages candidates through LDS
     hipError_t e = hipFuncSetAttribute((const void*)k_lhs_rows,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        mnei * RED_THREADS * (int)sizeof(double));
     if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)k_wd_scan, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              WD_SCAN_LDS);
+      e = hipFuncSetAttribute((const void*)k_wd_prop, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              WD_PROP_LDS);
     if (e != hipSuccess) rc = fail(ADCGPU_E_UNSUPPORTED, std::string("LDS request: ") +
                                                          hipGetErrorString(e));
   }
@@ -646,10 +649,14 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
   }
   hipLaunchKernelGGL(k_wd_d1, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->flags, c->prm.h0, c->habsmin);
-  WdPrm q{c->hoff, c->prm.velmin, c->iflinbf, c->ifnlbf, c->tkmax, c->cand_cap};
-  hipLaunchKernelGGL(k_wd_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), q, c->cand, c->flags);
-  hipLaunchKernelGGL(k_wd_scan, dim3(1), dim3(1024), WD_SCAN_LDS, c->stream, c->cand, c->flags,
+  WdPrm q{c->hoff, c->prm.velmin, c->iflinbf, c->ifnlbf, c->tkmax};
+  const int nb = nblk(ne, 256);
+  hipLaunchKernelGGL(k_wd_elem, dim3(nb), dim3(256), 0, c->stream, c->mesh(), c->state(), q,
+                     c->wd);
+  hipLaunchKernelGGL(k_wd_index, dim3(1), dim3(1024), 0, c->stream, c->wd.bcnt, nb, c->wd.bofs,
+                     c->flags);
+  hipLaunchKernelGGL(k_wd_pred, dim3(nb), dim3(256), 0, c->stream, c->mesh(), c->state(), c->wd);
+  hipLaunchKernelGGL(k_wd_prop, dim3(1), dim3(256), WD_PROP_LDS, c->stream, c->wd, c->flags,
                      c->tk);
   // cx is free until the momentum step: it holds NCEle*AREAS/2 per incidence slot
   hipLaunchKernelGGL(k_wd_area, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
@@ -690,7 +697,6 @@ int adcgpu_last_numitr(adcgpu_ctx* c, int* numitr, int* ier) {
   if (numitr) *numitr = f[F_NUMITR];
   if (ier) *ier = f[F_IER];
   if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
-  if (f[F_ERROR] & ERR_CAPACITY) return fail(ADCGPU_E_CAPACITY, "wetting candidate capacity");
   if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
   return 0;
 }
@@ -753,7 +759,6 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   c->run_rebuilds = f[F_NREBUILD];
   if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
   if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
-  if (f[F_ERROR] & ERR_CAPACITY) return fail(ADCGPU_E_CAPACITY, "wetting candidate capacity");
   return 0;
 }
 
@@ -773,8 +778,9 @@ int adcgpu_profile(adcgpu_ctx* c, int enable) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ev_used = 0;
-  c->profile = enable;
-  if (enable) return grow_events(c, std::max<size_t>(c->jcg_events.size(), 1024));
+  c->profile = enable != 0;
+  if (enable) return grow_events(c, std::max<size_t>(c->jcg_events.size(),
+                                                     (size_t)std::max(enable, 1024)));
   return 0;
 }
 

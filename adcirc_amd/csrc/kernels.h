@@ -20,13 +20,16 @@ constexpr int JCG_BAR_STRIDE = 64;       // words between barrier counters (256 
 constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BAR_STRIDE
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
 constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
-constexpr int WD_SORT_CAP = 4096;    // wetting candidates per step
-constexpr int WD_SCAN_LDS = WD_SORT_CAP * (4 * 4 + 8);  // dynamic LDS of k_wd_scan
+// wetting propagation (k_wd_prop): candidates staged through LDS in chunks,
+// the running maxima of the most recent WD_RING candidates kept in an LDS ring
+constexpr int WD_CHUNK = 2048;
+constexpr int WD_RING = 8192;   // power of two
+constexpr int WD_PROP_LDS = WD_RING * 8 + WD_CHUNK * (8 + 8 + 8);
 
 // device flag slots
 enum { F_NCCHANGE = 0, F_NUMITR = 2, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_NREBUILD,
        F_COUNT = 16 };
-enum { ERR_DIAG = 1, ERR_CAPACITY = 2, ERR_BARRIER = 4 };
+enum { ERR_DIAG = 1, ERR_BARRIER = 4 };
 // device scalar slots
 enum { S_EP = 0, S_COUNT = 8 };
 
@@ -81,12 +84,21 @@ struct MomPrm {
 
 struct WdPrm {
   double hoff, velmin, iflinbf, ifnlbf, tkmax;
-  int cand_cap;
 };
 
-struct WdCand {
-  int e, a, p;
-  double tk;
+// Wetting candidates of one step (src/wetdry.F:368-475 and its two twins).
+// Candidate k is the k-th candidate element in element order; every array is
+// sized for the worst case of one candidate per element.
+struct WdWork {
+  unsigned long long* mask;  // [ceil(ne/64)] bit e%64 of word e/64: element e is a candidate
+  int* bcnt;                 // [ceil(ne/256)] candidates per 256-element tile
+  int* bofs;                 // [ceil(ne/256)] candidates in earlier tiles
+  double* tkc;               // [ne] TK_TMP(NM123) of candidate element e
+  int* lc;                   // [ne] local vertex of NM123 | local vertex of the partner << 2
+  double* m0;                // [ne] candidate k: TK_TMP and the entry TK of nodes it opens
+  int2* pred;                // [ne] candidate k: previous candidate on NM123 / partner, or -1
+  int2* wnode;               // [ne] candidate k: NM123 / partner if k is its last candidate, or -1
+  double* M;                 // [ne] candidate k: TK of both nodes after k
 };
 
 struct HollandPrm {
@@ -134,8 +146,10 @@ __global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const d
                            const double* scal);
 __global__ void k_jcg(JcgArgs a);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
-__global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdCand* cand, int* flags);
-__global__ void k_wd_scan(const WdCand* cand, int* flags, double* tk);
+__global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);
+__global__ void k_wd_index(const int* bcnt, int nb, int* bofs, int* flags);
+__global__ void k_wd_pred(DevMesh m, DevState s, WdWork w);
+__global__ void k_wd_prop(WdWork w, const int* flags, double* tk);
 __global__ void k_wd_area(DevMesh m, DevState s, double* area_slot);
 __global__ void k_wd_d2(DevMesh m, DevState s, const double* area_slot, int* flags, double ifnlfa);
 __global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim);

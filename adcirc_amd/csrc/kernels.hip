@@ -352,7 +352,6 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
 // D1 (src/wetdry.F:234-285); also NNODECODE := NODECODE for the W1 writes.
 __global__ __launch_bounds__(256) void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin) {
   int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i == 0) flags[F_NCAND] = 0;  // wetting candidates of this step (k_wd_elem)
   if (i >= m.np) return;
   int nc = s.nodecode[i];
   if (nc == 1) {
@@ -369,161 +368,215 @@ __global__ __launch_bounds__(256) void k_wd_d1(DevMesh m, DevState s, int* flags
 
 // W1 candidates (src/wetdry.F:332-818) and DE1 (src/wetdry.F:1110-1233).
 // Candidate elements whose steady-flow velocity exceeds VELMIN wet their dry
-// node here (an idempotent store); their friction updates are order
-// dependent and are appended for the ordered scan below.
-__global__ __launch_bounds__(256) void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdCand* __restrict__ cand,
-                          int* __restrict__ flags) {
-  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e >= m.ne) return;
-  const int n[3] = {m.nm[3 * e], m.nm[3 * e + 1], m.nm[3 * e + 2]};
-  s.noffold[e] = s.noff[e];
-  int noff = 1;
-  const double e1 = s.eta2[n[0]], e2 = s.eta2[n[1]], e3 = s.eta2[n[2]];
-  const double h1 = m.dp[n[0]] + e1, h2 = m.dp[n[1]] + e2, h3 = m.dp[n[2]] + e3;
-  const int c0 = s.nodecode[n[0]], c1 = s.nodecode[n[1]], c2 = s.nodecode[n[2]];
-  if (c0 + c1 + c2 == 2) {
-    int a = -1, b = -1, c = -1;
-    if (c0 == 1 && c1 == 1) { a = 0; b = 1; c = 2; }
-    else if (c1 == 1 && c2 == 1) { a = 1; b = 2; c = 0; }
-    else if (c2 == 1 && c0 == 1) { a = 2; b = 0; c = 1; }
-    const double ht[3] = {h1, h2, h3};
-    if (a >= 0 && ht[a] >= q.hoff && ht[b] >= q.hoff) {
-      const int na = n[a], nb = n[b], nc = n[c];
-      int nm123 = na;
-      if (s.eta2[nb] > s.eta2[na]) nm123 = nb;
-      const double dy = m.y[nc] - m.y[nm123], dx = m.x[nc] - m.x[nm123];
-      const double deldist = __builtin_sqrt(dy * dy + dx * dx);
-      double deleta = s.eta2[nm123] - s.eta2[nc];
-      if (deleta < 0.0) deleta = 0.0;
-      const double H1 = s.eta2[nm123] + m.dp[nm123];
-      const double fr = s.fric[nm123];
-      const double tkwet = fmax(fr * (q.iflinbf + (q.velmin / H1) * (q.ifnlbf + 0.0)), 0.0001);
-      const double vel = m.g * (deleta / deldist) / tkwet;
-      if (vel > q.velmin) {
-        s.nnodecode[nc] = 1;
-        const double tkt = fmin(fr * (q.iflinbf + (1.0 * vel / H1) * (q.ifnlbf + 0.0)), q.tkmax);
-        const int partner = (nm123 == nb) ? na : nb;
-        const int slot = atomicAdd(&flags[F_NCAND], 1);
-        if (slot < q.cand_cap) {
-          cand[slot].e = e;
-          cand[slot].a = nm123;
-          cand[slot].p = partner;
-          cand[slot].tk = tkt;
+// node here (an idempotent store).  Their friction updates depend on element
+// order: each candidate leaves its TK_TMP value and local vertices at its
+// element and sets its bit in the candidate mask (one word per wave), and the
+// block's candidate count goes to bcnt[tile] for the compaction below.
+__global__ __launch_bounds__(256) void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w) {
+  __shared__ int wcount[4];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int e = tile * 256 + threadIdx.x;
+  bool cand = false;
+  if (e < m.ne) {
+    const int n[3] = {m.nm[3 * e], m.nm[3 * e + 1], m.nm[3 * e + 2]};
+    s.noffold[e] = s.noff[e];
+    int noff = 1;
+    const double e1 = s.eta2[n[0]], e2 = s.eta2[n[1]], e3 = s.eta2[n[2]];
+    const double h1 = m.dp[n[0]] + e1, h2 = m.dp[n[1]] + e2, h3 = m.dp[n[2]] + e3;
+    const int c0 = s.nodecode[n[0]], c1 = s.nodecode[n[1]], c2 = s.nodecode[n[2]];
+    if (c0 + c1 + c2 == 2) {
+      int a = -1, b = -1, c = -1;
+      if (c0 == 1 && c1 == 1) { a = 0; b = 1; c = 2; }
+      else if (c1 == 1 && c2 == 1) { a = 1; b = 2; c = 0; }
+      else if (c2 == 1 && c0 == 1) { a = 2; b = 0; c = 1; }
+      const double ht[3] = {h1, h2, h3};
+      if (a >= 0 && ht[a] >= q.hoff && ht[b] >= q.hoff) {
+        const int na = n[a], nb = n[b], nc = n[c];
+        int l123 = a;
+        if (s.eta2[nb] > s.eta2[na]) l123 = b;
+        const int nm123 = n[l123];
+        const double dy = m.y[nc] - m.y[nm123], dx = m.x[nc] - m.x[nm123];
+        const double deldist = __builtin_sqrt(dy * dy + dx * dx);
+        double deleta = s.eta2[nm123] - s.eta2[nc];
+        if (deleta < 0.0) deleta = 0.0;
+        const double H1 = s.eta2[nm123] + m.dp[nm123];
+        const double fr = s.fric[nm123];
+        const double tkwet = fmax(fr * (q.iflinbf + (q.velmin / H1) * (q.ifnlbf + 0.0)), 0.0001);
+        const double vel = m.g * (deleta / deldist) / tkwet;
+        if (vel > q.velmin) {
+          s.nnodecode[nc] = 1;
+          w.tkc[e] = fmin(fr * (q.iflinbf + (1.0 * vel / H1) * (q.ifnlbf + 0.0)), q.tkmax);
+          w.lc[e] = l123 | ((l123 == b ? a : b) << 2);
+          cand = true;
         }
       }
     }
+    const double HOFF = q.hoff;
+    if ((e1 >= e2) && (e2 > e3)) if ((h1 < HOFF) || (h2 < HOFF)) noff = 0;
+    if ((e2 >= e3) && (e3 > e1)) if ((h2 < HOFF) || (h3 < HOFF)) noff = 0;
+    if ((e3 >= e1) && (e1 > e2)) if ((h3 < HOFF) || (h1 < HOFF)) noff = 0;
+    if ((e1 >= e3) && (e3 > e2)) if ((h1 < HOFF) || (h3 < HOFF)) noff = 0;
+    if ((e2 >= e1) && (e1 > e3)) if ((h2 < HOFF) || (h1 < HOFF)) noff = 0;
+    if ((e3 >= e2) && (e2 > e1)) if ((h3 < HOFF) || (h2 < HOFF)) noff = 0;
+    s.noff[e] = noff;
   }
-  const double HOFF = q.hoff;
-  if ((e1 >= e2) && (e2 > e3)) if ((h1 < HOFF) || (h2 < HOFF)) noff = 0;
-  if ((e2 >= e3) && (e3 > e1)) if ((h2 < HOFF) || (h3 < HOFF)) noff = 0;
-  if ((e3 >= e1) && (e1 > e2)) if ((h3 < HOFF) || (h1 < HOFF)) noff = 0;
-  if ((e1 >= e3) && (e3 > e2)) if ((h1 < HOFF) || (h3 < HOFF)) noff = 0;
-  if ((e2 >= e1) && (e1 > e3)) if ((h2 < HOFF) || (h1 < HOFF)) noff = 0;
-  if ((e3 >= e2) && (e2 > e1)) if ((h3 < HOFF) || (h2 < HOFF)) noff = 0;
-  s.noff[e] = noff;
+  // every lane of the block reaches the ballot; a wave covers elements
+  // 64*word .. 64*word+63
+  const unsigned long long bits = __ballot(cand);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    if (tile * 256 + wave * 64 < m.ne) w.mask[tile * 4 + wave] = bits;
+    wcount[wave] = __popcll(bits);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) w.bcnt[tile] = (wcount[0] + wcount[1]) + (wcount[2] + wcount[3]);
 }
 
-// Ordered W1 friction updates (src/wetdry.F:447-472).  The reference visits
-// the wetting candidates in ascending element order; candidate k sets
+// Exclusive prefix of the per-tile candidate counts (one block of 1024) and
+// the step's candidate count.
+__global__ __launch_bounds__(1024) void k_wd_index(const int* __restrict__ bcnt, int nb,
+                                                   int* __restrict__ bofs, int* flags) {
+  __shared__ int wtot[16];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int lo = min(t * per, nb), hi = min(lo + per, nb);
+  int sum = 0;
+#pragma unroll 8
+  for (int i = lo; i < hi; ++i) sum += bcnt[i];
+  int incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if ((t & 63) >= off) incl += v;
+  }
+  if ((t & 63) == 63) wtot[t >> 6] = incl;
+  __syncthreads();
+  int base = 0;
+  for (int i = 0; i < (t >> 6); ++i) base += wtot[i];
+  int run = base + incl - sum;
+#pragma unroll 8
+  for (int i = lo; i < hi; ++i) {
+    bofs[i] = run;
+    run += bcnt[i];
+  }
+  if (t == 1023) flags[F_NCAND] = base + incl;
+}
+
+// Candidate index of candidate element e (element order); `word` = mask[e/64].
+__device__ __forceinline__ int wd_index(const WdWork& w, int e, unsigned long long word) {
+  const int tile = e >> 8, wi = (e >> 6) & 3;
+  int k = w.bofs[tile];
+  for (int j = 0; j < wi; ++j) k += __popcll(w.mask[tile * 4 + j]);
+  return k + __popcll(word & ((1ull << (e & 63)) - 1ull));
+}
+
+// Does candidate code 3*e2+k2 (element e2, local vertex k2) update the TK of
+// that vertex, i.e. is e2 a candidate and the vertex one of its two wet nodes?
+__device__ __forceinline__ bool wd_touches(const WdWork& w, int e2, int k2) {
+  if (!((w.mask[e2 >> 6] >> (e2 & 63)) & 1ull)) return false;
+  const int lc = w.lc[e2];
+  return k2 == (lc & 3) || k2 == ((lc >> 2) & 3);
+}
+
+// Ordered W1 friction updates (src/wetdry.F:447-472), links.  The reference
+// visits the candidates in ascending element order; candidate k sets
 // TK(NM123) = MAX(TK_TMP, TK(NM123)), gives the partner the larger of the two
 // and takes it back if the partner was larger, so afterwards both of its
 // nodes hold M_k = max(TK_TMP_k, their values before k).  A node's value
-// before k is M_j of the last earlier candidate j on that node (values only
-// grow), or its value on entry.  So: sort the candidates by element id
-// (bitonic, LDS), find each one's predecessors on its two nodes (parallel),
-// form M_k in element order (one lane, LDS only), and give each node the M
-// of its last candidate (parallel).  TK values are non-negative and finite,
-// so max is exact and order-free.  One block of 1024; dynamic LDS of
-// WD_SCAN_LDS bytes.
-__global__ __launch_bounds__(1024) void k_wd_scan(const WdCand* __restrict__ cand, int* flags,
-                                                  double* __restrict__ tk) {
-  extern __shared__ int wd_lds[];
-  int* pa = wd_lds;                    // sort key, then predecessor on node a
-  int* pp = pa + WD_SORT_CAP;          // sort index, then predecessor on node p
-  int* na = pp + WD_SORT_CAP;          // node a of sorted candidate k
-  int* np_ = na + WD_SORT_CAP;         // node p of sorted candidate k
-  double* M = reinterpret_cast<double*>(np_ + WD_SORT_CAP);
-  const int n = flags[F_NCAND];
-  const int t = threadIdx.x, nt = blockDim.x;
-  if (n == 0) return;
-  if (n > WD_SORT_CAP) {
-    if (t == 0) atomicOr(&flags[F_ERROR], ERR_CAPACITY);
-    return;
-  }
-  int P = 1;
-  while (P < n) P <<= 1;
-  for (int i = t; i < P; i += nt) {
-    pa[i] = i < n ? cand[i].e : 0x7fffffff;
-    pp[i] = i;
-  }
-  __syncthreads();
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = t; i < P; i += nt) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool up = (i & k) == 0;
-          const int ki = pa[i], kl = pa[l];
-          if ((ki > kl) == up) {
-            pa[i] = kl; pa[l] = ki;
-            const int x = pp[i]; pp[i] = pp[l]; pp[l] = x;
-          }
-        }
+// before k is M_j of the nearest earlier candidate j on that node, or its
+// value on entry.  The node incidence lists are sorted by element, so j and
+// "k is the node's last candidate" are found by walking the node's list
+// backward and forward from k's own slot.  Entry values fold into m0.
+__global__ __launch_bounds__(256) void k_wd_pred(DevMesh m, DevState s, WdWork w) {
+  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const unsigned long long word = w.mask[e >> 6];
+  if (!((word >> (e & 63)) & 1ull)) return;
+  const int k = wd_index(w, e, word);
+  const int lc = w.lc[e];
+  double v = w.tkc[e];
+  int pred[2], wn[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int l = (lc >> (2 * j)) & 3;
+    const int x = m.nm[3 * e + l];
+    const int pos = m.slot[3 * e + l];
+    int pk = -1;
+    for (int r = pos - 1; r >= m.inc_ptr[x]; --r) {
+      const int code = m.inc[r], e2 = code / 3;
+      if (wd_touches(w, e2, code - 3 * e2)) {
+        pk = wd_index(w, e2, w.mask[e2 >> 6]);
+        break;
       }
-      __syncthreads();
     }
-  }
-  for (int k = t; k < n; k += nt) {
-    const WdCand c = cand[pp[k]];
-    na[k] = c.a;
-    np_[k] = c.p;
-    M[k] = c.tk;
-  }
-  __syncthreads();
-  // predecessors: the nearest earlier sorted candidate on the same node
-  for (int k = t; k < n; k += nt) {
-    const int a = na[k], p = np_[k];
-    int qa = -1, qp = -1;
-    for (int j = k - 1; j >= 0 && (qa < 0 || qp < 0); --j) {
-      const int x = na[j], y = np_[j];
-      if (qa < 0 && (x == a || y == a)) qa = j;
-      if (qp < 0 && (x == p || y == p)) qp = j;
+    bool last = true;
+    const int end = m.inc_ptr[x + 1];
+    for (int r = pos + 1; r < end; ++r) {
+      const int code = m.inc[r], e2 = code / 3;
+      if (wd_touches(w, e2, code - 3 * e2)) {
+        last = false;
+        break;
+      }
     }
-    pa[k] = qa;
-    pp[k] = qp;
-    // entry values of nodes with no earlier candidate
-    double m = M[k];
-    if (qa < 0) m = fmax(m, tk[a]);
-    if (qp < 0) m = fmax(m, tk[p]);
-    M[k] = m;
+    if (pk < 0) v = fmax(v, s.tk[x]);
+    pred[j] = pk;
+    wn[j] = last ? x : -1;
   }
-  __syncthreads();
-  if (t == 0) {
-    // the previous candidate's M stays in a register: along a shoreline most
-    // predecessors are the candidate just before
-    double mprev = 0.0;
-#pragma unroll 4
-    for (int k = 0; k < n; ++k) {
-      double m = M[k];
-      const int qa = pa[k], qp = pp[k];
-      if (qa >= 0) m = fmax(m, qa == k - 1 ? mprev : M[qa]);
-      if (qp >= 0) m = fmax(m, qp == k - 1 ? mprev : M[qp]);
-      M[k] = m;
-      mprev = m;
+  w.m0[k] = v;
+  w.pred[k] = make_int2(pred[0], pred[1]);
+  w.wnode[k] = make_int2(wn[0], wn[1]);
+}
+
+// Ordered W1 friction updates, values: M_k = max(m0_k, M of k's predecessors)
+// in candidate order, then each node takes the M of its last candidate.  TK
+// values are finite, so max is exact and order-free.  The
+// candidates pass through LDS in chunks of WD_CHUNK; one lane forms M along
+// the chunk (the previous candidate's M stays in a register, the last WD_RING
+// values in an LDS ring, older ones are read back from w.M) and the block
+// writes the chunk's TK values.  One block of 256; dynamic LDS of
+// WD_PROP_LDS bytes.
+__global__ __launch_bounds__(256) void k_wd_prop(WdWork w, const int* flags,
+                                                 double* __restrict__ tk) {
+  extern __shared__ double wp_lds[];
+  double* ring = wp_lds;                                          // [WD_RING]
+  double* cm0 = ring + WD_RING;                                   // [WD_CHUNK]
+  int2* cpred = reinterpret_cast<int2*>(cm0 + WD_CHUNK);          // [WD_CHUNK]
+  int2* cwn = cpred + WD_CHUNK;                                   // [WD_CHUNK]
+  const int n = flags[F_NCAND];
+  const int t = threadIdx.x;
+  double mprev = 0.0;  // M of the previous candidate (lane 0)
+  for (int base = 0; base < n; base += WD_CHUNK) {
+    const int cn = min(WD_CHUNK, n - base);
+    for (int i = t; i < cn; i += blockDim.x) {
+      cm0[i] = w.m0[base + i];
+      cpred[i] = w.pred[base + i];
+      cwn[i] = w.wnode[base + i];
     }
-  }
-  __syncthreads();
-  // each node takes the M of its last candidate
-  for (int k = t; k < n; k += nt) {
-    const int a = na[k], p = np_[k];
-    bool la = true, lp = true;
-    for (int j = k + 1; j < n && (la || lp); ++j) {
-      const int x = na[j], y = np_[j];
-      if (x == a || y == a) la = false;
-      if (x == p || y == p) lp = false;
+    __syncthreads();
+    if (t == 0) {
+      for (int i = 0; i < cn; ++i) {
+        const int k = base + i;
+        double v = cm0[i];
+        const int2 q = cpred[i];
+        if (q.x >= 0)
+          v = fmax(v, q.x == k - 1 ? mprev
+                                   : (q.x >= k - WD_RING ? ring[q.x & (WD_RING - 1)] : w.M[q.x]));
+        if (q.y >= 0)
+          v = fmax(v, q.y == k - 1 ? mprev
+                                   : (q.y >= k - WD_RING ? ring[q.y & (WD_RING - 1)] : w.M[q.y]));
+        ring[k & (WD_RING - 1)] = v;
+        w.M[k] = v;
+        mprev = v;
+      }
     }
-    if (la) tk[a] = M[k];
-    if (lp) tk[p] = M[k];
+    __syncthreads();
+    for (int i = t; i < cn; i += blockDim.x) {
+      const double v = ring[(base + i) & (WD_RING - 1)];
+      const int2 x = cwn[i];
+      if (x.x >= 0) tk[x.x] = v;
+      if (x.y >= 0) tk[x.y] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -706,20 +759,34 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
   mx = dnci * (mx + WSX + (1.0 - VXX) * u1 - VXY * v1);
   my = dnci * (my + WSY + (1.0 - VYY) * v1 - VYX * u1);
   double A0 = 1.0 + VXX * dnci, A1 = 1.0 + VYY * dnci, A2 = VXY * dnci, A3 = VYX * dnci;
-  // land boundary constraints in ME2GW order (src/momentum.F:810-870); with
-  // TKM = (TK, TK, 0) the NORM2 test of :816 always selects its first branch
+  // land boundary constraints in ME2GW order (src/momentum.F:810-870).  The
+  // isotropy test of :816 is ABS(NORM2(TKM) - SQRT(2)*TK) < EPSILON with
+  // TKM = (TK, TK, 0): flang's scaled NORM2 gives |TK|*SQRT(2) exactly, so
+  // the first branch is taken unless TK is negative (a node whose total
+  // depth went negative) and not tiny.
+  const double n2tkm = fabs(tk) * __builtin_sqrt(2.0);
+  const bool isotropic = fabs(n2tkm - __builtin_sqrt(2.0) * tk) < 2.220446049250313e-16;
   for (int p = m.vbc_ptr[i]; p < m.vbc_ptr[i + 1]; ++p) {
     const int I = m.vbc[p];
     const int code = m.lbcodei[I];
     const double CS = m.csii[I], SI = m.siii[I];
     const double veln = -0.0 / s.h2[i];
     if (code >= 0 && code <= 9) {
-      mx = (SI * mx - CS * my - veln * A2) * dnci;
-      my = veln * A0 * dnci;
-      A2 = -CS * A0;
-      A3 = -A2;
-      A0 = SI * A0;
-      A1 = A0;
+      if (isotropic) {
+        mx = (SI * mx - CS * my - veln * A2) * dnci;
+        my = veln * A0 * dnci;
+        A2 = -CS * A0;
+        A3 = -A2;
+        A0 = SI * A0;
+        A1 = A0;
+      } else {
+        mx = (SI * mx - CS * my) * dnci;
+        my = veln * dnci;
+        A0 = A0 * SI - A3 * CS;
+        A2 = A2 * SI - A1 * CS;
+        A3 = CS;
+        A1 = SI;
+      }
     } else if (code >= 10 && code <= 19) {
       mx = 0.0 * dnci;
       my = veln * dnci;

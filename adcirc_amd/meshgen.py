@@ -273,7 +273,9 @@ def coastal(target_nodes: int = 1_000_000, seed: int = 7, band: float = 10.0e3,
         # landfall near y = ly/2 after ~5.5 h at 5 m/s shoreward
         hol = (940.0, 1013.0, 40.0e3, 1.3, COAST_X - 100.0e3, ly / 2.0, 5.0, 0.0,
                1.0e-4, 0.1, 1.15, 0.8, 20.0)
-    run = RunConfig(dtdp=1.0, nsteps=21600, nolifa=2, drampelev=0.25, h0=0.05,
+    # TAU0 = -1: 0.020 on the shelf edge and overland (DP <= 10 m), 0.005
+    # deeper; a uniform 0.005 lets the storm-driven wetting front go unstable
+    run = RunConfig(dtdp=1.0, nsteps=21600, nolifa=2, drampelev=0.25, h0=0.05, tau0=-1.0,
                     velmin=0.01, holland=hol, amig=(M2_OMEGA, K1_OMEGA),
                     ff=(1.0, 1.0), face=(0.0, 0.0))
     neta = ob.shape[0]

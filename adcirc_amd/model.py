@@ -16,6 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, dp_, ip_
+from .prep import tau0_nodal
 
 DEG2RAD = 3.141592653589793 / 180.0
 
@@ -53,7 +54,7 @@ def c_layout(tables: Dict[str, np.ndarray], run, tau0var=None, evm=None, fric=No
         "nbd": i32(t["nbd"] if neta else np.zeros(1)), "nbv": i32(t["nbv"]),
         "lbcodei": i32(t["lbcodei"]), "csii": f64(t["csii"]), "siii": f64(t["siii"]),
         "me2gw": i32(t["me2gw"]),
-        "tau0var": f64(tau0var if tau0var is not None else np.full(npn, run.tau0)),
+        "tau0var": f64(tau0var if tau0var is not None else tau0_nodal(run.tau0, tables["dp"])),
         "evm": f64(evm if evm is not None else np.full(npn, run.eslm)),
         "fric": f64(fric if fric is not None else np.full(npn, run.cf)),
         "corif": f64(corif if corif is not None else np.full(npn, run.cori)),
@@ -263,8 +264,11 @@ class Model:
         v = np.ascontiguousarray(list(prm) + [0.0] * (13 - len(prm)), np.float64)
         check(self.lib.adcgpu_holland(self.h, t, _p(v, dp_)), "holland")
 
-    def profile(self, enable: bool = True) -> None:
-        check(self.lib.adcgpu_profile(self.h, int(enable)), "profile")
+    def profile(self, enable: bool = True, launches: int = 0) -> None:
+        """Start (enable, with event pairs for `launches` JCG launches made
+        ahead) or stop timing the JCG launches."""
+        n = max(int(launches), 1) if enable else 0
+        check(self.lib.adcgpu_profile(self.h, n), "profile")
 
     def profile_read(self):
         ms, n = C.c_double(), C.c_int64()

@@ -13,7 +13,8 @@ for bit since they are formed by the same operations):
 * CSR_POSITION (compute_csr_positions, src/mesh.F:5261-5282);
 * elevation-boundary list NBD and flow-boundary tables NBV, LBCODEI, CSII, SIII,
   ME2GW for external boundaries of types 0/10/20
-  (initializeBoundaries, src/mesh.F:1769-2922).
+  (initializeBoundaries, src/mesh.F:1769-2922);
+* the GWCE weighting TAU0 per node from fort.15's TAU0 (src/nodalattr.F:1858-1895).
 """
 from __future__ import annotations
 
@@ -177,6 +178,26 @@ def flow_boundaries(x: np.ndarray, y: np.ndarray, land, costset: float):
                     me2gw[0] = jgw
     return (np.array(nbv, np.int32), np.array(lbc, np.int32), np.array(csii),
             np.array(siii), np.array(me2gw, np.int32))
+
+
+def tau0_nodal(tau0: float, dp: np.ndarray) -> np.ndarray:
+    """TAU0VAR from fort.15's TAU0 without a nodal-attribute file
+    (src/nodalattr.F:1858-1895).  TAU0 >= 0 holds everywhere.  A negative
+    value selects a depth scheme (Tau0NodalValue, src/nodalattr.F:2084-2102):
+    -2 gives 0.005 at DP >= 200 m, 1/DP between 1 and 200 m and 1.0 above
+    1 m; any other negative value gives 0.020 at DP <= 10 m and 0.005 deeper.
+    The -2 scheme's 0.005 is a default-real literal, so it carries that
+    literal's single-precision rounding; the other constants are exact or
+    double."""
+    dp = np.asarray(dp, np.float64)
+    if tau0 >= 0.0:
+        return np.full(dp.shape[0], float(tau0))
+    if tau0 == -2.0:
+        deep = float(np.float32(0.005))
+        with np.errstate(divide="ignore"):
+            mid = 1.0 / dp
+        return np.where(dp >= 200.0, deep, np.where(dp >= 1.0, mid, 1.0))
+    return np.where(dp <= 10.0, 0.020, 0.005)
 
 
 def build_tables(mesh: Mesh, nolifa: int, h0: float, anginn: float = 110.0) -> Dict[str, np.ndarray]:

@@ -6,9 +6,10 @@ wet/dry, momentum, peak surge) on the whole mesh, inputs resident in HBM.
 Workload: the synthetic coastal mesh (Delaunay of seed-sampled points, ~1.0M
 nodes / ~2.0M elements, 5 km -> 100 m), tanh shelf 4000 m -> -3 m, wet/dry
 with H0 = 0.05 m, M2 + K1 on the open boundary, Holland vortex (Pc 940 hPa,
-Rmax 40 km, B 1.3, 5 m/s shoreward), dt = 1 s.
+Rmax 40 km, B 1.3, 5 m/s shoreward), dt = 1 s; by default the
+configuration's 6 simulated hours (21,600 steps: 30 warm-up, 21,570 timed).
 
-    python bench.py [--gpus 1] [--steps 600] [--warmup 60] [--nodes 1000000]
+    python bench.py [--gpus 1] [--steps 21570] [--warmup 30] [--nodes 1000000]
 
 Prints one JSON line.  The dominant kernel's roofline entry is computed from
 the JCG launches timed with HIP events on the library's stream; the CPU
@@ -50,13 +51,14 @@ def cpu_baseline(case, tables, eta2, nodecode, noff, budget_s: float):
     on the first steps of the same run."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib
+    from adcirc_amd import prep
 
     run = case.run
     npn = int(tables["np"])
     d = dict(tables)
     d["emo"] = np.asarray(case.emo, np.float64).ravel()
     d["efa"] = (np.asarray(case.efa, np.float64) * (3.141592653589793 / 180.0)).ravel()
-    d["tau0var"] = np.full(npn, run.tau0)
+    d["tau0var"] = prep.tau0_nodal(run.tau0, tables["dp"])
     d["evm"] = np.full(npn, run.eslm)
     d["fric"] = np.full(npn, run.cf)
     d["nodecode0"], d["noff0"], d["eta2_0"] = nodecode, noff, eta2
@@ -79,8 +81,10 @@ def cpu_baseline(case, tables, eta2, nodecode, noff, budget_s: float):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--warmup", type=int, default=60)
+    # the configuration's 6 simulated hours at dt = 1 s: 30 warm-up steps and
+    # the remaining 21,570 timed
+    ap.add_argument("--steps", type=int, default=21570)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -124,14 +128,14 @@ def main() -> int:
         done += k
         log(f"  warmup {done}/{a.warmup} steps, {time.perf_counter() - t0:.2f} s")
     g.synchronize()
-    g.profile(not a.no_kernel_timing)
+    g.profile(not a.no_kernel_timing, launches=a.steps + 16)
     log(f"timed {a.steps} steps")
     # the timed steps are issued in chunks of up to 200 (each chunk ends with a
     # stream synchronisation inside the library) so that progress is visible
     t0 = time.perf_counter()
     iters, done, rebuilds = 0, 0, 0
     while done < a.steps:
-        k = min(200, a.steps - done)
+        k = min(1000, a.steps - done)
         s0 = a.warmup + done
         iters += g.run_resident(s0 + 1, k, holland=run.holland, table=table[s0:s0 + k])
         rebuilds += g.last_rebuilds
@@ -147,6 +151,12 @@ def main() -> int:
     # algorithmic bytes of the JCG launches: k*(12 Z + 48 NP) + (12 Z + 64 NP)
     jcg_bytes = iters * (12.0 * Z + 48.0 * npn) + launches * (12.0 * Z + 64.0 * npn)
     achieved = jcg_bytes / (jcg_ms * 1e-3) / 1e9 if jcg_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "k_jcg_coastal1m_traffic.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath))
+        if (rec.get("nodes"), rec.get("warmup"), rec.get("steps")) == (npn, a.warmup, a.steps):
+            traffic = rec["traffic_bytes_per_launch"] / 1e9
     wet = int(g.get("nodecode").sum())
     etamax = g.get("etamax")
     ms_step = wall * 1e3 / a.steps
@@ -166,7 +176,8 @@ def main() -> int:
         "config": {
             "workload": f"coastal Delaunay mesh {npn} nodes / {int(tables['ne'])} elements, "
                         f"wet/dry H0=0.05, M2+K1, Holland vortex, dt=1 s, JCG tol 1e-8, "
-                        f"plain extrapolated start",
+                        f"plain extrapolated start, steps {a.warmup + 1}-{a.warmup + a.steps} "
+                        f"({(a.warmup + a.steps) * run.dtdp / 3600.0:.2f} simulated hours)",
             "nodes": npn, "elements": int(tables["ne"]), "dt": run.dtdp, "convcr": run.convcr,
             "warm_start": "plain",
             "jcg_blocks": shape[0], "jcg_resident_cap": shape[1], "jcg_chunks": shape[2],
@@ -178,7 +189,7 @@ def main() -> int:
         "peak_surge_max_m": float(etamax.max()),
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_unit": "GB/launch",
             "kernel": "k_jcg", "kernel_ms_avg": jcg_ms / max(launches, 1),
             "kernel_share_of_step": (jcg_ms / 1e3) / wall,
         },

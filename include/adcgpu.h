@@ -146,8 +146,10 @@ int64_t adcgpu_run_iterations(adcgpu_ctx *ctx);
 int64_t adcgpu_run_rebuilds(adcgpu_ctx *ctx);
 
 /* Timing of the JCG launches with HIP events on the context's stream:
- * adcgpu_profile(ctx, 1) clears and starts recording, adcgpu_profile_read
- * returns the summed kernel time (ms) and the number of launches since. */
+ * adcgpu_profile(ctx, n) with n > 0 clears and starts recording (event pairs
+ * for max(n, 1024) launches are created at once, outside any timed region),
+ * adcgpu_profile(ctx, 0) stops; adcgpu_profile_read returns the summed kernel
+ * time (ms) and the number of launches recorded. */
 int adcgpu_profile(adcgpu_ctx *ctx, int enable);
 int adcgpu_profile_read(adcgpu_ctx *ctx, double *jcg_ms, int64_t *launches);
 

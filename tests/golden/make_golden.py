@@ -7,7 +7,7 @@ numpy archives.  Needs the reference build (``make -C oracle ref``), so it runs
 in the build container only; the fixtures it writes are what the tests use
 everywhere else.
 
-    python tests/golden/make_golden.py [basin] [beach] [jcg]
+    python tests/golden/make_golden.py [basin] [beach] [jcg] [tau0]
 """
 from __future__ import annotations
 
@@ -157,14 +157,37 @@ def make_jcg() -> str:
     return path
 
 
+def make_tau0() -> str:
+    """TAU0VAR as the reference's InitNodalAttr sets it from a negative
+    fort.15 TAU0 (schemes -1 and -2), on a small coastal mesh whose depths
+    span 4000 m to -3 m, with three nodes placed exactly on the schemes'
+    depth thresholds (1, 10 and 200 m)."""
+    out = {}
+    for scheme in (-1.0, -2.0):
+        case = meshgen.coastal(3000, storm=False)
+        case.mesh.dp[:3] = (1.0, 10.0, 200.0)
+        case.run.tau0 = scheme
+        with tempfile.TemporaryDirectory() as td:
+            refdump.write_inputs(case, td, nsteps=1, dump_every=1)
+            b = refdump.read_output(refdump.run_reference(td))
+        tag = f"m{int(-scheme)}"
+        out[f"{tag}_dp"] = b["dp"]
+        out[f"{tag}_tau0var"] = b["tau0var"]
+    path = os.path.join(HERE, "tau0.npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
 def main(argv):
-    todo = argv or ["basin", "beach", "jcg"]
+    todo = argv or ["basin", "beach", "jcg", "tau0"]
     if "basin" in todo:
         print(run_case("basin", meshgen.basin(), nsteps=43200, early=10, dump_every=43200))
     if "beach" in todo:
         print(run_case("beach", meshgen.beach(), nsteps=3600, early=10, dump_every=600))
     if "jcg" in todo:
         print(make_jcg())
+    if "tau0" in todo:
+        print(make_tau0())
 
 
 if __name__ == "__main__":

@@ -82,7 +82,7 @@ def case_setup(case):
     d = dict(t)
     d["emo"] = np.asarray(case.emo, np.float64).ravel()
     d["efa"] = (np.asarray(case.efa, np.float64) * (3.141592653589793 / 180.0)).ravel()
-    d["tau0var"] = np.full(npn, r.tau0)
+    d["tau0var"] = prep.tau0_nodal(r.tau0, t["dp"])
     d["evm"] = np.full(npn, r.eslm)
     d["fric"] = np.full(npn, r.cf)
     e2, nc, nf = cold_wetdry_state(t, r)

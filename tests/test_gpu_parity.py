@@ -68,6 +68,51 @@ def test_coastal_large_mesh_bitexact():
     _run_bitexact(d, case.run, 12, check_every=4)
 
 
+def test_wetting_many_candidates_shuffled_elements():
+    """A step with tens of thousands of wetting candidates whose friction
+    updates chain through shared nodes, on a mesh with shuffled element
+    numbering, so that a candidate's predecessor on a node usually lies
+    thousands of candidates back: bit-exact vs oracle.  About a third of the
+    nodes start dry and 0.5 m below their wet neighbours, so nearly every
+    element with two wet nodes passes the wetting criterion."""
+    from gpu_helpers import case_setup
+    case = meshgen.coastal(40000)
+    rng = np.random.default_rng(11)
+    case.mesh.nm = case.mesh.nm[rng.permutation(case.mesh.ne)]
+    d = case_setup(case)
+    nc = np.asarray(d["nodecode0"], np.int32).copy()
+    eta2 = np.asarray(d["eta2_0"], np.float64).copy()
+    dry = (rng.random(nc.shape[0]) < 1.0 / 3.0) & (nc == 1)
+    nc[dry] = 0
+    eta2[dry] -= 0.5
+    d["nodecode0"], d["eta2_0"] = nc, eta2
+    nm0 = np.asarray(d["nm"], np.int64).reshape(-1, 3) - 1
+    two_wet = int((nc[nm0].sum(axis=1) == 2).sum())
+    log(f"case coastal {int(d['np'])} nodes, shuffled elements, {dry.sum()} nodes dried, "
+        f"{two_wet} elements with two wet nodes: 4 steps bit-exact vs oracle")
+    assert two_wet > 2 * 8192
+    _run_bitexact(d, case.run, 4)
+
+
+def test_holland_forcing_vs_oracle():
+    """The device Holland vortex (the resident run's met forcing) against the
+    oracle's host evaluation of the same formula at four times of the storm:
+    they differ only by the device and host libm (relative 1e-13)."""
+    from gpu_helpers import case_setup
+    case = meshgen.coastal(20000)
+    d = case_setup(case)
+    g, o = build_pair(d, case.run)
+    for t in (0.0, 3600.0, 10800.0, 21600.0):
+        g.holland(t, case.run.holland)
+        ref = o.forcing(t)
+        for name, r in zip(("wsx2", "wsy2", "pr2"), ref):
+            a = g.get(name)
+            # the wind ramp is zero at t = 0: both sides exactly zero there
+            err = np.abs(a - r).max() / max(np.abs(r).max(), 1e-300)
+            log(f"holland t={t:.0f} s {name}: max relative difference {err:.2e}")
+            assert err <= 1e-13, (t, name)
+
+
 def test_basin_full_day_resident_vs_reference(basin_golden):
     """43,200 steps through the resident run (one launch sequence per step,
     elevation forcing from a precomputed table): final state within the

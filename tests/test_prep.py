@@ -41,3 +41,19 @@ def test_cold_start_matches_reference(beach_golden):
     assert np.array_equal(noff, d["noff0"])
     assert np.array_equal(eta2, d["eta2_0"])
     assert (nc == 0).sum() > 50  # the case really starts partly dry
+
+
+@pytest.mark.parametrize("scheme", [-1.0, -2.0])
+def test_tau0_schemes_match_reference(scheme):
+    """TAU0VAR from a negative fort.15 TAU0, as the reference's InitNodalAttr
+    set it on a coastal mesh spanning 4000 m to -3 m with nodes exactly on the
+    1, 10 and 200 m thresholds: bit-exact, including the -2 scheme's
+    single-precision 0.005."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "tau0.npz"))
+    tag = f"m{int(-scheme)}"
+    dp, ref = g[f"{tag}_dp"], g[f"{tag}_tau0var"]
+    mine = prep.tau0_nodal(scheme, dp)
+    assert np.array_equal(mine.view(np.uint64), ref.view(np.uint64))
+    assert len(np.unique(ref)) >= 2 and (dp <= 10.0).any() and (dp > 200.0).any()
+    assert np.array_equal(prep.tau0_nodal(0.03, dp), np.full(dp.shape[0], 0.03))
