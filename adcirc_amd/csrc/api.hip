@@ -414,6 +414,10 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     A(&c->wd.mask, nb * 4); A(&c->wd.bcnt, nb); A(&c->wd.bofs, nb);
     A(&c->wd.tkc, ne); A(&c->wd.lc, ne); A(&c->wd.m0, ne); A(&c->wd.pred, ne);
     A(&c->wd.wnode, ne); A(&c->wd.M, ne);
+    const size_t nwb = (size_t)(ne + WD_BLOCK - 1) / WD_BLOCK;
+    A(&c->wd.cv, ne); A(&c->wd.cE, (size_t)ne * WD_XW); A(&c->wd.xtab, nwb * WD_XMAX);
+    A(&c->wd.xcnt, nwb);
+    A(&c->wd.xval, nwb * WD_XMAX);
   }
   A(&c->flags, F_COUNT); A(&c->jpart, (size_t)c->nch * 4); A(&c->jsaved, 8);
   A(&c->jbar, JCG_BAR_WORDS);
@@ -429,8 +433,11 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        mnei * RED_THREADS * (int)sizeof(double));
     if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)k_wd_prop, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              WD_PROP_LDS);
+      e = hipFuncSetAttribute((const void*)k_wd_link, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              WD_LINK_LDS);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_wd_close, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              WD_CLOSE_LDS);
     if (e != hipSuccess) rc = fail(ADCGPU_E_UNSUPPORTED, std::string("LDS request: ") +
                                                          hipGetErrorString(e));
   }
@@ -465,9 +472,9 @@ void adcgpu_destroy(adcgpu_ctx* c) {
     (void)hipEventDestroy(ev.first);
     (void)hipEventDestroy(ev.second);
   }
-  for (void* p : c->allocs) hipFree(p);
-  if (c->eta_bc_table) hipFree(c->eta_bc_table);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->eta_bc_table) (void)hipFree(c->eta_bc_table);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -656,8 +663,13 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
   hipLaunchKernelGGL(k_wd_index, dim3(1), dim3(1024), 0, c->stream, c->wd.bcnt, nb, c->wd.bofs,
                      c->flags);
   hipLaunchKernelGGL(k_wd_pred, dim3(nb), dim3(256), 0, c->stream, c->mesh(), c->state(), c->wd);
-  hipLaunchKernelGGL(k_wd_prop, dim3(1), dim3(256), WD_PROP_LDS, c->stream, c->wd, c->flags,
+  const int nwb = std::min((ne + WD_BLOCK - 1) / WD_BLOCK, 256);
+  hipLaunchKernelGGL(k_wd_close, dim3(nwb), dim3(WD_BLOCK), WD_CLOSE_LDS, c->stream, c->wd,
+                     c->flags);
+  hipLaunchKernelGGL(k_wd_link, dim3(1), dim3(256), WD_LINK_LDS, c->stream, c->wd, c->flags,
                      c->tk);
+  hipLaunchKernelGGL(k_wd_final, dim3(std::min(nb, 1024)), dim3(256), 0, c->stream, c->wd,
+                     c->flags, c->tk);
   // cx is free until the momentum step: it holds NCEle*AREAS/2 per incidence slot
   hipLaunchKernelGGL(k_wd_area, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->cx);
@@ -726,7 +738,7 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   if (c->neta > 0) {
     if (!eta_bc_table) return fail(ADCGPU_E_ARG, "eta_bc_table required");
     if (c->eta_bc_table_steps < nsteps) {
-      if (c->eta_bc_table) hipFree(c->eta_bc_table);
+      if (c->eta_bc_table) (void)hipFree(c->eta_bc_table);
       c->eta_bc_table = nullptr;
       HIPCHK(hipMalloc(&c->eta_bc_table, (size_t)nsteps * c->neta * sizeof(double)));
       c->eta_bc_table_steps = nsteps;
@@ -871,8 +883,8 @@ int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double*
     double saved[8] = {0};
     if (c) {
       (void)hipMemcpy(saved, c->jsaved, sizeof(saved), hipMemcpyDeviceToHost);
-      for (void* p : c->allocs) hipFree(p);
-      if (c->stream) hipStreamDestroy(c->stream);
+      for (void* p : c->allocs) (void)hipFree(p);
+      if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
     }
     c = new adcgpu_ctx();

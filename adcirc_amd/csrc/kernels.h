@@ -20,14 +20,24 @@ constexpr int JCG_BAR_STRIDE = 64;       // words between barrier counters (256 
 constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BAR_STRIDE
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
 constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
-// wetting propagation (k_wd_prop): candidates staged through LDS in chunks,
-// the running maxima of the most recent WD_RING candidates kept in an LDS ring
+// ordered wetting updates: candidates are closed in blocks of WD_BLOCK, each
+// block naming at most WD_XMAX earlier candidates it depends on
+constexpr int WD_BLOCK = 1024;
+constexpr int WD_XMAX = 256;                 // four 64-bit mask words per candidate
+constexpr int WD_XW = WD_XMAX / 64;
+constexpr int WD_HASH = 2 * WD_XMAX;         // LDS hash slots for a block's external names
+constexpr int WD_CLOSE_LDS = 2 * WD_BLOCK * (8 + 8 * WD_XW + 4) + 2 * WD_HASH * 4;
+constexpr int WD_LINK_BLOCKS = 64;           // blocks whose link values fit the LDS of k_wd_link
+// serial fallback of k_wd_link: candidates staged through LDS in chunks, the
+// running maxima of the most recent WD_RING candidates kept in an LDS ring
 constexpr int WD_CHUNK = 2048;
 constexpr int WD_RING = 8192;   // power of two
-constexpr int WD_PROP_LDS = WD_RING * 8 + WD_CHUNK * (8 + 8 + 8);
+constexpr int WD_SERIAL_LDS = WD_RING * 8 + WD_CHUNK * (8 + 8 + 8);
+constexpr int WD_LINK_LDS = WD_LINK_BLOCKS * WD_XMAX * 8 > WD_SERIAL_LDS
+                                ? WD_LINK_BLOCKS * WD_XMAX * 8 : WD_SERIAL_LDS;
 
 // device flag slots
-enum { F_NCCHANGE = 0, F_NUMITR = 2, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_NREBUILD,
+enum { F_NCCHANGE = 0, F_NUMITR = 2, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_NREBUILD, F_WDSERIAL,
        F_COUNT = 16 };
 enum { ERR_DIAG = 1, ERR_BARRIER = 4 };
 // device scalar slots
@@ -98,7 +108,15 @@ struct WdWork {
   double* m0;                // [ne] candidate k: TK_TMP and the entry TK of nodes it opens
   int2* pred;                // [ne] candidate k: previous candidate on NM123 / partner, or -1
   int2* wnode;               // [ne] candidate k: NM123 / partner if k is its last candidate, or -1
-  double* M;                 // [ne] candidate k: TK of both nodes after k
+  double* M;                 // [ne] candidate k: TK of both nodes after k (serial fallback)
+  // block closure (k_wd_close): candidate k's value is
+  // max(cv[k], X values of the bits of cE[k]), the X values being those of the
+  // earlier candidates xtab[b*WD_XMAX + i] of its block b = k / WD_BLOCK
+  double* cv;                // [ne] maximum over k's ancestors inside its block
+  unsigned long long* cE;    // [ne*WD_XW] external names reached inside the block
+  int* xtab;                 // [nblk*WD_XMAX] block b's external names (candidate indices)
+  int* xcnt;                 // [nblk] number of external names of block b
+  double* xval;              // [nblk*WD_XMAX] final value of each external name
 };
 
 struct HollandPrm {
@@ -149,7 +167,9 @@ __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double hab
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);
 __global__ void k_wd_index(const int* bcnt, int nb, int* bofs, int* flags);
 __global__ void k_wd_pred(DevMesh m, DevState s, WdWork w);
-__global__ void k_wd_prop(WdWork w, const int* flags, double* tk);
+__global__ void k_wd_close(WdWork w, int* flags);
+__global__ void k_wd_link(WdWork w, int* flags, double* tk);
+__global__ void k_wd_final(WdWork w, const int* flags, double* tk);
 __global__ void k_wd_area(DevMesh m, DevState s, double* area_slot);
 __global__ void k_wd_d2(DevMesh m, DevState s, const double* area_slot, int* flags, double ifnlfa);
 __global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim);

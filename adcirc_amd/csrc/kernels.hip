@@ -459,7 +459,10 @@ __global__ __launch_bounds__(1024) void k_wd_index(const int* __restrict__ bcnt,
     bofs[i] = run;
     run += bcnt[i];
   }
-  if (t == 1023) flags[F_NCAND] = base + incl;
+  if (t == 1023) {
+    flags[F_NCAND] = base + incl;
+    flags[F_WDSERIAL] = 0;
+  }
 }
 
 // Candidate index of candidate element e (element order); `word` = mask[e/64].
@@ -529,20 +532,185 @@ __global__ __launch_bounds__(256) void k_wd_pred(DevMesh m, DevState s, WdWork w
 
 // Ordered W1 friction updates, values: M_k = max(m0_k, M of k's predecessors)
 // in candidate order, then each node takes the M of its last candidate.  TK
-// values are finite, so max is exact and order-free.  The
-// candidates pass through LDS in chunks of WD_CHUNK; one lane forms M along
-// the chunk (the previous candidate's M stays in a register, the last WD_RING
-// values in an LDS ring, older ones are read back from w.M) and the block
-// writes the chunk's TK values.  One block of 256; dynamic LDS of
-// WD_PROP_LDS bytes.
-__global__ __launch_bounds__(256) void k_wd_prop(WdWork w, const int* flags,
-                                                 double* __restrict__ tk) {
-  extern __shared__ double wp_lds[];
-  double* ring = wp_lds;                                          // [WD_RING]
+// values are finite, so max is exact and order-free, and M_k is the maximum of
+// m0 over k and all of its ancestors in the predecessor graph (edges run from
+// lower to higher candidate index).
+//
+// Block closure.  Candidates are cut into blocks of WD_BLOCK.  Inside a block,
+// M_k splits into the maximum over k's ancestors inside the block (cv) and the
+// set of earlier-block candidates those ancestors name as predecessors (cE,
+// bits into the block's table of external names xtab).  Pointer doubling
+// forms both: each candidate keeps a frontier of at most two ancestors not
+// yet folded in; a round replaces frontier members by their own frontiers
+// (folding in their values and name sets) whenever the result still has at
+// most two members.  After every such step ancestors(k) = folded(k) plus
+// ancestors(frontier(k)), and the lowest candidate with a non-empty frontier
+// can always expand it, so the rounds end; along a wetting front, where
+// candidates chain through shared nodes, they take about log2 of the chain
+// length.  A block naming more than WD_XMAX external candidates sets
+// F_WDSERIAL, and k_wd_link then forms every value serially instead.
+__device__ __forceinline__ void wd_add(int* u, int& nu, int f) {
+  if (f == 0xffff) return;
+  for (int i = 0; i < nu; ++i)
+    if (u[i] == f) return;
+  u[nu++] = f;
+}
+
+__global__ __launch_bounds__(1024) void k_wd_close(WdWork w, int* flags) {
+  extern __shared__ double wc_lds[];
+  double* sv = wc_lds;                                                       // [2][WD_BLOCK]
+  unsigned long long* sE = reinterpret_cast<unsigned long long*>(sv + 2 * WD_BLOCK);  // [2][WD_XW][WD_BLOCK]
+  int* sF = reinterpret_cast<int*>(sE + 2 * WD_XW * WD_BLOCK);               // [2][WD_BLOCK]
+  int* hkey = sF + 2 * WD_BLOCK;                                             // [WD_HASH]
+  int* hidx = hkey + WD_HASH;                                                // [WD_HASH]
+  __shared__ int xn;
+  const int n = flags[F_NCAND];
+  const int nblk = (n + WD_BLOCK - 1) / WD_BLOCK;
+  const int l = threadIdx.x;
+  for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const int base = b * WD_BLOCK, k = base + l;
+    for (int i = l; i < WD_HASH; i += WD_BLOCK) hkey[i] = -1;
+    if (l == 0) xn = 0;
+    __syncthreads();
+    int qq[2] = {-1, -1};
+    double v = 0.0;
+    if (k < n) {
+      const int2 q = w.pred[k];
+      qq[0] = q.x;
+      qq[1] = q.y;
+      v = w.m0[k];
+    }
+    // the block's external names: one hash slot per distinct candidate
+    int slot[2] = {-1, -1};
+    bool won[2] = {false, false};
+    bool ovf = false;
+    for (int j = 0; j < 2; ++j) {
+      const int x = qq[j];
+      if (x < 0 || x >= base) continue;
+      int h = (int)(((unsigned)x * 2654435761u) >> 23);  // 9 bits: WD_HASH slots
+      for (int probes = 0; probes < WD_HASH; ++probes) {
+        const int old = atomicCAS(&hkey[h], -1, x);
+        if (old == -1 || old == x) {
+          won[j] = old == -1;
+          slot[j] = h;
+          break;
+        }
+        h = (h + 1) & (WD_HASH - 1);
+      }
+      if (slot[j] < 0) ovf = true;
+    }
+    __syncthreads();
+    for (int j = 0; j < 2; ++j)
+      if (won[j]) {
+        const int i = atomicAdd(&xn, 1);
+        hidx[slot[j]] = i;
+        if (i < WD_XMAX) w.xtab[b * WD_XMAX + i] = qq[j];
+      }
+    __syncthreads();
+    unsigned long long E[WD_XW] = {};
+    int f0 = 0xffff, f1 = 0xffff;
+    for (int j = 0; j < 2; ++j) {
+      const int x = qq[j];
+      if (x < 0) continue;
+      if (x >= base) {
+        const int li = x - base;
+        if (f0 == 0xffff) f0 = li;
+        else if (li != f0) f1 = li;
+      } else if (slot[j] >= 0) {
+        const int i = hidx[slot[j]];
+        if (i < WD_XMAX) E[i >> 6] |= 1ull << (i & 63);
+        else ovf = true;
+      }
+    }
+    if (__syncthreads_or(ovf)) {
+      if (l == 0) atomicOr(&flags[F_WDSERIAL], 1);
+      continue;
+    }
+    if (l == 0) w.xcnt[b] = xn;
+    int cur = 0;
+    sv[l] = v;
+#pragma unroll
+    for (int i = 0; i < WD_XW; ++i) sE[i * WD_BLOCK + l] = E[i];
+    sF[l] = f0 | (f1 << 16);
+    __syncthreads();
+    while (true) {
+      const int nx = cur ^ 1;
+      const double* cv_ = sv + cur * WD_BLOCK;
+      const unsigned long long* cE_ = sE + cur * WD_XW * WD_BLOCK;
+      const int* cF_ = sF + cur * WD_BLOCK;
+      int F = cF_[l];
+      const int a = F & 0xffff, c = (F >> 16) & 0xffff;
+      // members to fold in this round: bit 0 = a, bit 1 = c
+      int fold = 0;
+      if (a != 0xffff) {
+        const int Fa = cF_[a];
+        const int Fc = c != 0xffff ? cF_[c] : -1;
+        int u[4], nu = 0;
+        wd_add(u, nu, Fa & 0xffff);
+        wd_add(u, nu, (Fa >> 16) & 0xffff);
+        if (c != 0xffff) {
+          wd_add(u, nu, Fc & 0xffff);
+          wd_add(u, nu, (Fc >> 16) & 0xffff);
+        }
+        if (nu <= 2) {
+          fold = c != 0xffff ? 3 : 1;
+          F = (nu > 0 ? u[0] : 0xffff) | ((nu > 1 ? u[1] : 0xffff) << 16);
+        } else {
+          // expand one member alone
+          int ua[3], na = 0;
+          wd_add(ua, na, c);
+          wd_add(ua, na, Fa & 0xffff);
+          wd_add(ua, na, (Fa >> 16) & 0xffff);
+          if (na <= 2) {
+            fold = 1;
+            F = (na > 0 ? ua[0] : 0xffff) | ((na > 1 ? ua[1] : 0xffff) << 16);
+          } else if (c != 0xffff) {
+            int uc[3], nc = 0;
+            wd_add(uc, nc, a);
+            wd_add(uc, nc, Fc & 0xffff);
+            wd_add(uc, nc, (Fc >> 16) & 0xffff);
+            if (nc <= 2) {
+              fold = 2;
+              F = (nc > 0 ? uc[0] : 0xffff) | ((nc > 1 ? uc[1] : 0xffff) << 16);
+            }
+          }
+        }
+      }
+      if (fold & 1) {
+        v = fmax(v, cv_[a]);
+#pragma unroll
+        for (int i = 0; i < WD_XW; ++i) E[i] |= cE_[i * WD_BLOCK + a];
+      }
+      if (fold & 2) {
+        v = fmax(v, cv_[c]);
+#pragma unroll
+        for (int i = 0; i < WD_XW; ++i) E[i] |= cE_[i * WD_BLOCK + c];
+      }
+      sv[nx * WD_BLOCK + l] = v;
+#pragma unroll
+      for (int i = 0; i < WD_XW; ++i) sE[(nx * WD_XW + i) * WD_BLOCK + l] = E[i];
+      sF[nx * WD_BLOCK + l] = F;
+      cur = nx;
+      if (!__syncthreads_or((F & 0xffff) != 0xffff)) break;
+    }
+    if (k < n) {
+      w.cv[k] = v;
+#pragma unroll
+      for (int i = 0; i < WD_XW; ++i) w.cE[(size_t)k * WD_XW + i] = E[i];
+    }
+  }
+}
+
+// Serial form of the ordered updates (blocks with too many external names):
+// the candidates pass through LDS in chunks of WD_CHUNK; one lane forms M
+// along the chunk (the previous candidate's M in a register, the last WD_RING
+// values in an LDS ring, older ones read back from w.M) and the block writes
+// the chunk's TK values.  `lds` holds WD_SERIAL_LDS bytes.
+__device__ void wd_serial(const WdWork& w, int n, double* __restrict__ tk, double* lds) {
+  double* ring = lds;                                             // [WD_RING]
   double* cm0 = ring + WD_RING;                                   // [WD_CHUNK]
   int2* cpred = reinterpret_cast<int2*>(cm0 + WD_CHUNK);          // [WD_CHUNK]
   int2* cwn = cpred + WD_CHUNK;                                   // [WD_CHUNK]
-  const int n = flags[F_NCAND];
   const int t = threadIdx.x;
   double mprev = 0.0;  // M of the previous candidate (lane 0)
   for (int base = 0; base < n; base += WD_CHUNK) {
@@ -577,6 +745,65 @@ __global__ __launch_bounds__(256) void k_wd_prop(WdWork w, const int* flags,
       if (x.y >= 0) tk[x.y] = v;
     }
     __syncthreads();
+  }
+}
+
+// Values of the external names, block after block: name i of block c is an
+// earlier candidate x of block b < c, so its value is max(cv[x], values of
+// the names of block b set in cE[x]), all known by then.  One block of 256
+// (one lane per name); dynamic LDS of WD_LINK_LDS bytes (the name values of
+// up to WD_LINK_BLOCKS blocks; beyond that they stay in w.xval).
+__global__ __launch_bounds__(256) void k_wd_link(WdWork w, int* flags, double* __restrict__ tk) {
+  extern __shared__ double lk_lds[];
+  const int n = flags[F_NCAND];
+  if (n == 0) return;
+  if (flags[F_WDSERIAL]) {
+    wd_serial(w, n, tk, lk_lds);
+    return;
+  }
+  const int nblk = (n + WD_BLOCK - 1) / WD_BLOCK;
+  double* X = nblk <= WD_LINK_BLOCKS ? lk_lds : w.xval;
+  const int t = threadIdx.x;
+  for (int c = 1; c < nblk; ++c) {
+    if (t < w.xcnt[c]) {
+      const int x = w.xtab[c * WD_XMAX + t];
+      double v = w.cv[x];
+      const double* Xb = X + (x / WD_BLOCK) * WD_XMAX;
+#pragma unroll
+      for (int i = 0; i < WD_XW; ++i) {
+        unsigned long long E = w.cE[(size_t)x * WD_XW + i];
+        while (E) {
+          v = fmax(v, Xb[64 * i + __ffsll((long long)E) - 1]);
+          E &= E - 1;
+        }
+      }
+      X[c * WD_XMAX + t] = v;
+    }
+    __syncthreads();
+  }
+  if (X != w.xval)
+    for (int i = t; i < nblk * WD_XMAX; i += blockDim.x) w.xval[i] = X[i];
+}
+
+// Final values and the TK of each node from its last candidate.
+__global__ __launch_bounds__(256) void k_wd_final(WdWork w, const int* flags,
+                                                  double* __restrict__ tk) {
+  const int n = flags[F_NCAND];
+  if (flags[F_WDSERIAL]) return;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    double v = w.cv[k];
+    const double* Xb = w.xval + (k / WD_BLOCK) * WD_XMAX;
+#pragma unroll
+    for (int i = 0; i < WD_XW; ++i) {
+      unsigned long long E = w.cE[(size_t)k * WD_XW + i];
+      while (E) {
+        v = fmax(v, Xb[64 * i + __ffsll((long long)E) - 1]);
+        E &= E - 1;
+      }
+    }
+    const int2 x = w.wnode[k];
+    if (x.x >= 0) tk[x.x] = v;
+    if (x.y >= 0) tk[x.y] = v;
   }
 }
 
