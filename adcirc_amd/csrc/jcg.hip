@@ -473,20 +473,40 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
     }
     if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
 
-    // reduce the chunk partials (every block, identical tree)
+    // the eigenvalue block's estimate for this iteration, loaded alongside
+    // the partials
+    double cme_in = 0.0, cmold_in = 0.0;
+    if (t == 0) {
+      cme_in = a.saved[6];
+      cmold_in = a.saved[7];
+    }
+    // reduce the chunk partials (every block, identical tree); the loads of
+    // two chunks are issued before their sums
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-    for (int cc = t; t < RED_XTHREADS && cc < nch; cc += RED_XTHREADS) {
-      const double* pc = a.part + (size_t)cc * 4;
-      r0 = r0 + pc[0];
-      r1 = r1 + pc[1];
-      r2 = r2 + pc[2];
+    if (t < RED_XTHREADS) {
+      int cc = t;
+      for (; cc + RED_XTHREADS < nch; cc += 2 * RED_XTHREADS) {
+        const double* pa = a.part + (size_t)cc * 4;
+        const double* pb = a.part + (size_t)(cc + RED_XTHREADS) * 4;
+        const double a0 = pa[0], a1 = pa[1], a2 = pa[2];
+        const double b0 = pb[0], b1 = pb[1], b2 = pb[2];
+        r0 = (r0 + a0) + b0;
+        r1 = (r1 + a1) + b1;
+        r2 = (r2 + a2) + b2;
+      }
+      if (cc < nch) {
+        const double* pa = a.part + (size_t)cc * 4;
+        r0 = r0 + pa[0];
+        r1 = r1 + pa[1];
+        r2 = r2 + pa[2];
+      }
     }
     block_tree3(r0, r1, r2, lds);
     const double delnnm = r0, dtnrm = r1, unorm_s = r2;
     if (t == 0) {
       L.in = in;
-      L.cme = a.saved[6];
-      L.cmold = a.saved[7];
+      L.cme = cme_in;
+      L.cmold = cmold_in;
       L.delnnm = delnnm;
       jcg_pstop(L, need_u ? unorm_s : 1.0, delnnm);
       if (!L.halt) jcg_parcon(L, dtnrm);
@@ -504,7 +524,9 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         uc[r] = un;
         dp[r] = dc[r];
         dc[r] = dn;
-        if (i < np) at(a.d, i) = dn;
+        // written through to memory: the next grid barrier's release then
+        // has no pseudo-residual lines to write back from this XCD's L2
+        if (i < np) __hip_atomic_store(&at(a.d, i), dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
