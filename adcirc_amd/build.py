@@ -44,7 +44,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     # the native driver (C ABI only), linked against the library beside it
     cmd = [hipcc(), *HIPCC_FLAGS, os.path.join(CSRC, "run_case.hip"), "-L", HERE,
-           "-ladcgpu", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,$ORIGIN", "-o", DRIVER]
+           "-ladcgpu", "-Wl,-rpath,$ORIGIN", "-o", DRIVER]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

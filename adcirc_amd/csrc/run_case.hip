@@ -6,14 +6,9 @@
 //   -> adcgpu_run (warm-up steps) -> adcgpu_run (timed steps)
 //
 //   adcgpu_run_case CASE [--warmup W] [--steps K] [--chunk C] [--copy-mib N]
-//                        [--profile-from S --profile-to T]
 //   adcgpu_run_case --copy-mib N
 //
 // Prints one line of timing: steps, ms per step, JCG iterations per step.
-// --profile-from/--profile-to bracket time steps S..T (inclusive, numbered
-// from 1 like IT) with roctxProfilerResume / roctxProfilerPause, so that a
-// profiler run with --selected-regions records exactly those steps (without
-// a profiler attached both calls do nothing).
 // --copy-mib runs a plain device-to-device copy kernel over N MiB ten times
 // (after the case, when one is given):
 // the reference reading of the HBM traffic counters (FETCH_SIZE, WRITE_SIZE)
@@ -29,7 +24,6 @@
 #include <vector>
 
 #include <hip/hip_runtime.h>
-#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "../../include/adcgpu.h"
 
@@ -123,16 +117,13 @@ int main(int argc, char** argv) {
     return 2;
   }
   if (argc == 3 && !std::strcmp(argv[1], "--copy-mib")) return copy_probe(std::atoi(argv[2]));
-  int warmup = 10, steps = 40, chunk = 200, copy_mib = 0, prof_from = 0, prof_to = -1;
+  int warmup = 10, steps = 40, chunk = 200, copy_mib = 0;
   for (int a = 2; a + 1 < argc; a += 2) {
     if (!std::strcmp(argv[a], "--warmup")) warmup = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--steps")) steps = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--chunk")) chunk = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--copy-mib")) copy_mib = std::atoi(argv[a + 1]);
-    else if (!std::strcmp(argv[a], "--profile-from")) prof_from = std::atoi(argv[a + 1]);
-    else if (!std::strcmp(argv[a], "--profile-to")) prof_to = std::atoi(argv[a + 1]);
   }
-  if (chunk < 1) chunk = 1;
   std::map<std::string, Record> c;
   if (!read_case(argv[1], c)) {
     std::fprintf(stderr, "cannot read case file %s\n", argv[1]);
@@ -174,35 +165,20 @@ int main(int argc, char** argv) {
   if (hol) check(adcgpu_holland(ctx, 0.0, hol), "holland");
   std::fprintf(stderr, "[run_case] %d nodes, %d elements: warm-up %d steps\n", m.np, m.ne,
                warmup);
-  // steps first..last run in chunks that never cross a profiling boundary;
-  // at a boundary the stream is drained before the profiler is resumed or
-  // paused
-  auto run_steps = [&](int first, int last, int64_t* iters, bool report) {
-    for (int it = first; it <= last;) {
-      int end = std::min(last, it + chunk - 1);
-      if (it < prof_from && end >= prof_from) end = prof_from - 1;
-      if (it <= prof_to && end > prof_to) end = prof_to;
-      if (it == prof_from) {
-        check(adcgpu_synchronize(ctx), "synchronize");
-        roctxProfilerResume(0);
-      }
-      check(adcgpu_run(ctx, it, end - it + 1, table.data() + (size_t)(it - 1) * neta, hol, 0),
-            "run");
-      if (iters) *iters += adcgpu_run_iterations(ctx);
-      if (end == prof_to) {
-        check(adcgpu_synchronize(ctx), "synchronize");
-        roctxProfilerPause(0);
-      }
-      if (report) std::fprintf(stderr, "[run_case]   %d/%d steps\n", end - warmup, steps);
-      it = end + 1;
-    }
-  };
-  run_steps(1, warmup, nullptr, false);
+  if (warmup > 0) check(adcgpu_run(ctx, 1, warmup, table.data(), hol, 0), "run");
   check(adcgpu_synchronize(ctx), "synchronize");
   std::fprintf(stderr, "[run_case] timed %d steps\n", steps);
   int64_t iters = 0;
   const auto t0 = std::chrono::steady_clock::now();
-  run_steps(warmup + 1, warmup + steps, &iters, true);
+  for (int done = 0; done < steps;) {
+    const int k = std::min(chunk, steps - done);
+    check(adcgpu_run(ctx, warmup + done + 1, k, table.data() + (size_t)(warmup + done) * neta,
+                     hol, 0),
+          "run");
+    iters += adcgpu_run_iterations(ctx);
+    done += k;
+    std::fprintf(stderr, "[run_case]   %d/%d steps\n", done, steps);
+  }
   check(adcgpu_synchronize(ctx), "synchronize");
   const double ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

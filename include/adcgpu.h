@@ -41,7 +41,7 @@ extern "C" {
 #define ADCGPU_E_UNSUPPORTED (-2) /* configuration outside the implemented subset */
 #define ADCGPU_E_HIP (-3)         /* HIP runtime error (message via adcgpu_last_error) */
 #define ADCGPU_E_DIAG (-4)        /* GWCE diagonal <= 0 (src/gwce.F:678-707 terminates) */
-#define ADCGPU_E_CAPACITY (-5)    /* a fixed device capacity was exceeded */
+#define ADCGPU_E_CAPACITY (-5)    /* the mesh has more rows than the resident JCG grid holds */
 #define ADCGPU_E_NONFINITE (-6)   /* non-finite JCG residual */
 
 typedef struct adcgpu_ctx adcgpu_ctx;
