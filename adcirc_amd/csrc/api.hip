@@ -61,7 +61,8 @@ struct adcgpu_ctx {
   // mesh
   int *nm = nullptr, *inc_ptr = nullptr, *inc = nullptr, *slot = nullptr, *nneigh = nullptr;
   int *col = nullptr, *slot_cols = nullptr;
-  int *csrpos = nullptr, *bnd_index = nullptr, *nbd = nullptr, *obc_slot = nullptr;
+  int *csrpos = nullptr, *bnd_index = nullptr, *nbd = nullptr, *obc_q = nullptr, *obc_s = nullptr;
+  int nobc = 0;
   int *obcg_ptr = nullptr, *obcg = nullptr, *vbc_ptr = nullptr, *vbc = nullptr, *lbcodei = nullptr;
   double *fdx = nullptr, *fdy = nullptr, *areas = nullptr, *x = nullptr, *y = nullptr, *dp = nullptr;
   double *tau0 = nullptr, *evm = nullptr, *corif = nullptr, *csii = nullptr, *siii = nullptr;
@@ -84,6 +85,7 @@ struct adcgpu_ctx {
   long long barrier_ticks = 0;  // 0.5 s of the device wall clock
   int jcg_cap = 0;              // resident k_jcg blocks on this device
   int64_t run_iters = 0, run_rebuilds = 0;
+  bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // optional timing of the JCG launches with HIP events on the context stream
   int profile = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> jcg_events;  // pool, created ahead
@@ -98,7 +100,7 @@ struct adcgpu_ctx {
     m.tau0 = tau0; m.evm = evm; m.corif = corif; m.inc_ptr = inc_ptr; m.inc = inc;
     m.slot = slot; m.slot_cols = slot_cols;
     m.nneigh = nneigh; m.col = col; m.csrpos = csrpos; m.bnd_index = bnd_index; m.nbd = nbd;
-    m.obc_slot = obc_slot; m.obcg_ptr = obcg_ptr; m.obcg = obcg; m.vbc_ptr = vbc_ptr;
+    m.obc_q = obc_q; m.obc_s = obc_s; m.nobc = nobc; m.obcg_ptr = obcg_ptr; m.obcg = obcg; m.vbc_ptr = vbc_ptr;
     m.vbc = vbc; m.lbcodei = lbcodei; m.csii = csii; m.siii = siii;
     return m;
   }
@@ -182,9 +184,13 @@ int grow_events(adcgpu_ctx* c, size_t n) {
   return 0;
 }
 
+int jcg_barrier_words(const adcgpu_ctx* c) {
+  return ((c->jcg_grid + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE;
+}
+
 int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
                double* u, int n, int itmax, double zeta, double* eta2, const double* eta1,
-               const int* nodecode) {
+               const int* nodecode, bool prescaled, bool defer_unscal) {
   JcgArgs a;
   a.n = n;
   a.itmax = itmax;
@@ -208,15 +214,20 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   a.w = c->jw;
   a.part = c->jpart;
   a.saved = c->jsaved;
+  a.prescaled = prescaled;
+  a.defer_unscal = defer_unscal;
   a.barrier = c->jbar;
   a.barrier_ticks = c->barrier_ticks;
   a.flags = c->flags;
   a.eta2 = eta2;
   a.eta1 = eta1;
   a.nodecode = nodecode;
-  const int bar_words = ((c->jcg_grid + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE;
-  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(bar_words, 256)), dim3(256), 0, c->stream, c->jbar, 0u,
-                     bar_words);
+  // a prescaling pass (k_rhs_node) has already cleared the barrier words
+  if (!prescaled) {
+    const int bar_words = jcg_barrier_words(c);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(bar_words, 256)), dim3(256), 0, c->stream, c->jbar,
+                       0u, bar_words);
+  }
   void* args[] = {&a};
   if (c->profile) {
     if (c->ev_used == c->jcg_events.size())
@@ -330,6 +341,20 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
       obcl[nb].push_back(I * mnei + (J - 2));
     }
   }
+  // the COEF positions that involve a boundary node, for k_lhs_bc
+  std::vector<int> obc_q, obc_s;
+  for (int j = 1; j < mnei; ++j)
+    for (int i = 0; i < np; ++i) {
+      const int sl = obc_slot[(size_t)j * np + i];
+      if (sl < 0) continue;
+      obc_q.push_back(j * np + i);
+      obc_s.push_back(bnd_index[i] >= 0 ? -1 - sl : sl);
+    }
+  const int nobc = (int)obc_q.size();
+  if (obc_q.empty()) {  // device arrays of at least one element
+    obc_q.push_back(0);
+    obc_s.push_back(0);
+  }
   std::vector<int> obcg_ptr(np + 1, 0), obcg;
   for (int i = 0; i < np; ++i) {
     obcg_ptr[i + 1] = obcg_ptr[i] + (int)obcl[i].size();
@@ -393,7 +418,8 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   std::vector<double> vcor(P->corif, P->corif + np), vfric(P->fric, P->fric + np);
   std::vector<double> var(M->areas, M->areas + ne);
   U(&c->nm, nm0); U(&c->inc_ptr, cnt); U(&c->inc, inc); U(&c->slot, slot); U(&c->slot_cols, slot_cols); U(&c->nneigh, nneigh); U(&c->col, col);
-  U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_slot, obc_slot);
+  U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_q, obc_q);
+  U(&c->obc_s, obc_s); c->nobc = nobc;
   U(&c->obcg_ptr, obcg_ptr); U(&c->obcg, obcg); U(&c->vbc_ptr, vbc_ptr); U(&c->vbc, vbc);
   U(&c->lbcodei, lbc); U(&c->fdx, fdx); U(&c->fdy, fdy); U(&c->areas, var); U(&c->x, vx);
   U(&c->y, vy); U(&c->dp, vdp); U(&c->tau0, vt); U(&c->evm, ve); U(&c->corif, vcor);
@@ -529,10 +555,9 @@ int adcgpu_set_field(adcgpu_ctx* c, int f, const void* host, int64_t count) {
   if (count != field_len(c, f)) return fail(ADCGPU_E_ARG, "field length");
   HIPCHK(hipSetDevice(c->device));
   if (double** p = field_ptr(c, f)) {
-    if (f == ADCGPU_COEF) {
-      // host COEF(NP,MNEI) column-major -> device rows (same layout, unused
-      // padding slots stay as given)
-    }
+    // COEF(NP,MNEI) column-major is the device layout too; a COEF from the
+    // host replaces any scaled off-diagonals a deferred solve left
+    if (f == ADCGPU_COEF) c->coef_scaled = false;
     HIPCHK(hipMemcpyAsync(*p, host, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
   } else if (int** q = ifield_ptr(c, f)) {
     HIPCHK(hipMemcpyAsync(*q, host, count * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -543,10 +568,14 @@ int adcgpu_set_field(adcgpu_ctx* c, int f, const void* host, int64_t count) {
   return 0;
 }
 
+static int restore_coef(adcgpu_ctx* c);
+
 int adcgpu_get_field(adcgpu_ctx* c, int f, void* host, int64_t count) {
   if (!c || !host) return fail(ADCGPU_E_ARG, "null argument");
   if (count != field_len(c, f)) return fail(ADCGPU_E_ARG, "field length");
   HIPCHK(hipSetDevice(c->device));
+  if (f == ADCGPU_COEF)
+    if (int rc = restore_coef(c)) return rc;
   if (double** p = field_ptr(c, f)) {
     HIPCHK(hipMemcpyAsync(host, *p, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   } else if (int** q = ifield_ptr(c, f)) {
@@ -608,30 +637,46 @@ int adcgpu_set_forcing(adcgpu_ctx* c, const double* wsx2, const double* wsy2, co
   return 0;
 }
 
-static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc) {
+// One GWCE assembly and solve.  With `deferred` the solve leaves the
+// off-diagonals of COEF scaled and the next assembly (or adcgpu_run's end)
+// restores them; the resident run uses it, since most steps rebuild COEF.
+static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   const int np = c->np, ne = c->ne;
   // contrib, cx and cy are free until the GWCE right-hand side
   hipLaunchKernelGGL(k_lhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->flags, c->contrib, c->cx, c->cy);
   hipLaunchKernelGGL(k_lhs_rows, dim3(c->nch), dim3(RED_THREADS),
                      (size_t)c->mnei * RED_THREADS * sizeof(double), c->stream, c->mesh(),
-                     c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part);
-  hipLaunchKernelGGL(k_ep, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->nch, np, c->ep_part,
-                     c->flags, c->scal);
-  hipLaunchKernelGGL(k_lhs_bc, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), c->flags, c->scal);
+                     c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
+                     c->coef_scaled ? (const double*)c->cs : nullptr,
+                     c->coef_scaled ? (const double*)c->jw : nullptr);
+  c->coef_scaled = false;
+  hipLaunchKernelGGL(k_lhs_bc, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(), c->state(),
+                     c->flags, c->nch, c->ep_part, c->scal);
   RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, c->prm.nws};
   hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), q, c->flags, c->contrib);
   // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
   std::swap(c->eta1, c->eta2);
   std::swap(c->h1, c->h2);
+  JcgPre pre{c->jw, c->jbar, jcg_barrier_words(c), c->flags};
   hipLaunchKernelGGL(k_rhs_node, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), c->contrib, d_eta_bc, c->scal);
+                     c->state(), c->contrib, d_eta_bc, c->scal, pre);
   int rc = check_launch("gwce assembly");
   if (rc) return rc;
-  return launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, np, c->prm.itmax,
-                    c->prm.convcr, c->eta2, c->eta1, c->nodecode);
+  rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, np, c->prm.itmax,
+                  c->prm.convcr, c->eta2, c->eta1, c->nodecode, true, deferred);
+  if (!rc && deferred) c->coef_scaled = true;
+  return rc;
+}
+
+// COEF with the off-diagonals of a deferred unscaling restored.
+static int restore_coef(adcgpu_ctx* c) {
+  if (!c->coef_scaled) return 0;
+  hipLaunchKernelGGL(k_coef_restore, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), (const double*)c->cs, (const double*)c->jw);
+  c->coef_scaled = false;
+  return check_launch("coef restore");
 }
 
 int adcgpu_solve_gwce(adcgpu_ctx* c, int it, const double* eta_bc) {
@@ -642,7 +687,7 @@ int adcgpu_solve_gwce(adcgpu_ctx* c, int it, const double* eta_bc) {
     HIPCHK(hipMemcpyAsync(c->eta_bc, eta_bc, c->neta * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
   }
-  return gwce_device(c, c->eta_bc);
+  return gwce_device(c, c->eta_bc, false);
 }
 
 int adcgpu_wetdry(adcgpu_ctx* c, int it) {
@@ -753,7 +798,8 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
     int rc = adcgpu_begin_step(c, it);
     if (!rc) rc = adcgpu_friction(c);
     if (!rc && holland) rc = adcgpu_holland(c, it * c->prm.dtdp, holland);
-    if (!rc) rc = gwce_device(c, c->neta > 0 ? c->eta_bc_table + (size_t)s * c->neta : c->eta_bc);
+    if (!rc)
+      rc = gwce_device(c, c->neta > 0 ? c->eta_bc_table + (size_t)s * c->neta : c->eta_bc, true);
     if (!rc) rc = adcgpu_wetdry(c, it);
     if (!rc) rc = adcgpu_momentum(c);
     if (rc) return rc;
@@ -764,6 +810,7 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
       if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
     }
   }
+  if (int rc = restore_coef(c)) return rc;
   int f[F_COUNT];
   HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -922,7 +969,7 @@ int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double*
   HIPCHK(hipMemcpyAsync(c->etas, u, n * 8, hipMemcpyHostToDevice, c->stream));
   // the device matrix is stored with leading dimension n
   int rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, n, itmax, rparm[0],
-                      nullptr, nullptr, nullptr);
+                      nullptr, nullptr, nullptr, false, false);
   if (rc) return rc;
   double saved[8];
   int f[F_COUNT];

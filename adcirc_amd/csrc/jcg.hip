@@ -302,28 +302,35 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   for (int k = 0; k < (RED_PER_THREAD + 3) / 4; ++k) nnp[k] = 0x01010101u;
 #define NN(r) ((int)((nnp[(r) >> 2] >> (8 * ((r) & 3))) & 255u))
 
-  // ---- scal, part 1: diagonal scaling (src/itpackv.F:2374-2393)
+  // ---- scal, part 1: diagonal scaling (src/itpackv.F:2374-2393), unless
+  // the caller's last pass already did it (then w, u and rhs are scaled and
+  // every row's w is in memory before this launch)
   if (!eig_block) {
 #pragma unroll
     for (int r = 0; r < RED_PER_THREAD; ++r) {
       const int i = base + RED_THREADS * r;
       uc[r] = 0.0;
       if (i < np) {
-        const double dg = at(a.coef, i);
-        if (!(dg > 0.0)) atomicOr(err, ERR_DIAG);
-        const double sq = __builtin_sqrt(dg);
-        const double u0 = at(a.u, i) * sq;
-        at(a.u, i) = u0;
-        uc[r] = u0;
-        const double w = 1.0 / sq;
-        at(a.w, i) = w;
-        at(a.rhs, i) = at(a.rhs, i) * w;
+        if (a.prescaled) {
+          uc[r] = at(a.u, i);
+        } else {
+          const double dg = at(a.coef, i);
+          if (!(dg > 0.0)) atomicOr(err, ERR_DIAG);
+          const double sq = __builtin_sqrt(dg);
+          const double u0 = at(a.u, i) * sq;
+          at(a.u, i) = u0;
+          uc[r] = u0;
+          const double w = 1.0 / sq;
+          at(a.w, i) = w;
+          at(a.rhs, i) = at(a.rhs, i) * w;
+        }
         const unsigned sh = 8 * (r & 3);
         nnp[r >> 2] = (nnp[r >> 2] & ~(255u << sh)) | ((unsigned)at(a.nneigh, i) << sh);
       }
     }
   }
-  if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+  if (!a.prescaled)
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
 
   // ---- scal, part 2: scale off-diagonals; initial pseudo-residual
   // d0 = rhs - N u - u (src/itpackv.F:2394-2400, 346-353)
@@ -555,8 +562,9 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       }
       jm = NN(r) > jm ? NN(r) : jm;
     }
-    // off-diagonals, four rows per slot so that their gathers overlap
-    for (int j = 1; j < jm; ++j) {
+    // off-diagonals, four rows per slot so that their gathers overlap;
+    // deferred to the next matrix pass when the caller asks
+    for (int j = 1; j < (a.defer_unscal ? 1 : jm); ++j) {
       const unsigned q0 = (unsigned)(j * np + base);
 #pragma unroll
       for (int r = 0; r < RED_PER_THREAD; ++r) {

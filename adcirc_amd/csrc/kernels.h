@@ -65,7 +65,10 @@ struct DevMesh {
   const int* csrpos;    // [ne*3*2] 0-based column slots
   const int* bnd_index; // [np] elevation-boundary index or -1
   const int* nbd;       // [neta] 0-based
-  const int* obc_slot;  // [mnei*np] boundary-column slot I*mnei+(J-2) or -1
+  const int* obc_q;     // [nobc] COEF positions j*np+i that couple row i to an
+  const int* obc_s;     //        elevation-boundary node, and their OBCCOEF slot
+                        //        I*mnei+(J-2); -1-slot when row i is itself a boundary row
+  int nobc;
   const int* obcg_ptr;  // [np+1] node -> boundary-column slots, (I,J) order
   const int* obcg;
   const int* vbc_ptr;   // [np+1] node -> flow-boundary entries in ME2GW order
@@ -138,6 +141,8 @@ struct JcgArgs {
   double* part;       // [nch*4] chunk partials (d.d, d.Nd, u.u)
   double* saved;      // [8] ITPACK SAVE state q1, gamold, rhoold, cmold, cme,
                       // stptst; [6..7] = cme, cmold of the running iteration
+  int prescaled;      // rows already scaled (w, u, rhs) by k_rhs_node
+  int defer_unscal;   // leave the off-diagonals scaled (k_lhs_rows restores them)
   unsigned* barrier;  // [JCG_BAR_WORDS] grid barrier counters
   long long barrier_ticks;  // longest grid-barrier wait, in wall-clock ticks
   int* flags;
@@ -156,12 +161,21 @@ __global__ void k_holland(int np, const double* x, const double* y, const double
 __global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
                            double* sb);
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
-                           const double* sa, const double* sb, double* ep_part);
-__global__ void k_ep(int nch, int np, const double* part, const int* flags, double* scal);
-__global__ void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* scal);
+                           const double* sa, const double* sb, double* ep_part,
+                           const double* restore_cs, const double* restore_w);
+__global__ void k_lhs_bc(DevMesh m, DevState s, const int* flags, int nch, const double* ep_part,
+                         double* scal);
+__global__ void k_coef_restore(DevMesh m, DevState s, const double* cs, const double* w);
 __global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags, double* contrib);
+// Jacobi scaling of the GWCE rows ahead of k_jcg (its first scaling pass)
+struct JcgPre {
+  double* w;          // 1/sqrt(diag)
+  unsigned* barrier;  // grid barrier words of the coming k_jcg launch, cleared here
+  int barrier_words;
+  int* flags;
+};
 __global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const double* eta_bc,
-                           const double* scal);
+                           const double* scal, JcgPre pre);
 __global__ void k_jcg(JcgArgs a);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);

@@ -135,15 +135,32 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
                                                           const double* __restrict__ sd,
                                                           const double* __restrict__ sa,
                                                           const double* __restrict__ sb,
-                                                          double* __restrict__ ep_part) {
+                                                          double* __restrict__ ep_part,
+                                                          const double* __restrict__ restore_cs,
+                                                          const double* __restrict__ restore_w) {
   __shared__ double lds[8];
   extern __shared__ double racc_lds[];
   double(*racc)[RED_THREADS] = reinterpret_cast<double(*)[RED_THREADS]>(racc_lds);
-  if (flags[F_NCCHANGE] == 0) return;
   const int np = m.np;
   const int t = threadIdx.x;
   const int chunk = xcd_tile(blockIdx.x, gridDim.x);
   const int base = chunk * RED_CHUNK + t;
+  if (flags[F_NCCHANGE] == 0) {
+    // no rebuild: the previous solve left the off-diagonals scaled; restore
+    // them as unscal does (src/itpackv.F:2460-2479), same operations
+    if (restore_cs)
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        const int i = base + RED_THREADS * r;
+        if (i >= np) break;
+        const int nn = m.nneigh[i];
+        const double ri = 1.0 / restore_w[i];
+        for (int j = 1; j < nn; ++j) {
+          const size_t q = (size_t)j * np + i;
+          s.coef[q] = restore_cs[q] * ri * (1.0 / restore_w[m.col[q]]);
+        }
+      }
+    return;
+  }
   double acc = 0.0;
   for (int r = 0; r < RED_PER_THREAD; ++r) {
     const int i = base + RED_THREADS * r;
@@ -166,45 +183,60 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
   if (t == 0) ep_part[chunk] = tot;
 }
 
-// EP = sqrt(sum/NP) from the chunk partials (one block).
-__global__ __launch_bounds__(256) void k_ep(int nch, int np, const double* __restrict__ part,
-                                            const int* __restrict__ flags, double* __restrict__ scal) {
+// EP = sqrt(sum of squared diagonals / NP) from the chunk partials, and the
+// elevation-boundary rows (src/gwce.F:700-743): a boundary row becomes EP on
+// the diagonal with its off-diagonals and OBCCOEF entries zeroed; an entry
+// that couples another row to a boundary node moves to OBCCOEF.  Only the
+// listed positions are touched: one block.  Zero diagonals of dry rows are
+// set to EP by k_rhs_node.
+__global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, const int* flags, int nch,
+                                                const double* __restrict__ ep_part,
+                                                double* __restrict__ scal) {
   __shared__ double lds[4];
+  __shared__ double ep_s;
   if (flags[F_NCCHANGE] == 0) return;
+  const int np = m.np, t = threadIdx.x;
   double acc = 0.0;
-  for (int c = threadIdx.x; c < nch; c += RED_XTHREADS) acc = acc + part[c];
-  double tot = block_tree(acc, lds);
-  if (threadIdx.x == 0) scal[S_EP] = __builtin_sqrt(tot / (double)np);
-}
-
-// Boundary rows/columns and diagonal checks (src/gwce.F:654-707).
-__global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, int* flags, const double* __restrict__ scal) {
-  if (flags[F_NCCHANGE] == 0) return;
-  const int np = m.np;
-  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const double ep = scal[S_EP];
-  const int nn = m.nneigh[i];
+  for (int c = t; c < nch; c += RED_XTHREADS) acc = acc + ep_part[c];
+  const double tot = block_tree(acc, lds);
+  if (t == 0) {
+    ep_s = __builtin_sqrt(tot / (double)np);
+    scal[S_EP] = ep_s;
+  }
+  __syncthreads();
+  const double ep = ep_s;
   double* c = s.coef;
-  if (m.bnd_index[i] >= 0) {
+  for (int k = t; k < m.neta; k += blockDim.x) {
+    const int i = m.nbd[k];
     c[i] = ep;
-    for (int j = 1; j < nn; ++j) {
-      c[(size_t)j * np + i] = 0.0;
-      const int slot = m.obc_slot[(size_t)j * np + i];
-      if (slot >= 0) s.obc[slot] = 0.0;
-    }
-  } else {
-    for (int j = 1; j < nn; ++j) {
-      const int slot = m.obc_slot[(size_t)j * np + i];
-      if (slot >= 0) {
-        s.obc[slot] = c[(size_t)j * np + i];
-        c[(size_t)j * np + i] = 0.0;
-      }
+    const int nn = m.nneigh[i];
+    for (int j = 1; j < nn; ++j) c[(size_t)j * np + i] = 0.0;
+  }
+  for (int k = t; k < m.nobc; k += blockDim.x) {
+    const int sl = m.obc_s[k];
+    if (sl < 0) {
+      s.obc[-1 - sl] = 0.0;
+    } else {
+      const int q = m.obc_q[k];
+      s.obc[sl] = c[q];
+      c[q] = 0.0;
     }
   }
-  double d = c[i];
-  if (d == 0.0) c[i] = ep;
-  if (d < 0.0) atomicOr(&flags[F_ERROR], ERR_DIAG);
+}
+
+// The off-diagonals of a solve whose unscaling was deferred, restored as
+// unscal does (src/itpackv.F:2460-2479).
+__global__ __launch_bounds__(256) void k_coef_restore(DevMesh m, DevState s,
+                                                      const double* __restrict__ cs,
+                                                      const double* __restrict__ w) {
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= m.np) return;
+  const int np = m.np, nn = m.nneigh[i];
+  const double ri = 1.0 / w[i];
+  for (int j = 1; j < nn; ++j) {
+    const size_t q = (size_t)j * np + i;
+    s.coef[q] = cs[q] * ri * (1.0 / w[m.col[q]]);
+  }
 }
 
 // --------------------------------------------------------------- GWCE RHS
@@ -323,29 +355,47 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
 // boundary conditions (src/gwce.F:1613-1631, 1638-1651, 1846-1855).  `eta1`
 // already points at the previous ETA2; `eta2` is rewritten.
 __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ contrib,
-                           const double* __restrict__ eta_bc, const double* __restrict__ scal) {
+                           const double* __restrict__ eta_bc, const double* __restrict__ scal,
+                           JcgPre pre) {
   int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i < pre.barrier_words) pre.barrier[i] = 0u;
   if (i >= m.np) return;
   const int bi = m.bnd_index[i];
+  double rhs, u;
   if (bi >= 0) {
     const double e2 = eta_bc[bi];
     const double es = e2 - s.eta1[i];
     s.eta2[i] = e2;
-    s.etas[i] = es;
-    s.gwce_lv[i] = es * (double)s.nodecode[i] * scal[S_EP];
-    return;
+    u = es;
+    rhs = es * (double)s.nodecode[i] * scal[S_EP];
+  } else {
+    double acc = 0.0;
+    for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
+    for (int p = m.obcg_ptr[i]; p < m.obcg_ptr[i + 1]; ++p) {
+      const int slot = m.obcg[p];
+      const int I = slot / m.mnei;
+      const int b = m.nbd[I];
+      const double es = eta_bc[I] - s.eta1[b];
+      acc = acc - es * s.obc[slot];
+    }
+    rhs = acc;
+    u = s.etas[i];
+    s.eta2[i] = 0.0;
   }
-  double acc = 0.0;
-  for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
-  for (int p = m.obcg_ptr[i]; p < m.obcg_ptr[i + 1]; ++p) {
-    const int slot = m.obcg[p];
-    const int I = slot / m.mnei;
-    const int b = m.nbd[I];
-    const double es = eta_bc[I] - s.eta1[b];
-    acc = acc - es * s.obc[slot];
+  // a zero diagonal (a dry row) becomes EP (src/gwce.F:737-741); then the
+  // first scaling pass of JCG (src/itpackv.F:2374-2393): w = 1/sqrt(diag),
+  // u and rhs scaled in place
+  double dg = s.coef[i];
+  if (dg == 0.0) {
+    dg = scal[S_EP];
+    s.coef[i] = dg;
   }
-  s.gwce_lv[i] = acc;
-  s.eta2[i] = 0.0;
+  if (!(dg > 0.0)) atomicOr(&pre.flags[F_ERROR], ERR_DIAG);
+  const double sq = __builtin_sqrt(dg);
+  const double w = 1.0 / sq;
+  s.etas[i] = u * sq;
+  pre.w[i] = w;
+  s.gwce_lv[i] = rhs * w;
 }
 
 // ---------------------------------------------------------------- wet/dry
