@@ -620,11 +620,15 @@ int adcgpu_begin_step(adcgpu_ctx* c, int it) {
   return check_launch("begin_step");
 }
 
+static FricPrm fric_prm(const adcgpu_ctx* c) {
+  return FricPrm{c->uu1, c->vv1, c->dp, c->eta2, c->fric, c->ifnlfa, c->iflinbf, c->ifnlbf,
+                 c->tkmax};
+}
+
 int adcgpu_friction(adcgpu_ctx* c) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
-  hipLaunchKernelGGL(k_friction, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->uu1,
-                     c->vv1, c->dp, c->eta2, c->fric, c->ifnlfa, c->iflinbf, c->ifnlbf, c->tkmax,
-                     c->tk);
+  hipLaunchKernelGGL(k_friction, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np,
+                     fric_prm(c), c->tk);
   return check_launch("friction");
 }
 
@@ -690,9 +694,9 @@ int adcgpu_solve_gwce(adcgpu_ctx* c, int it, const double* eta_bc) {
   return gwce_device(c, c->eta_bc, false);
 }
 
-int adcgpu_wetdry(adcgpu_ctx* c, int it) {
-  (void)it;
-  if (!c) return fail(ADCGPU_E_ARG, "null context");
+// Wet/dry; without `alpha_pass` the ALPHAL pass is left to the momentum
+// element pass (the resident run).
+static int wetdry_device(adcgpu_ctx* c, bool alpha_pass) {
   const int np = c->np, ne = c->ne;
   if (c->prm.nolifa != 2) {
     hipLaunchKernelGGL(k_h2, dim3(nblk(np, 256)), dim3(256), 0, c->stream, np, c->dp, c->eta2,
@@ -720,19 +724,35 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
                      c->state(), c->cx);
   hipLaunchKernelGGL(k_wd_d2, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->cx, c->flags, c->ifnlfa);
-  hipLaunchKernelGGL(k_wd_alpha, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), c->flags, 1.0e9);
+  if (alpha_pass)
+    hipLaunchKernelGGL(k_wd_alpha, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                       c->state(), c->flags, 1.0e9);
   return check_launch("wetdry");
+}
+
+int adcgpu_wetdry(adcgpu_ctx* c, int it) {
+  (void)it;
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  return wetdry_device(c, true);
+}
+
+// Momentum; with `alpha` the element pass forms ALPHAL first.
+static int momentum_device(adcgpu_ctx* c, bool alpha) {
+  MomPrm q{c->ifnlct, c->prm.nws};
+  if (alpha)
+    hipLaunchKernelGGL(k_mom_elem<true>, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), q, c->cx, c->cy, c->flags);
+  else
+    hipLaunchKernelGGL(k_mom_elem<false>, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), q, c->cx, c->cy, c->flags);
+  hipLaunchKernelGGL(k_mom_node, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), q, c->cx, c->cy);
+  return check_launch("momentum");
 }
 
 int adcgpu_momentum(adcgpu_ctx* c) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
-  MomPrm q{c->ifnlct, c->prm.nws};
-  hipLaunchKernelGGL(k_mom_elem, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), q, c->cx, c->cy);
-  hipLaunchKernelGGL(k_mom_node, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), q, c->cx, c->cy);
-  return check_launch("momentum");
+  return momentum_device(c, false);
 }
 
 int adcgpu_timestep(adcgpu_ctx* c, int it, const double* eta_bc, const double* wsx2,
@@ -770,7 +790,7 @@ int adcgpu_holland(adcgpu_ctx* c, double t, const double* prm13) {
   for (int k = 0; k < 13; ++k) p.v[k] = prm13[k];
   p.t = t;
   hipLaunchKernelGGL(k_holland, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->x,
-                     c->y, (const double*)nullptr, 0, p, c->wsx2, c->wsy2, c->pr2);
+                     c->y, p, c->wsx2, c->wsy2, c->pr2);
   return check_launch("holland");
 }
 
@@ -796,12 +816,25 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   for (int s = 0; s < nsteps; ++s) {
     const int it = it0 + s;
     int rc = adcgpu_begin_step(c, it);
-    if (!rc) rc = adcgpu_friction(c);
-    if (!rc && holland) rc = adcgpu_holland(c, it * c->prm.dtdp, holland);
+    if (!rc) {
+      if (holland) {
+        // friction and this step's forcing in one pass
+        HollandPrm hp;
+        for (int k = 0; k < 13; ++k) hp.v[k] = holland[k];
+        hp.t = it * c->prm.dtdp;
+        hipLaunchKernelGGL(k_friction_holland, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream,
+                           c->np, fric_prm(c), c->tk, c->x, c->y, hp, c->wsx2, c->wsy2, c->pr2);
+        rc = check_launch("friction and forcing");
+      } else {
+        rc = adcgpu_friction(c);
+      }
+    }
     if (!rc)
       rc = gwce_device(c, c->neta > 0 ? c->eta_bc_table + (size_t)s * c->neta : c->eta_bc, true);
-    if (!rc) rc = adcgpu_wetdry(c, it);
-    if (!rc) rc = adcgpu_momentum(c);
+    // with wet/dry the ALPHAL pass rides on the momentum element pass
+    const bool wd = c->prm.nolifa == 2;
+    if (!rc) rc = wetdry_device(c, !wd);
+    if (!rc) rc = momentum_device(c, wd);
     if (rc) return rc;
     if ((s & 63) == 63) {
       int f[F_COUNT];

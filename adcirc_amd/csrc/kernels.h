@@ -127,6 +127,11 @@ struct HollandPrm {
   double t;
 };
 
+struct FricPrm {
+  const double *uu1, *vv1, *dp, *eta2, *fric;
+  double ifnlfa, iflinbf, ifnlbf, tkmax;
+};
+
 struct JcgArgs {
   int n, itmax, maxfn;
   double zeta, tz;
@@ -153,11 +158,11 @@ struct JcgArgs {
 
 __global__ void k_init_h(int np, const double* dp, const double* eta2, double ifnlfa, double* h1,
                          double* h2);
-__global__ void k_friction(int np, const double* uu1, const double* vv1, const double* dp,
-                           const double* eta2, const double* fric, double ifnlfa, double iflinbf,
-                           double ifnlbf, double tkmax, double* tk);
-__global__ void k_holland(int np, const double* x, const double* y, const double* tptr, int tstep,
-                          HollandPrm p, double* wsx, double* wsy, double* pr);
+__global__ void k_friction(int np, FricPrm f, double* tk);
+__global__ void k_holland(int np, const double* x, const double* y, HollandPrm p, double* wsx,
+                          double* wsy, double* pr);
+__global__ void k_friction_holland(int np, FricPrm f, double* tk, const double* x, const double* y,
+                                   HollandPrm p, double* wsx, double* wsy, double* pr);
 __global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
                            double* sb);
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
@@ -188,7 +193,8 @@ __global__ void k_wd_area(DevMesh m, DevState s, double* area_slot);
 __global__ void k_wd_d2(DevMesh m, DevState s, const double* area_slot, int* flags, double ifnlfa);
 __global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim);
 __global__ void k_h2(int np, const double* dp, const double* eta2, double ifnlfa, double* h2);
-__global__ void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* cx, double* cy);
+template <bool ALPHA>
+__global__ void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* cx, double* cy, int* flags);
 __global__ void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* cx, const double* cy);
 __global__ void k_fill_u32(unsigned* p, unsigned v, int n);
 __global__ void k_fill_i32(int* p, int v, int n);

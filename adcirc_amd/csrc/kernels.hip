@@ -30,30 +30,29 @@ __global__ __launch_bounds__(256) void k_init_h(int np, const double* __restrict
 // Apply2DBottomFriction (src/nodalattr.F:2400-2411) for IFHYBF = 0, and the
 // TKM = (TK, TK, 0) of Apply2DInternalWaveDrag without wave drag; TKM is
 // represented by TK alone.
-__global__ __launch_bounds__(256) void k_friction(int np, const double* __restrict__ uu1, const double* __restrict__ vv1,
-                           const double* __restrict__ dp, const double* __restrict__ eta2,
-                           const double* __restrict__ fric, double ifnlfa, double iflinbf,
-                           double ifnlbf, double tkmax, double* __restrict__ tk) {
+__device__ __forceinline__ void friction_node(int i, const FricPrm& f, double* __restrict__ tk) {
+  double u = f.uu1[i], v = f.vv1[i];
+  double uv1 = __builtin_sqrt(u * u + v * v);
+  double h1 = f.dp[i] + f.ifnlfa * f.eta2[i];
+  double hyb = 0.0;  // IFHYBF * (...) with IFHYBF = 0
+  double a = f.fric[i] * (f.iflinbf + (uv1 / h1) * (f.ifnlbf + hyb));
+  tk[i] = fmin(a, f.tkmax);
+}
+
+__global__ __launch_bounds__(256) void k_friction(int np, FricPrm f, double* __restrict__ tk) {
   int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  double u = uu1[i], v = vv1[i];
-  double uv1 = __builtin_sqrt(u * u + v * v);
-  double h1 = dp[i] + ifnlfa * eta2[i];
-  double hyb = 0.0;  // IFHYBF * (...) with IFHYBF = 0
-  double a = fric[i] * (iflinbf + (uv1 / h1) * (ifnlbf + hyb));
-  tk[i] = fmin(a, tkmax);
+  friction_node(i, f, tk);
 }
 
 // Holland (1980) vortex forcing on the device (synthetic storms for the
 // resident benchmark run).  Same formulas as the oracle's host version; device
 // exp/pow/sqrt differ from host libm by a few ulp.
-__global__ __launch_bounds__(256) void k_holland(int np, const double* __restrict__ x, const double* __restrict__ y,
-                          const double* __restrict__ tptr, int tstep, HollandPrm p,
-                          double* __restrict__ wsx, double* __restrict__ wsy,
-                          double* __restrict__ pr) {
-  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  double t = tptr ? tptr[0] : p.t;
+__device__ __forceinline__ void holland_node(int i, const double* __restrict__ x,
+                                             const double* __restrict__ y, const HollandPrm& p,
+                                             double* __restrict__ wsx, double* __restrict__ wsy,
+                                             double* __restrict__ pr) {
+  const double t = p.t;
   const double pc = p.v[0] * 100.0, pn = p.v[1] * 100.0, rmax = p.v[2], B = p.v[3];
   const double xc = p.v[4] + p.v[6] * t, yc = p.v[5] + p.v[7] * t, f = p.v[8];
   const double rampd = p.v[9], rhoa = p.v[10], wfac = p.v[11];
@@ -75,11 +74,31 @@ __global__ __launch_bounds__(256) void k_holland(int np, const double* __restric
   double wm = __builtin_sqrt(wx * wx + wy * wy);
   double cd = 0.001 * (0.75 + 0.067 * wm);
   if (cd > 0.0035) cd = 0.0035;
-  double s = ramp * cd * (rhoa / rho0) * wm;
-  wsx[i] = s * wx;
-  wsy[i] = s * wy;
+  double sc = ramp * cd * (rhoa / rho0) * wm;
+  wsx[i] = sc * wx;
+  wsy[i] = sc * wy;
   pr[i] = (pn + ramp * (pp - pn)) / (rho0 * g);
-  (void)tstep;
+}
+
+__global__ __launch_bounds__(256) void k_holland(int np, const double* __restrict__ x, const double* __restrict__ y,
+                          HollandPrm p, double* __restrict__ wsx, double* __restrict__ wsy,
+                          double* __restrict__ pr) {
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  holland_node(i, x, y, p, wsx, wsy, pr);
+}
+
+// Friction and the step's Holland forcing in one pass (the resident run).
+__global__ __launch_bounds__(256) void k_friction_holland(int np, FricPrm f, double* __restrict__ tk,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ y,
+                                                          HollandPrm p, double* __restrict__ wsx,
+                                                          double* __restrict__ wsy,
+                                                          double* __restrict__ pr) {
+  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  friction_node(i, f, tk);
+  holland_node(i, x, y, p, wsx, wsy, pr);
 }
 
 // --------------------------------------------------------------- GWCE LHS
@@ -900,10 +919,9 @@ __global__ __launch_bounds__(256) void k_wd_d2(DevMesh m, DevState s,
 }
 
 // NOFF changes count toward NCCHANGE (src/wetdry.F:1428-1430), and
-// computeAlpha (src/wetdry.F:1561-1652).
-__global__ __launch_bounds__(256) void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim) {
-  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e >= m.ne) return;
+// computeAlpha (src/wetdry.F:1561-1652): ALPHAL of element e.
+__device__ __forceinline__ double wd_alpha(const DevMesh& m, const DevState& s, int e, int* flags,
+                                           double slim) {
   if (s.noff[e] != s.noffold[e]) atomicOr(&flags[F_NCCHANGE], 1);
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   double al = 1.0;
@@ -933,6 +951,13 @@ __global__ __launch_bounds__(256) void k_wd_alpha(DevMesh m, DevState s, int* fl
     }
   }
   s.alphal[e] = al;
+  return al;
+}
+
+__global__ __launch_bounds__(256) void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim) {
+  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  wd_alpha(m, s, e, flags, slim);
 }
 
 __global__ __launch_bounds__(256) void k_h2(int np, const double* __restrict__ dp, const double* __restrict__ eta2,
@@ -945,10 +970,14 @@ __global__ __launch_bounds__(256) void k_h2(int np, const double* __restrict__ d
 // --------------------------------------------------------------- momentum
 // Element loop of Mom_Eqs_New_NC (src/momentum.F:298-699): advection,
 // barotropic pressure gradient, IBPV lateral stress; contributions x NCEle.
+// With ALPHA the element's wet/dry ALPHAL is formed here first (the resident
+// run folds k_wd_alpha into this pass; nothing reads ALPHAL in between).
+template <bool ALPHA>
 __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* __restrict__ cx,
-                           double* __restrict__ cy) {
+                           double* __restrict__ cy, int* flags) {
   int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
+  const double al = ALPHA ? wd_alpha(m, s, e, flags, 1.0e9) : s.alphal[e];
   const double THIRD = 1.0 / 3.0, DT = m.dt, GO2 = m.g / 2.0;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
@@ -966,7 +995,6 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
   const double DV1DXA = (V1N1 * FDX1 + V1N2 * FDX2 + V1N3 * FDX3) * 0.5;
   const double DV1DYA = (V1N1 * FDY1 + V1N2 * FDY2 + V1N3 * FDY3) * 0.5;
   const double EVMEle = THIRD * (m.evm[n1] + m.evm[n2] + m.evm[n3]);
-  const double al = s.alphal[e];
   double B1N1 = al * s.eta1[n1], B2N1 = al * s.eta2[n1];
   double B1N2 = al * s.eta1[n2], B2N2 = al * s.eta2[n2];
   double B1N3 = al * s.eta1[n3], B2N3 = al * s.eta2[n3];
@@ -1002,6 +1030,9 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
     cy[sl] = tb;
   }
 }
+
+template __global__ void k_mom_elem<false>(DevMesh, DevState, MomPrm, double*, double*, int*);
+template __global__ void k_mom_elem<true>(DevMesh, DevState, MomPrm, double*, double*, int*);
 
 // Nodal part of Mom_Eqs_New_NC (src/momentum.F:713-1182): gather, /TotalArea,
 // wind, friction/Coriolis 2x2, land-boundary constraints, solve, fluxes;

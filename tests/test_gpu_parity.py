@@ -113,6 +113,56 @@ def test_holland_forcing_vs_oracle():
             assert err <= 1e-13, (t, name)
 
 
+def test_coastal_resident_run_equals_per_call_api():
+    """The resident run (adcgpu_run: friction and device Holland forcing in one
+    pass, the wet/dry ALPHAL pass inside the momentum element pass, COEF's
+    off-diagonals left scaled between solves) against the same steps made one
+    ABI call at a time (adcgpu_begin_step, _friction, _holland, _solve_gwce,
+    _wetdry, _momentum) on the 20k-node coastal storm: every state array,
+    COEF and ALPHAL identical after 300 steps of wetting and drying."""
+    import ctypes as C
+    from gpu_helpers import case_setup, golden_tables
+    from adcirc_amd.model import Model
+    case = meshgen.coastal(20000)
+    d = case_setup(case)
+    run = case.run
+    t = golden_tables(d)
+    nb = run.nbfr
+
+    def model():
+        g = Model(t, run, tau0var=d["tau0var"], evm=d["evm"], fric=d["fric"],
+                  corif=np.full(t["np"], run.cori),
+                  emo=np.asarray(d["emo"]).reshape(t["neta"], nb),
+                  efa=np.asarray(d["efa"]).reshape(t["neta"], nb))
+        g.cold_start(d["eta2_0"], d["nodecode0"], d["noff0"])
+        g.holland(0.0, run.holland)
+        return g
+
+    n = 300
+    log(f"case coastal {t['np']} nodes: {n} steps resident vs per call")
+    g1, g2 = model(), model()
+    g1.run_resident(1, n, holland=run.holland)
+    hol = np.ascontiguousarray(list(run.holland) + [0.0] * (13 - len(run.holland)), np.float64)
+    dp_ = C.POINTER(C.c_double)
+    lib, h = g2.lib, g2.h
+    for it in range(1, n + 1):
+        bc = np.ascontiguousarray(g2.eta_bc(it), np.float64)
+        assert lib.adcgpu_begin_step(h, it) == 0
+        assert lib.adcgpu_friction(h) == 0
+        assert lib.adcgpu_holland(h, it * run.dtdp, hol.ctypes.data_as(dp_)) == 0
+        assert lib.adcgpu_solve_gwce(h, it, bc.ctypes.data_as(dp_)) == 0
+        assert lib.adcgpu_wetdry(h, it) == 0
+        assert lib.adcgpu_momentum(h) == 0
+    # steps that rebuild COEF and steps that restore it from its scaled form
+    # (the basin's resident test covers runs that never rebuild)
+    log(f"  COEF rebuilt on {g1.last_rebuilds} of {n} steps")
+    assert g1.last_rebuilds > 0
+    for k in ("eta1", "eta2", "etas", "uu2", "vv2", "qx2", "qy2", "tk", "etamax", "coef",
+              "alphal", "nodecode", "noff"):
+        a, b = g1.get(k), g2.get(k)
+        assert np.array_equal(a, b), (k, int((a != b).sum()))
+
+
 def test_basin_full_day_resident_vs_reference(basin_golden):
     """43,200 steps through the resident run (one launch sequence per step,
     elevation forcing from a precomputed table): final state within the
