@@ -1,9 +1,10 @@
 """Per-kernel summary of rocprofv3 kernel-trace databases of a resident run
 (profiling aid; reads the rocpd SQLite files rocprofv3 writes).
 
-Each time step of ``adcgpu_run`` starts with ``k_friction`` and makes exactly
+Each time step of ``adcgpu_run`` starts with the friction pass
+(``k_friction``, or ``k_friction_holland`` with device forcing), makes exactly
 one ``k_jcg`` launch and ends with ``k_mom_node``, so a dispatch belongs to
-the step of the last ``k_friction`` before it.  A database whose window opens
+the step of the last friction pass before it.  A database whose window opens
 with the run is numbered forward from its first step, one whose window closes
 with the run backward from the last.  Several databases recorded over
 overlapping windows of the same run (``rocprofv3 -P``) are stitched: a step
@@ -26,18 +27,19 @@ from collections import defaultdict
 
 
 def _name(full: str) -> str:
-    return full.replace("(anonymous namespace)::", "").split("(")[0].replace("adcgpu::", "")
+    n = full.replace("(anonymous namespace)::", "").split("(")[0].replace("adcgpu::", "")
+    return n[5:] if n.startswith("void ") else n
 
 
 def _groups(path: str):
     rows = sqlite3.connect(path).execute(
         "select name, start, end from kernels order by start").fetchall()
-    # a step opens at k_friction; dispatches before the first one belong to
-    # a step cut off by the window's start
+    # a step opens at the friction pass; dispatches before the first one
+    # belong to a step cut off by the window's start
     groups, cur = [], None
     for name, t0, t1 in rows:
         name = _name(name)
-        if name == "k_friction":
+        if name in ("k_friction", "k_friction_holland"):
             cur = []
             groups.append(cur)
         if cur is not None:
