@@ -455,9 +455,13 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   if (!rc) {
     // the LHS row kernel keeps MNEI x 512 doubles of LDS per block; the
     // wetting propagation stages candidates through LDS
-    hipError_t e = hipFuncSetAttribute((const void*)k_lhs_rows,
+    hipError_t e = hipFuncSetAttribute((const void*)k_lhs_rows<false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        mnei * RED_THREADS * (int)sizeof(double));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_lhs_rows<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              mnei * RED_THREADS * (int)sizeof(double));
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)k_wd_link, hipFuncAttributeMaxDynamicSharedMemorySize,
                               WD_LINK_LDS);
@@ -646,25 +650,44 @@ int adcgpu_set_forcing(adcgpu_ctx* c, const double* wsx2, const double* wsy2, co
 // restores them; the resident run uses it, since most steps rebuild COEF.
 static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   const int np = c->np, ne = c->ne;
+  const bool lumped = c->prm.ilump != 0;
+  const size_t rows_lds = (size_t)c->mnei * RED_THREADS * sizeof(double);
+  const double* rcs = c->coef_scaled ? (const double*)c->cs : nullptr;
+  const double* rw = c->coef_scaled ? (const double*)c->jw : nullptr;
   // contrib, cx and cy are free until the GWCE right-hand side
-  hipLaunchKernelGGL(k_lhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), c->flags, c->contrib, c->cx, c->cy);
-  hipLaunchKernelGGL(k_lhs_rows, dim3(c->nch), dim3(RED_THREADS),
-                     (size_t)c->mnei * RED_THREADS * sizeof(double), c->stream, c->mesh(),
-                     c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
-                     c->coef_scaled ? (const double*)c->cs : nullptr,
-                     c->coef_scaled ? (const double*)c->jw : nullptr);
+  if (lumped) {
+    hipLaunchKernelGGL(k_lhs_elem<true>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
+    hipLaunchKernelGGL(k_lhs_rows<true>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
+                       rcs, rw);
+    hipLaunchKernelGGL(k_lhs_bc<true>, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(),
+                       c->state(), c->flags, c->nch, c->ep_part, c->scal);
+  } else {
+    hipLaunchKernelGGL(k_lhs_elem<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
+    hipLaunchKernelGGL(k_lhs_rows<false>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
+                       rcs, rw);
+    hipLaunchKernelGGL(k_lhs_bc<false>, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(),
+                       c->state(), c->flags, c->nch, c->ep_part, c->scal);
+  }
   c->coef_scaled = false;
-  hipLaunchKernelGGL(k_lhs_bc, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(), c->state(),
-                     c->flags, c->nch, c->ep_part, c->scal);
-  RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, c->prm.nws};
+  RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, lumped ? 4.0 : 2.0, lumped ? 0.0 : 1.0,
+           lumped ? 1 : 0, c->prm.nws};
   hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), q, c->flags, c->contrib);
   // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
   std::swap(c->eta1, c->eta2);
   std::swap(c->h1, c->h2);
   JcgPre pre{c->jw, c->jbar, jcg_barrier_words(c), c->flags};
-  hipLaunchKernelGGL(k_rhs_node, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
+  if (lumped) {
+    // the explicit solve of the lumped GWCE ends the node pass; no JCG
+    hipLaunchKernelGGL(k_rhs_node<true>, dim3(nblk(np, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), c->contrib, d_eta_bc, c->scal, pre);
+    return check_launch("gwce assembly and lumped solve");
+  }
+  hipLaunchKernelGGL(k_rhs_node<false>, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->contrib, d_eta_bc, c->scal, pre);
   int rc = check_launch("gwce assembly");
   if (rc) return rc;

@@ -87,6 +87,8 @@ struct DevState {
 
 struct RhsPrm {
   double ifnlfa, ifnlct, ifnlcat;
+  double ondiag, offdiag;  // (1+ILump)*2 and 1-ILump (src/gwce.F:419-420)
+  int lumped;              // ILump = 1: no ETA1 gradient term (src/gwce.F:1233-1238)
   int nws;
 };
 
@@ -163,11 +165,16 @@ __global__ void k_holland(int np, const double* x, const double* y, HollandPrm p
                           double* wsy, double* pr);
 __global__ void k_friction_holland(int np, FricPrm f, double* tk, const double* x, const double* y,
                                    HollandPrm p, double* wsx, double* wsy, double* pr);
+// LUMPED: the lumped GWCE (ILump = 1), whose LHS is the diagonal COEFD only
+// (kept in COEF's diagonal column) and whose solve is explicit
+template <bool LUMPED>
 __global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
                            double* sb);
+template <bool LUMPED>
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
                            const double* sa, const double* sb, double* ep_part,
                            const double* restore_cs, const double* restore_w);
+template <bool LUMPED>
 __global__ void k_lhs_bc(DevMesh m, DevState s, const int* flags, int nch, const double* ep_part,
                          double* scal);
 __global__ void k_coef_restore(DevMesh m, DevState s, const double* cs, const double* w);
@@ -179,6 +186,7 @@ struct JcgPre {
   int barrier_words;
   int* flags;
 };
+template <bool LUMPED>
 __global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const double* eta_bc,
                            const double* scal, JcgPre pre);
 __global__ void k_jcg(JcgArgs a);

@@ -110,6 +110,7 @@ __global__ __launch_bounds__(256) void k_friction_holland(int np, FricPrm f, dou
 // GWCE matrix, element part (src/gwce.F:456-592): for element e and its local
 // vertex k, the diagonal term and the two off-diagonal terms of row NM(e,k),
 // written to the row's incidence slot p = slot[3e+k] (sd, sa, sb).
+template <bool LUMPED>
 __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
                                                   const int* __restrict__ flags,
                                                   double* __restrict__ sd, double* __restrict__ sa,
@@ -121,6 +122,15 @@ __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
   const double NCELE = (double)ncele;
+  if (LUMPED) {
+    // COEFD: MsFacLOnDiag*SPM*NCELE with OnDiag = 4 (src/gwce.F:596-635)
+    const double area = m.areas[e] * 0.5;
+    const double t0avg = (m.tau0[n1] + m.tau0[n2] + m.tau0[n3]) / 3.0;
+    const double mson = (4.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sd[m.slot[3 * e + k]] = mson * 1.0 * NCELE;
+    return;
+  }
   const double fx[3] = {m.fdx[3 * e], m.fdx[3 * e + 1], m.fdx[3 * e + 2]};
   const double fy[3] = {m.fdy[3 * e], m.fdy[3 * e + 1], m.fdy[3 * e + 2]};
   const double area2 = m.areas[e];
@@ -143,12 +153,15 @@ __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
     sb[p] = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[kb] + 1.0 * fy[k] * fy[kb])) * NCELE;
   }
 }
+template __global__ void k_lhs_elem<false>(DevMesh, DevState, const int*, double*, double*, double*);
+template __global__ void k_lhs_elem<true>(DevMesh, DevState, const int*, double*, double*, double*);
 
 // GWCE matrix, row part: row i of COEF is the sum of its incidence slots in
 // ascending element order, each row held in LDS while it is formed (the
 // reference adds into COEF(i,j) element by element, src/gwce.F:538-592);
 // also the EP chunk partials of sum COEF(i,1)^2 (src/gwce.F:646-650) in the
 // fixed blocked tree.  Dynamic LDS: MNEI x RED_THREADS doubles, slot-major.
+template <bool LUMPED>
 __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
                                                           const int* __restrict__ flags,
                                                           const double* __restrict__ sd,
@@ -185,6 +198,15 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
     const int i = base + RED_THREADS * r;
     if (i >= np) break;
     const int nn = m.nneigh[i];
+    if (LUMPED) {
+      // COEFD only; the off-diagonal columns stay zero
+      double diag = 0.0;
+      for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) diag = diag + sd[p];
+      s.coef[i] = diag;
+      for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = 0.0;
+      acc = acc + diag * diag;
+      continue;
+    }
     for (int j = 1; j < nn; ++j) racc[j][t] = 0.0;
     double diag = 0.0;
     for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
@@ -201,6 +223,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
   double tot = chunk_tree(acc, lds);
   if (t == 0) ep_part[chunk] = tot;
 }
+template __global__ void k_lhs_rows<false>(DevMesh, DevState, const int*, const double*, const double*, const double*, double*, const double*, const double*);
+template __global__ void k_lhs_rows<true>(DevMesh, DevState, const int*, const double*, const double*, const double*, double*, const double*, const double*);
 
 // EP = sqrt(sum of squared diagonals / NP) from the chunk partials, and the
 // elevation-boundary rows (src/gwce.F:700-743): a boundary row becomes EP on
@@ -208,6 +232,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
 // that couples another row to a boundary node moves to OBCCOEF.  Only the
 // listed positions are touched: one block.  Zero diagonals of dry rows are
 // set to EP by k_rhs_node.
+template <bool LUMPED>
 __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, const int* flags, int nch,
                                                 const double* __restrict__ ep_part,
                                                 double* __restrict__ scal) {
@@ -231,6 +256,7 @@ __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, const int
     const int nn = m.nneigh[i];
     for (int j = 1; j < nn; ++j) c[(size_t)j * np + i] = 0.0;
   }
+  if (LUMPED) return;  // a lumped LHS has no boundary columns
   for (int k = t; k < m.nobc; k += blockDim.x) {
     const int sl = m.obc_s[k];
     if (sl < 0) {
@@ -242,6 +268,8 @@ __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, const int
     }
   }
 }
+template __global__ void k_lhs_bc<false>(DevMesh, DevState, const int*, int, const double*, double*);
+template __global__ void k_lhs_bc<true>(DevMesh, DevState, const int*, int, const double*, double*);
 
 // The off-diagonals of a solve whose unscaling was deferred, restored as
 // unscal does (src/itpackv.F:2460-2479).
@@ -301,8 +329,8 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
   const double AreaIE4 = 2.0 * AreaIE2;
   const double FDX1 = m.fdx[3 * e], FDX2 = m.fdx[3 * e + 1], FDX3 = m.fdx[3 * e + 2];
   const double FDY1 = m.fdy[3 * e], FDY2 = m.fdy[3 * e + 1], FDY3 = m.fdy[3 * e + 2];
-  const double E0XGrad2A = E0N1 * FDX1 + E0N2 * FDX2 + E0N3 * FDX3;
-  const double E0YGrad2A = E0N1 * FDY1 + E0N2 * FDY2 + E0N3 * FDY3;
+  const double E0XGrad2A = q.lumped ? 0.0 : E0N1 * FDX1 + E0N2 * FDX2 + E0N3 * FDX3;
+  const double E0YGrad2A = q.lumped ? 0.0 : E0N1 * FDY1 + E0N2 * FDY2 + E0N3 * FDY3;
   const double E1XGrad2A = E1N1 * FDX1 + E1N2 * FDX2 + E1N3 * FDX3;
   const double E1YGrad2A = E1N1 * FDY1 + E1N2 * FDY2 + E1N3 * FDY3;
   const double Tau0XGrad2A = 1.0 * (T0N1 * FDX1 + T0N2 * FDX2 + T0N3 * FDX3);
@@ -352,15 +380,15 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
           q.ifnlct * (QX1Avg * (V1N1 * FDX1 + V1N2 * FDX2 + V1N3 * FDX3) +
                       QY1Avg * (V1N1 * FDY1 + V1N2 * FDY2 + V1N3 * FDY3)) / AreaIE2 +
           q.ifnlcat * V1Avg * ESAvg / DT;
-  const double A1 = (2.0 * ESN1 * 1.0 + 1.0 * (ESN2 * 1.0 + ESN3 * 1.0)) * MsFacR -
+  const double A1 = (q.ondiag * ESN1 * 1.0 + q.offdiag * (ESN2 * 1.0 + ESN3 * 1.0)) * MsFacR -
                     GDP * (C00 * (1.0 * FDX1 * E0XGrad2A + 1.0 * FDY1 * E0YGrad2A)) -
                     GDP * (A00pB00 * (1.0 * FDX1 * E1XGrad2A + 1.0 * FDY1 * E1YGrad2A)) +
                     (1.0 * JXAvg * FDX1 + 1.0 * JYAvg * FDY1) * 0.5 + Tau0SpaVar;
-  const double A2 = (2.0 * ESN2 * 1.0 + 1.0 * (ESN1 * 1.0 + ESN3 * 1.0)) * MsFacR -
+  const double A2 = (q.ondiag * ESN2 * 1.0 + q.offdiag * (ESN1 * 1.0 + ESN3 * 1.0)) * MsFacR -
                     GDP * (C00 * (1.0 * FDX2 * E0XGrad2A + 1.0 * FDY2 * E0YGrad2A)) -
                     GDP * (A00pB00 * (1.0 * FDX2 * E1XGrad2A + 1.0 * FDY2 * E1YGrad2A)) +
                     (1.0 * JXAvg * FDX2 + 1.0 * JYAvg * FDY2) * 0.5 + Tau0SpaVar;
-  const double A3 = (2.0 * ESN3 * 1.0 + 1.0 * (ESN1 * 1.0 + ESN2 * 1.0)) * MsFacR -
+  const double A3 = (q.ondiag * ESN3 * 1.0 + q.offdiag * (ESN1 * 1.0 + ESN2 * 1.0)) * MsFacR -
                     GDP * (C00 * (1.0 * FDX3 * E0XGrad2A + 1.0 * FDY3 * E0YGrad2A)) -
                     GDP * (A00pB00 * (1.0 * FDX3 * E1XGrad2A + 1.0 * FDY3 * E1YGrad2A)) +
                     (1.0 * JXAvg * FDX3 + 1.0 * JYAvg * FDY3) * 0.5 + Tau0SpaVar;
@@ -373,6 +401,7 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
 // Node gather of GWCE_LV, the time-level shift of ETA and the elevation
 // boundary conditions (src/gwce.F:1613-1631, 1638-1651, 1846-1855).  `eta1`
 // already points at the previous ETA2; `eta2` is rewritten.
+template <bool LUMPED>
 __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ contrib,
                            const double* __restrict__ eta_bc, const double* __restrict__ scal,
                            JcgPre pre) {
@@ -386,11 +415,12 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
     const double es = e2 - s.eta1[i];
     s.eta2[i] = e2;
     u = es;
-    rhs = es * (double)s.nodecode[i] * scal[S_EP];
+    // lumped: the boundary row's own COEFD (src/gwce.F:1856-1862)
+    rhs = es * (double)s.nodecode[i] * (LUMPED ? s.coef[i] : scal[S_EP]);
   } else {
     double acc = 0.0;
     for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
-    for (int p = m.obcg_ptr[i]; p < m.obcg_ptr[i + 1]; ++p) {
+    for (int p = m.obcg_ptr[i]; p < (LUMPED ? m.obcg_ptr[i] : m.obcg_ptr[i + 1]); ++p) {
       const int slot = m.obcg[p];
       const int I = slot / m.mnei;
       const int b = m.nbd[I];
@@ -409,6 +439,20 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
     dg = scal[S_EP];
     s.coef[i] = dg;
   }
+  if (LUMPED) {
+    // explicit solve (src/gwce.F:2009-2022); no JCG follows
+    if (dg < 0.0) atomicOr(&pre.flags[F_ERROR], ERR_DIAG);
+    if (i == 0) {
+      pre.flags[F_NUMITR] = 0;
+      pre.flags[F_IER] = 0;
+    }
+    s.gwce_lv[i] = rhs;
+    const double rdiag = dg == 0.0 ? 0.0 : 1.0 / dg;
+    const double es = rhs * rdiag;
+    s.etas[i] = es;
+    s.eta2[i] = (double)s.nodecode[i] * (es + 0.0) + s.eta1[i];
+    return;
+  }
   if (!(dg > 0.0)) atomicOr(&pre.flags[F_ERROR], ERR_DIAG);
   const double sq = __builtin_sqrt(dg);
   const double w = 1.0 / sq;
@@ -416,6 +460,8 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
   pre.w[i] = w;
   s.gwce_lv[i] = rhs * w;
 }
+template __global__ void k_rhs_node<false>(DevMesh, DevState, const double*, const double*, const double*, JcgPre);
+template __global__ void k_rhs_node<true>(DevMesh, DevState, const double*, const double*, const double*, JcgPre);
 
 // ---------------------------------------------------------------- wet/dry
 // D1 (src/wetdry.F:234-285); also NNODECODE := NODECODE for the W1 writes.

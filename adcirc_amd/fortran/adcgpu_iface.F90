@@ -30,6 +30,7 @@ module adcgpu_iface
      real(c_double) :: dtdp, g, h0, velmin, a00, b00, c00, convcr, prbckgrnd_mh2o
      integer(c_int32_t) :: nolifa, nolica, nolicat, nolibf, itmax, nws, precision
      type(c_ptr) :: tau0var, evm, fric, corif
+     integer(c_int32_t) :: ilump
   end type
 
   !> the context of this process (one GPU per process)

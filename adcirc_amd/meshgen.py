@@ -53,6 +53,7 @@ class RunConfig:
     c00: float = 0.35
     itmax: int = 100
     convcr: float = 1.0e-8
+    ilump: int = 0                # GWCE LHS: 0 consistent (JCG), 1 lumped (explicit)
     anginn: float = 110.0
     g: float = 9.80665
     # periodic elevation forcing on the open boundary

@@ -74,6 +74,8 @@ typedef struct {
   const double *evm;        /* EVM(NP) lateral eddy viscosity */
   const double *fric;       /* FRIC(NP) bottom friction coefficient */
   const double *corif;      /* CORIF(NP) */
+  int32_t ilump;            /* ILump: 0 consistent GWCE LHS solved by JCG, 1 lumped
+                               diagonal LHS solved explicitly (src/gwce.F:141-155) */
 } adcgpu_params;
 
 #define ADCGPU_FP64 0
@@ -92,7 +94,7 @@ enum adcgpu_field {
 
 /* Create a context on `device` (HIP ordinal) from the mesh and parameters.
  * Returns ADCGPU_E_UNSUPPORTED for configurations outside the implemented
- * subset (hybrid friction, lumped GWCE, 3D, sponge, subgrid, weirs). */
+ * subset (hybrid friction, 3D, sponge, subgrid, weirs). */
 int adcgpu_create(const adcgpu_mesh *mesh, const adcgpu_params *prm, int device,
                   adcgpu_ctx **out);
 void adcgpu_destroy(adcgpu_ctx *ctx);

@@ -91,6 +91,8 @@ typedef struct {
   /* initial state (cold start) */
   const int *nodecode0, *noff0;
   const double *eta2_0;
+  /* GWCE LHS: 0 consistent (JCG), 1 lumped diagonal (explicit solve) */
+  int ilump;
 } adco_setup;
 
 typedef struct adco_model adco_model;

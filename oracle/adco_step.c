@@ -201,10 +201,49 @@ static void friction(adco_model *m) {
 }
 
 /* GWCE LHS, src/gwce.F:419-743 (ILump = 0) */
+/* Lumped LHS (ILump = 1): the diagonal COEFD only, src/gwce.F:596-635,
+ * with EP, the boundary rows and the zero-diagonal fix of :709-743; COEFD
+ * is kept in COEF's diagonal column. */
+static int gwce_lhs_lumped(adco_model *m) {
+  const int np = m->np, ne = m->ne;
+  const double OnDiag = 4.0, DT = m->dt; /* (1+ILump)*2 */
+  memset(m->coef, 0, sizeof(double) * (size_t)np * m->mnei);
+  const double *t0 = m->s.tau0var;
+  for (int ie = 0; ie < ne; ++ie) {
+    int n1 = NM(ie, 0), n2 = NM(ie, 1), n3 = NM(ie, 2);
+    double AreaIE2 = m->s.areas[ie];
+    double AreaIE = AreaIE2 * 0.5;
+    int ncele = m->nodecode[n1] * m->nodecode[n2] * m->nodecode[n3] * m->noff[ie];
+    double NCELE = ncele;
+    double Tau0Avg = (t0[n1] + t0[n2] + t0[n3]) / 3.0;
+    double Tau0Avg_S = Tau0Avg;
+    double MsOn = OnDiag * AreaIE * (1.0 / DT + Tau0Avg_S / 2.0) / DT / 12.0;
+    double SPM1 = 1.0, SPM2 = 1.0, SPM3 = 1.0;
+    COEF(n1, 1) = COEF(n1, 1) + MsOn * SPM1 * NCELE;
+    COEF(n2, 1) = COEF(n2, 1) + MsOn * SPM2 * NCELE;
+    COEF(n3, 1) = COEF(n3, 1) + MsOn * SPM3 * NCELE;
+  }
+  if (m->order == ADCO_ORDER_BLOCKED) {
+    m->ep = adco_dot(ADCO_ORDER_BLOCKED, np, m->coef, m->coef);
+  } else {
+    m->ep = 0.0;
+    for (int i = 0; i < np; ++i) m->ep = m->ep + COEF(i, 1) * COEF(i, 1);
+  }
+  m->ep = sqrt(m->ep / np);
+  for (int I = 0; I < m->neta; ++I) COEF(m->s.nbd[I] - 1, 1) = m->ep;
+  int die = 0;
+  for (int i = 0; i < np; ++i) {
+    if (COEF(i, 1) == 0.0) COEF(i, 1) = m->ep;
+    if (COEF(i, 1) < 0.0) die = 1;
+  }
+  return die ? -1 : 0;
+}
+
 static int gwce_lhs(adco_model *m) {
   const int np = m->np, ne = m->ne;
   const double OnDiag = 2.0, OffDiag = 1.0, DT = m->dt;
   const double A00 = m->s.a00, G = m->s.g;
+  if (m->s.ilump) return gwce_lhs_lumped(m);
   memset(m->coef, 0, sizeof(double) * (size_t)np * m->mnei);
   for (int ie = 0; ie < ne; ++ie) {
     int n1 = NM(ie, 0), n2 = NM(ie, 1), n3 = NM(ie, 2);
@@ -277,7 +316,9 @@ static int gwce_lhs(adco_model *m) {
 /* GWCE RHS element loop, src/gwce.F:1000-1615 */
 static void gwce_rhs(adco_model *m) {
   const int ne = m->ne;
-  const double OnDiag = 2.0, OffDiag = 1.0, DT = m->dt, G = m->s.g;
+  /* OnDiag = (1+ILump)*2, OffDiag = 1-ILump (src/gwce.F:419-420) */
+  const double OnDiag = m->s.ilump ? 4.0 : 2.0, OffDiag = m->s.ilump ? 0.0 : 1.0;
+  const double DT = m->dt, G = m->s.g;
   const double A00 = m->s.a00, B00 = m->s.b00, C00 = m->s.c00;
   const double *dp = m->s.dp, *evm = m->s.evm, *t0 = m->s.tau0var, *corif = m->s.corif;
   const int nws = m->s.nws;
@@ -324,8 +365,12 @@ static void gwce_rhs(adco_model *m) {
     double TANPHIAvg = 0.0, SFCXADJ = 1.0, SFCYADJ = 1.0;
     double FDX1 = FDXE(ie, 0) * 1.0, FDX2 = FDXE(ie, 1) * 1.0, FDX3 = FDXE(ie, 2) * 1.0;
     double FDY1 = FDYE(ie, 0) * 1.0, FDY2 = FDYE(ie, 1) * 1.0, FDY3 = FDYE(ie, 2) * 1.0;
-    double E0XGrad2A = E0N1 * FDX1 + E0N2 * FDX2 + E0N3 * FDX3;
-    double E0YGrad2A = E0N1 * FDY1 + E0N2 * FDY2 + E0N3 * FDY3;
+    /* zero with the lumped LHS (src/gwce.F:1233-1238) */
+    double E0XGrad2A = 0.0, E0YGrad2A = 0.0;
+    if (!m->s.ilump) {
+      E0XGrad2A = E0N1 * FDX1 + E0N2 * FDX2 + E0N3 * FDX3;
+      E0YGrad2A = E0N1 * FDY1 + E0N2 * FDY2 + E0N3 * FDY3;
+    }
     double E1XGrad2A = E1N1 * FDX1 + E1N2 * FDX2 + E1N3 * FDX3;
     double E1YGrad2A = E1N1 * FDY1 + E1N2 * FDY2 + E1N3 * FDY3;
     double Tau0XGrad2A = T0N1 * FDX1 + T0N2 * FDX2 + T0N3 * FDX3;
@@ -485,6 +530,23 @@ static int gwce_new(adco_model *m, double timeh) {
   }
   eta_bc_periodic(m, timeh, m->eta2);
   const int *nbd = m->s.nbd, *nneigh = m->s.nneigh;
+  if (m->s.ilump) {
+    /* lumped: boundary rows from COEFD, explicit solve (src/gwce.F:1856-1862,
+     * 2009-2022) */
+    for (int I = 0; I < m->neta; ++I) {
+      int b = nbd[I] - 1;
+      m->etas[b] = m->eta2[b] - m->eta1[b];
+      m->gwce_lv[b] = m->etas[b] * m->nodecode[b] * COEF(b, 1);
+    }
+    for (int i = 0; i < np; ++i) {
+      double rdiag = COEF(i, 1) == 0.0 ? 0.0 : 1.0 / COEF(i, 1);
+      m->etas[i] = m->gwce_lv[i] * rdiag;
+    }
+    m->numitr = 0;
+    for (int i = 0; i < np; ++i)
+      m->eta2[i] = m->nodecode[i] * (m->etas[i] + 0.0) + m->eta1[i];
+    return 0;
+  }
   for (int I = 0; I < m->neta; ++I) {
     int b = nbd[I] - 1;
     m->etas[b] = m->eta2[b] - m->eta1[b];

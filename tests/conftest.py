@@ -30,6 +30,11 @@ def beach_golden():
 
 
 @pytest.fixture(scope="session")
+def beach_lumped_golden():
+    return load_golden("beach_lumped")
+
+
+@pytest.fixture(scope="session")
 def basin_golden():
     return load_golden("basin")
 

@@ -7,7 +7,7 @@ numpy archives.  Needs the reference build (``make -C oracle ref``), so it runs
 in the build container only; the fixtures it writes are what the tests use
 everywhere else.
 
-    python tests/golden/make_golden.py [basin] [beach] [jcg] [tau0]
+    python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [jcg] [tau0]
 """
 from __future__ import annotations
 
@@ -179,11 +179,16 @@ def make_tau0() -> str:
 
 
 def main(argv):
-    todo = argv or ["basin", "beach", "jcg", "tau0"]
+    todo = argv or ["basin", "beach", "beach_lumped", "jcg", "tau0"]
     if "basin" in todo:
         print(run_case("basin", meshgen.basin(), nsteps=43200, early=10, dump_every=43200))
     if "beach" in todo:
         print(run_case("beach", meshgen.beach(), nsteps=3600, early=10, dump_every=600))
+    if "beach_lumped" in todo:
+        # the same beach with the lumped (explicit) GWCE, ILump = 1
+        case = meshgen.beach()
+        case.run.ilump = 1
+        print(run_case("beach_lumped", case, nsteps=3600, early=10, dump_every=600))
     if "jcg" in todo:
         print(make_jcg())
     if "tau0" in todo:

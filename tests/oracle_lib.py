@@ -36,7 +36,7 @@ class Setup(C.Structure):
         ("convcr", C.c_double), ("drampelev", C.c_double), ("prbckgrnd_mh2o", C.c_double),
         ("nramp", C.c_int), ("nolifa", C.c_int), ("nolica", C.c_int), ("nolicat", C.c_int),
         ("nolibf", C.c_int), ("itmax", C.c_int), ("nws", C.c_int),
-        ("nodecode0", _ip), ("noff0", _ip), ("eta2_0", _dp),
+        ("nodecode0", _ip), ("noff0", _ip), ("eta2_0", _dp), ("ilump", C.c_int),
     ]
 
 
@@ -130,6 +130,7 @@ class OracleModel:
         s.nolibf, s.itmax, s.nws = run.nolibf, run.itmax, run.nws
         s.nodecode0 = ia("nodecode0", d["nodecode0"]); s.noff0 = ia("noff0", d["noff0"])
         s.eta2_0 = da("eta2_0", d["eta2_0"])
+        s.ilump = int(getattr(run, "ilump", 0))
         self.setup = s
         self.run = run
         self.h = self.lib.adco_create(C.byref(s), order)

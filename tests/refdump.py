@@ -45,6 +45,7 @@ def write_inputs(case, workdir: str, nsteps: int, dump_every: int = 0) -> None:
         "nws": str(r.nws), "nbfr": str(r.nbfr), "anginn_in": fl(r.anginn),
         "amig_in": ",".join(fl(v) for v in r.amig), "ff_in": ",".join(fl(v) for v in r.ff),
         "face_in": ",".join(fl(v) for v in r.face), "hprm": ",".join(fl(v) for v in hol),
+        "ilump_in": str(r.ilump),
     }
     with open(os.path.join(workdir, "run.nml"), "w") as f:
         f.write("&run\n")

@@ -113,17 +113,13 @@ def test_holland_forcing_vs_oracle():
             assert err <= 1e-13, (t, name)
 
 
-def test_coastal_resident_run_equals_per_call_api():
-    """The resident run (adcgpu_run: friction and device Holland forcing in one
-    pass, the wet/dry ALPHAL pass inside the momentum element pass, COEF's
-    off-diagonals left scaled between solves) against the same steps made one
-    ABI call at a time (adcgpu_begin_step, _friction, _holland, _solve_gwce,
-    _wetdry, _momentum) on the 20k-node coastal storm: every state array,
-    COEF and ALPHAL identical after 300 steps of wetting and drying."""
+def _resident_vs_per_call(case, n):
+    """Run ``n`` steps through the resident run and the same steps one ABI
+    call at a time; return the resident model's COEF rebuild count after
+    checking every state array, COEF and ALPHAL identical."""
     import ctypes as C
     from gpu_helpers import case_setup, golden_tables
     from adcirc_amd.model import Model
-    case = meshgen.coastal(20000)
     d = case_setup(case)
     run = case.run
     t = golden_tables(d)
@@ -138,8 +134,7 @@ def test_coastal_resident_run_equals_per_call_api():
         g.holland(0.0, run.holland)
         return g
 
-    n = 300
-    log(f"case coastal {t['np']} nodes: {n} steps resident vs per call")
+    log(f"case {t['np']} nodes, ilump {run.ilump}: {n} steps resident vs per call")
     g1, g2 = model(), model()
     g1.run_resident(1, n, holland=run.holland)
     hol = np.ascontiguousarray(list(run.holland) + [0.0] * (13 - len(run.holland)), np.float64)
@@ -153,14 +148,50 @@ def test_coastal_resident_run_equals_per_call_api():
         assert lib.adcgpu_solve_gwce(h, it, bc.ctypes.data_as(dp_)) == 0
         assert lib.adcgpu_wetdry(h, it) == 0
         assert lib.adcgpu_momentum(h) == 0
-    # steps that rebuild COEF and steps that restore it from its scaled form
-    # (the basin's resident test covers runs that never rebuild)
     log(f"  COEF rebuilt on {g1.last_rebuilds} of {n} steps")
-    assert g1.last_rebuilds > 0
     for k in ("eta1", "eta2", "etas", "uu2", "vv2", "qx2", "qy2", "tk", "etamax", "coef",
               "alphal", "nodecode", "noff"):
         a, b = g1.get(k), g2.get(k)
         assert np.array_equal(a, b), (k, int((a != b).sum()))
+    return g1.last_rebuilds
+
+
+def test_coastal_resident_run_equals_per_call_api():
+    """The resident run (adcgpu_run: friction and device Holland forcing in one
+    pass, the wet/dry ALPHAL pass inside the momentum element pass, COEF's
+    off-diagonals left scaled between solves) against the same steps made one
+    ABI call at a time (adcgpu_begin_step, _friction, _holland, _solve_gwce,
+    _wetdry, _momentum) on the 20k-node coastal storm: every state array,
+    COEF and ALPHAL identical after 300 steps of wetting and drying."""
+    # steps that rebuild COEF and steps that restore it from its scaled form
+    # (the basin's resident test covers runs that never rebuild)
+    assert _resident_vs_per_call(meshgen.coastal(20000), 300) > 0
+
+
+def test_beach_lumped_resident_run_equals_per_call_api():
+    """The lumped GWCE (ILump = 1, no JCG) through the resident run against
+    the per-call ABI on the wet/dry beach storm: identical after 600 steps."""
+    case = meshgen.beach()
+    case.run.ilump = 1
+    _resident_vs_per_call(case, 600)
+
+
+def test_beach_lumped_hour_bitexact_and_reference(beach_lumped_golden):
+    """The lumped GWCE (ILump = 1: diagonal LHS, explicit solve) on the wet/dry
+    beach storm, 3600 steps: bit-exact vs oracle, and within the reference's
+    bar of the reference build's own run."""
+    d = beach_lumped_golden
+    run = meshgen.beach().run
+    run.ilump = 1
+    log("case beach lumped: 3600 steps (1 h, wet/dry + storm), bit-exact vs oracle")
+    g, o = _run_bitexact(d, run, 3600, check_every=50)
+    s = 3600
+    assert g.numitr()[0] == 0
+    assert np.array_equal(g.get("nodecode"), d[f"s{s}_nodecode"])
+    assert np.array_equal(g.get("noff"), d[f"s{s}_noff"])
+    assert np.abs(g.get("eta2") - d[f"s{s}_eta2"]).max() <= 1e-6
+    assert np.abs(g.get("uu2") - d[f"s{s}_uu2"]).max() <= 1e-6
+    assert np.abs(g.get("vv2") - d[f"s{s}_vv2"]).max() <= 1e-6
 
 
 def test_basin_full_day_resident_vs_reference(basin_golden):

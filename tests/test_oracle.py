@@ -46,6 +46,15 @@ def test_beach_wetdry_storm_hour_matches_reference(beach_golden):
     _run(beach_golden, meshgen.beach().run, 3600)
 
 
+def test_beach_lumped_gwce_hour_matches_reference(beach_lumped_golden):
+    """The same beach with the lumped GWCE (ILump = 1: diagonal LHS, explicit
+    solve, src/gwce.F:596-635, 1856-1862, 2009-2022), 3600 steps:
+    bit-identical every step."""
+    run = meshgen.beach().run
+    run.ilump = 1
+    _run(beach_lumped_golden, run, 3600)
+
+
 def test_basin_matches_reference_first_3000_steps(basin_golden):
     _run(basin_golden, meshgen.basin().run, 3000)
 
