@@ -70,7 +70,8 @@ struct adcgpu_ctx {
   double *eta1, *eta2, *etas, *uu1, *vv1, *uu2, *vv2, *qx1, *qy1, *qx2, *qy2, *h1, *h2, *tk;
   double *etamax, *totalarea, *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2, *gwce_lv, *alphal;
   double *coef, *obc, *fric;
-  int *nodecode, *nnodecode, *noff, *noffold, *mju;
+  int *nodecode, *nnodecode, *noff, *noffold, *mju, *etamax_it;
+  int cur_it = 0;  // step opened by adcgpu_begin_step
   // work
   double *contrib = nullptr, *cx = nullptr, *cy = nullptr, *ep_part = nullptr, *scal = nullptr;
   double *eta_bc = nullptr, *eta_bc_table = nullptr;
@@ -112,6 +113,7 @@ struct adcgpu_ctx {
     s.wsx2 = wsx2; s.wsy2 = wsy2; s.pr1 = pr1; s.pr2 = pr2; s.gwce_lv = gwce_lv;
     s.alphal = alphal; s.coef = coef; s.obc = obc; s.fric = fric; s.nodecode = nodecode;
     s.nnodecode = nnodecode; s.noff = noff; s.noffold = noffold; s.mju = mju;
+    s.etamax_it = etamax_it;
     return s;
   }
 };
@@ -432,7 +434,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   A(&c->alphal, ne); A(&c->coef, (size_t)mnei * np); A(&c->cs, (size_t)mnei * np);
   A(&c->obc, (size_t)std::max(neta, 1) * mnei);
   A(&c->nodecode, np); A(&c->nnodecode, np); A(&c->noff, ne); A(&c->noffold, ne);
-  A(&c->mju, np);
+  A(&c->mju, np); A(&c->etamax_it, np);
   A(&c->contrib, 3 * (size_t)ne); A(&c->cx, 3 * (size_t)ne); A(&c->cy, 3 * (size_t)ne);
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
   {
@@ -544,6 +546,7 @@ static int** ifield_ptr(adcgpu_ctx* c, int f) {
     case ADCGPU_NODECODE: return &c->nodecode;
     case ADCGPU_NOFF: return &c->noff;
     case ADCGPU_MJU: return &c->mju;
+    case ADCGPU_ETAMAX_IT: return &c->etamax_it;
   }
   return nullptr;
 }
@@ -612,6 +615,7 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
 
 int adcgpu_begin_step(adcgpu_ctx* c, int it) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
+  c->cur_it = it;
   if (it == 1)
     hipLaunchKernelGGL(k_init_h, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->dp,
                        c->eta2, c->ifnlfa, c->h1, c->h2);
@@ -761,7 +765,7 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
 
 // Momentum; with `alpha` the element pass forms ALPHAL first.
 static int momentum_device(adcgpu_ctx* c, bool alpha) {
-  MomPrm q{c->ifnlct, c->prm.nws};
+  MomPrm q{c->ifnlct, c->prm.nws, c->cur_it};
   if (alpha)
     hipLaunchKernelGGL(k_mom_elem<true>, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream,
                        c->mesh(), c->state(), q, c->cx, c->cy, c->flags);

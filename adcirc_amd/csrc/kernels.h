@@ -83,6 +83,7 @@ struct DevState {
   double *h1, *h2, *tk, *etamax, *totalarea, *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2;
   double *gwce_lv, *alphal, *coef, *obc, *fric;
   int *nodecode, *nnodecode, *noff, *noffold, *mju;
+  int *etamax_it;  // step of the last ETAMAX rise (0: never)
 };
 
 struct RhsPrm {
@@ -95,6 +96,7 @@ struct RhsPrm {
 struct MomPrm {
   double ifnlct;
   int nws;
+  int it;  // this step, recorded where ETAMAX rises
 };
 
 struct WdPrm {

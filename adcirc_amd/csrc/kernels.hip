@@ -1159,7 +1159,12 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
   s.qx2[i] = u2 * h2;
   s.qy2[i] = v2 * h2;
   const double e2 = s.eta2[i];
-  if (nci == 1 && e2 > s.etamax[i]) s.etamax[i] = e2;
+  // peak surge and its step (src/write_output.F:6357-6364; the host turns
+  // the step into ETAMAX_Time = IT*DTDP + STATIM*86400)
+  if (nci == 1 && e2 > s.etamax[i]) {
+    s.etamax[i] = e2;
+    s.etamax_it[i] = q.it;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, unsigned v, int n) {

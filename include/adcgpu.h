@@ -88,6 +88,9 @@ enum adcgpu_field {
   ADCGPU_TK, ADCGPU_ETAMAX, ADCGPU_TOTALAREA, ADCGPU_WSX1, ADCGPU_WSY1, ADCGPU_WSX2,
   ADCGPU_WSY2, ADCGPU_PR1, ADCGPU_PR2, ADCGPU_GWCE_LV,
   ADCGPU_NODECODE = 100, ADCGPU_NOFF, ADCGPU_MJU,
+  ADCGPU_ETAMAX_IT, /* step IT of each node's last ETAMAX rise, 0 if none: the
+                       reference's ETAMAX_Time is IT*DTDP + STATIM*86400
+                       (src/write_output.F:6357-6364) */
   ADCGPU_COEF = 200, /* COEF(NP,MNEI) column-major */
   ADCGPU_ALPHAL = 300
 };

@@ -106,7 +106,7 @@ int adco_step(adco_model *m, int it, const double *wsx2, const double *wsy2,
 /* Access to state arrays by name ("eta1","eta2","etas","uu2","vv2","qx2",
  * "qy2","tk","etamax","totalarea","coef","gwce_lv","h2","alphal"). */
 double *adco_array(adco_model *m, const char *name);
-int *adco_iarray(adco_model *m, const char *name); /* nodecode, noff, mju */
+int *adco_iarray(adco_model *m, const char *name); /* nodecode, noff, mju, etamax_it */
 int adco_numitr(const adco_model *m);
 int adco_ncchange(const adco_model *m);
 uint64_t adco_state_hash(const adco_model *m);

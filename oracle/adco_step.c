@@ -28,7 +28,7 @@ struct adco_model {
   double *h1, *h2, *tk, *tkm, *gwce_lv, *mom_lv_x, *mom_lv_y;
   double *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2, *etamax, *totalarea, *auv;
   double *tk_tmp;
-  int *nodecode, *nnodecode, *mju, *nibcnt;
+  int *nodecode, *nnodecode, *mju, *nibcnt, *etamax_it;
   /* element state */
   int *noff, *noffold;
   double *alphal;
@@ -83,6 +83,7 @@ adco_model *adco_create(const adco_setup *s, int order) {
   m->nodecode = ialloc(np);
   m->nnodecode = ialloc(np);
   m->mju = ialloc(np);
+  m->etamax_it = ialloc(np);
   m->nibcnt = ialloc(np);
   m->noff = ialloc(ne);
   m->noffold = ialloc(ne);
@@ -154,7 +155,7 @@ void adco_destroy(adco_model *m) {
                m->vv2, m->qx1, m->qy1, m->qx2, m->qy2, m->h1, m->h2, m->tk, m->gwce_lv,
                m->mom_lv_x, m->mom_lv_y, m->wsx1, m->wsy1, m->wsx2, m->wsy2, m->pr1,
                m->pr2, m->etamax, m->totalarea, m->tk_tmp, m->tkm, m->auv, m->nodecode,
-               m->nnodecode, m->mju, m->nibcnt, m->noff, m->noffold, m->alphal, m->coef,
+               m->nnodecode, m->mju, m->etamax_it, m->nibcnt, m->noff, m->noffold, m->alphal, m->coef,
                m->obccoef, m->wksp};
   for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
   free(m);
@@ -924,8 +925,13 @@ int adco_step(adco_model *m, int it, const double *wsx2, const double *wsy2,
   }
   for (int i = 0; i < np; ++i) m->h2[i] = dp[i] + m->ifnlfa * m->eta2[i];
   momentum(m);
+  /* collectMinMaxData (src/write_output.F:6357-6364): the step stands for
+   * ETAMAX_Time = IT*DTDP + STATIM*86400 */
   for (int i = 0; i < np; ++i)
-    if (m->nodecode[i] == 1 && m->eta2[i] > m->etamax[i]) m->etamax[i] = m->eta2[i];
+    if (m->nodecode[i] == 1 && m->eta2[i] > m->etamax[i]) {
+      m->etamax[i] = m->eta2[i];
+      m->etamax_it[i] = it;
+    }
   return ier;
 }
 
@@ -948,6 +954,7 @@ int *adco_iarray(adco_model *m, const char *name) {
   if (!strcmp(name, "nodecode")) return m->nodecode;
   if (!strcmp(name, "noff")) return m->noff;
   if (!strcmp(name, "mju")) return m->mju;
+  if (!strcmp(name, "etamax_it")) return m->etamax_it;
   if (!strcmp(name, "neitab")) return m->neitab;
   return NULL;
 }

@@ -17,6 +17,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs libadcgpu.so on the device)")
 
 
+def pytest_collection_modifyitems(session, config, items):
+    # The Fortran interface test runs its check program as a child process.
+    # A process's cooperative launch of the JCG kernel is refused while
+    # another process on the same GPU has made one, so this test goes first,
+    # before the test process itself launches any.
+    items.sort(key=lambda it: 0 if it.nodeid.startswith("tests/test_gpu_fortran.py") else 1)
+
+
 def load_golden(name):
     path = os.path.join(GOLDEN, f"{name}.npz")
     if not os.path.exists(path):
