@@ -46,7 +46,7 @@ class Params(C.Structure):
                 ("nolifa", C.c_int32), ("nolica", C.c_int32), ("nolicat", C.c_int32),
                 ("nolibf", C.c_int32), ("itmax", C.c_int32), ("nws", C.c_int32),
                 ("precision", C.c_int32), ("tau0var", dp_), ("evm", dp_), ("fric", dp_),
-                ("corif", dp_), ("ilump", C.c_int32)]
+                ("corif", dp_), ("ilump", C.c_int32), ("warm_start", C.c_int32)]
 
 
 _lib = None

@@ -69,7 +69,7 @@ struct adcgpu_ctx {
   // state (pointers rotate between time levels)
   double *eta1, *eta2, *etas, *uu1, *vv1, *uu2, *vv2, *qx1, *qy1, *qx2, *qy2, *h1, *h2, *tk;
   double *etamax, *totalarea, *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2, *gwce_lv, *alphal;
-  double *coef, *obc, *fric;
+  double *coef, *obc, *fric, *etas_prev = nullptr;
   int *nodecode, *nnodecode, *noff, *noffold, *mju, *etamax_it;
   int cur_it = 0;  // step opened by adcgpu_begin_step
   // work
@@ -260,6 +260,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   if (P->nolibf < 0 || P->nolibf > 3) return fail(ADCGPU_E_ARG, "NOLIBF");
   if (P->itmax <= 0 || P->itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
   if (P->precision != ADCGPU_FP64) return fail(ADCGPU_E_UNSUPPORTED, "only the FP64 path");
+  if (P->warm_start != 0 && P->warm_start != 1) return fail(ADCGPU_E_ARG, "warm_start must be 0 or 1");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
     return fail(ADCGPU_E_HIP, "no HIP device " + std::to_string(device));
@@ -434,7 +435,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   A(&c->alphal, ne); A(&c->coef, (size_t)mnei * np); A(&c->cs, (size_t)mnei * np);
   A(&c->obc, (size_t)std::max(neta, 1) * mnei);
   A(&c->nodecode, np); A(&c->nnodecode, np); A(&c->noff, ne); A(&c->noffold, ne);
-  A(&c->mju, np); A(&c->etamax_it, np);
+  A(&c->mju, np); A(&c->etamax_it, np); A(&c->etas_prev, np);
   A(&c->contrib, 3 * (size_t)ne); A(&c->cx, 3 * (size_t)ne); A(&c->cy, 3 * (size_t)ne);
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
   {
@@ -684,7 +685,7 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
   std::swap(c->eta1, c->eta2);
   std::swap(c->h1, c->h2);
-  JcgPre pre{c->jw, c->jbar, jcg_barrier_words(c), c->flags};
+  JcgPre pre{c->jw, c->jbar, jcg_barrier_words(c), c->flags, c->etas_prev, c->prm.warm_start};
   if (lumped) {
     // the explicit solve of the lumped GWCE ends the node pass; no JCG
     hipLaunchKernelGGL(k_rhs_node<true>, dim3(nblk(np, 256)), dim3(256), 0, c->stream,

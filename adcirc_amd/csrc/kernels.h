@@ -187,6 +187,8 @@ struct JcgPre {
   unsigned* barrier;  // grid barrier words of the coming k_jcg launch, cleared here
   int barrier_words;
   int* flags;
+  double* etas_prev;  // the previous step's ETAS (warm start only)
+  int warm;           // 1: start JCG from 2*ETAS - ETAS_prev instead of ETAS
 };
 template <bool LUMPED>
 __global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const double* eta_bc,

@@ -429,6 +429,13 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
     }
     rhs = acc;
     u = s.etas[i];
+    if (pre.warm) {
+      // the increment extrapolated from the last two: ETA2 extrapolated to
+      // second order instead of the first order the previous increment gives
+      const double up = pre.etas_prev[i];
+      pre.etas_prev[i] = u;
+      u = 2.0 * u - up;
+    }
     s.eta2[i] = 0.0;
   }
   // a zero diagonal (a dry row) becomes EP (src/gwce.F:737-741); then the

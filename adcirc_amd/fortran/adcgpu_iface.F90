@@ -31,6 +31,7 @@ module adcgpu_iface
      integer(c_int32_t) :: nolifa, nolica, nolicat, nolibf, itmax, nws, precision
      type(c_ptr) :: tau0var, evm, fric, corif
      integer(c_int32_t) :: ilump
+     integer(c_int32_t) :: warm_start
   end type
 
   !> the context of this process (one GPU per process)

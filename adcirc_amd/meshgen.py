@@ -54,6 +54,7 @@ class RunConfig:
     itmax: int = 100
     convcr: float = 1.0e-8
     ilump: int = 0                # GWCE LHS: 0 consistent (JCG), 1 lumped (explicit)
+    warm_start: int = 0           # JCG start: 0 previous increment, 1 extrapolated increment
     anginn: float = 110.0
     g: float = 9.80665
     # periodic elevation forcing on the open boundary

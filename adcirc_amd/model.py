@@ -41,7 +41,8 @@ def c_layout(tables: Dict[str, np.ndarray], run, tau0var=None, evm=None, fric=No
           "prbckgrnd_mh2o": 100.0 * 1013.0 / (1000.0 * run.g),
           "nolifa": run.nolifa, "nolica": run.nolica, "nolicat": run.nolicat,
           "nolibf": run.nolibf, "itmax": run.itmax, "nws": run.nws,
-          "ilump": int(getattr(run, "ilump", 0))}
+          "ilump": int(getattr(run, "ilump", 0)),
+          "warm_start": int(getattr(run, "warm_start", 0))}
     i32 = lambda a: np.ascontiguousarray(a, np.int32)  # noqa: E731
     f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
     ar = {
@@ -67,7 +68,7 @@ _MESH_I = ("np", "ne", "mnei", "neta", "nvel", "nvelme")
 _MESH_A = ("nm", "x", "y", "dp", "fdxe", "fdye", "areas", "nneigh", "neitab", "csr_position",
            "nbd", "nbv", "lbcodei", "csii", "siii", "me2gw")
 _PRM_D = ("dtdp", "g", "h0", "velmin", "a00", "b00", "c00", "convcr", "prbckgrnd_mh2o")
-_PRM_I = ("nolifa", "nolica", "nolicat", "nolibf", "itmax", "nws", "ilump")
+_PRM_I = ("nolifa", "nolica", "nolicat", "nolibf", "itmax", "nws", "ilump", "warm_start")
 _PRM_A = ("tau0var", "evm", "fric", "corif")
 
 

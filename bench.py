@@ -11,7 +11,11 @@ configuration's 6 simulated hours (21,600 steps: 30 warm-up, 21,570 timed).
 
     python bench.py [--gpus 1] [--steps 21570] [--warmup 30] [--nodes 1000000]
 
-Prints one JSON line.  The dominant kernel's roofline entry is computed from
+Prints one JSON line.  ``value`` is the plain path (the reference's JCG start,
+results identical to the oracle's); ``warm_started`` reports the same run
+again with the JCG started from the extrapolated increment 2*ETAS - ETAS_prev
+(same stopping test) and its differences from the plain run (--no-warm skips
+it).  The dominant kernel's roofline entry is computed from
 the JCG launches timed with HIP events on the library's stream; the CPU
 baseline is the oracle (serial C restatement of the reference, bit-exact to
 it) timed on a bounded sample of the same run on this host.
@@ -19,6 +23,7 @@ it) timed on a bounded sample of the same run on this host.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -88,6 +93,8 @@ def main() -> int:
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-warm", action="store_true",
+                    help="skip the second run with the extrapolated JCG start")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without HIP events around the JCG launches "
                          "(no roofline entry)")
@@ -112,37 +119,45 @@ def main() -> int:
         write_case(a.dump, tables, run, eta2, nodecode, noff, table, run.holland)
         log(f"case written to {a.dump}")
         return 0
-    g = Model(tables, run, emo=case.emo, efa=case.efa * (3.141592653589793 / 180.0))
-    g.cold_start(eta2, nodecode, noff)
+    def model(r):
+        m = Model(tables, r, emo=case.emo, efa=case.efa * (3.141592653589793 / 180.0))
+        m.cold_start(eta2, nodecode, noff)
+        m.holland(0.0, r.holland)
+        return m
+
+    def timed(m, r, table, kernel_timing):
+        log(f"warmup {a.warmup} steps")
+        t0 = time.perf_counter()
+        done = 0
+        while done < a.warmup:
+            k = min(10 if done == 0 else 50, a.warmup - done)
+            m.run_resident(done + 1, k, holland=r.holland, table=table[done:done + k])
+            done += k
+            log(f"  warmup {done}/{a.warmup} steps, {time.perf_counter() - t0:.2f} s")
+        m.synchronize()
+        m.profile(kernel_timing, launches=a.steps + 16)
+        log(f"timed {a.steps} steps")
+        # the timed steps are issued in chunks of up to 1000 (each chunk ends
+        # with a stream synchronisation inside the library) so that progress
+        # is visible
+        t0 = time.perf_counter()
+        iters, done, rebuilds = 0, 0, 0
+        while done < a.steps:
+            k = min(1000, a.steps - done)
+            s0 = a.warmup + done
+            iters += m.run_resident(s0 + 1, k, holland=r.holland, table=table[s0:s0 + k])
+            rebuilds += m.last_rebuilds
+            done += k
+            log(f"  {done}/{a.steps} steps, {time.perf_counter() - t0:.2f} s")
+        m.synchronize()
+        return time.perf_counter() - t0, iters, rebuilds
+
+    g = model(run)
     log(f"JCG launch: {g.jcg_shape()} (blocks, resident-block limit, 2048-row chunks)")
-    g.holland(0.0, run.holland)
     nsteps = a.warmup + a.steps
     log(f"elevation boundary table for {nsteps} steps")
     table = g.eta_bc_table(1, nsteps)
-    log(f"warmup {a.warmup} steps")
-    t0 = time.perf_counter()
-    done = 0
-    while done < a.warmup:
-        k = min(10 if done == 0 else 50, a.warmup - done)
-        g.run_resident(done + 1, k, holland=run.holland, table=table[done:done + k])
-        done += k
-        log(f"  warmup {done}/{a.warmup} steps, {time.perf_counter() - t0:.2f} s")
-    g.synchronize()
-    g.profile(not a.no_kernel_timing, launches=a.steps + 16)
-    log(f"timed {a.steps} steps")
-    # the timed steps are issued in chunks of up to 200 (each chunk ends with a
-    # stream synchronisation inside the library) so that progress is visible
-    t0 = time.perf_counter()
-    iters, done, rebuilds = 0, 0, 0
-    while done < a.steps:
-        k = min(1000, a.steps - done)
-        s0 = a.warmup + done
-        iters += g.run_resident(s0 + 1, k, holland=run.holland, table=table[s0:s0 + k])
-        rebuilds += g.last_rebuilds
-        done += k
-        log(f"  {done}/{a.steps} steps, {time.perf_counter() - t0:.2f} s")
-    g.synchronize()
-    wall = time.perf_counter() - t0
+    wall, iters, rebuilds = timed(g, run, table, not a.no_kernel_timing)
     jcg_ms, launches = g.profile_read()
     shape = g.jcg_shape()
     g.profile(False)
@@ -194,6 +209,32 @@ def main() -> int:
             "kernel_share_of_step": (jcg_ms / 1e3) / wall,
         },
     }
+    if not a.no_warm:
+        # the same run started from the extrapolated increment: same JCG
+        # stopping test, results within the JCG tolerance of the plain run's
+        log("warm-started JCG: initial guess 2*ETAS - ETAS_prev")
+        wrun = dataclasses.replace(run, warm_start=1)
+        w = model(wrun)
+        wwall, witers, _ = timed(w, wrun, table, False)
+        de = w.get("eta2") - g.get("eta2")
+        em_g, em_w = etamax, w.get("etamax")
+        both = (em_g > -99999.0) & (em_w > -99999.0)
+        out["warm_started"] = {
+            "method": "JCG initial guess 2*ETAS - ETAS_prev (second-order extrapolated "
+                      "increment) instead of ETAS; not the learned start",
+            "value": a.steps * run.dtdp / wwall,
+            "ms_per_step": wwall * 1e3 / a.steps,
+            "jcg_iterations_per_step": witers / a.steps,
+            "speedup_vs_plain": wall / wwall,
+            "vs_plain_max_abs_deta_m": float(np.abs(de).max()),
+            "vs_plain_rms_deta_m": float(np.sqrt(np.mean(de * de))),
+            "vs_plain_peak_surge_max_abs_diff_m": float(np.abs(em_w - em_g)[both].max()),
+            "vs_plain_peak_surge_defined_mismatch": int((both != ((em_g > -99999.0) |
+                                                                  (em_w > -99999.0))).sum()),
+            "vs_plain_nodecode_mismatch_end": int((w.get("nodecode") != g.get("nodecode")).sum()),
+            "vs_plain_noff_mismatch_end": int((w.get("noff") != g.get("noff")).sum()),
+        }
+        w.close()
     if not a.no_cpu:
         log("cpu baseline (oracle, 1 core)")
         out["cpu_baseline"] = cpu_baseline(case, tables, eta2, nodecode, noff, a.cpu_budget)

@@ -76,6 +76,11 @@ typedef struct {
   const double *corif;      /* CORIF(NP) */
   int32_t ilump;            /* ILump: 0 consistent GWCE LHS solved by JCG, 1 lumped
                                diagonal LHS solved explicitly (src/gwce.F:141-155) */
+  int32_t warm_start;       /* JCG initial guess: 0 the previous increment ETAS, as the
+                               reference (src/gwce.F:2002-2004); 1 the increment
+                               extrapolated from the last two, 2*ETAS - ETAS_prev (same
+                               stopping test, results within the JCG tolerance, not
+                               bit-identical to the reference) */
 } adcgpu_params;
 
 #define ADCGPU_FP64 0

@@ -261,3 +261,33 @@ def test_jcg_dropin_sequence(jcg_golden):
         assert abs(int(iparm[0]) - int(g[f"{name}_iparm1"])) <= 1, name
         ur = g[f"{name}_u"]
         assert np.linalg.norm(u - ur) <= 100 * zeta * max(np.linalg.norm(ur), 1e-300), name
+
+
+def test_warm_start_within_jcg_tolerance_of_plain():
+    """The extrapolated JCG start (warm_start = 1) on the 20k-node coastal
+    storm: the same stopping test, so the state stays within the solver's
+    tolerance of the plain start's (no bit-exactness: a different initial
+    guess ends at a different point inside the stopping ball)."""
+    import dataclasses
+    from gpu_helpers import case_setup, golden_tables
+    from adcirc_amd.model import Model
+    case = meshgen.coastal(20000)
+    d = case_setup(case)
+    t = golden_tables(d)
+    nb = case.run.nbfr
+    out = {}
+    for ws in (0, 1):
+        run = dataclasses.replace(case.run, warm_start=ws)
+        g = Model(t, run, tau0var=d["tau0var"], evm=d["evm"], fric=d["fric"],
+                  corif=np.full(t["np"], run.cori),
+                  emo=np.asarray(d["emo"]).reshape(t["neta"], nb),
+                  efa=np.asarray(d["efa"]).reshape(t["neta"], nb))
+        g.cold_start(d["eta2_0"], d["nodecode0"], d["noff0"])
+        g.holland(0.0, run.holland)
+        it = g.run_resident(1, 600, holland=run.holland)
+        out[ws] = (it, g.get("eta2"), g.get("nodecode"))
+        g.close()
+    de = np.abs(out[1][1] - out[0][1]).max()
+    log(f"600 steps: JCG iterations plain {out[0][0]}, extrapolated {out[1][0]}; "
+        f"max |d eta| {de:.3e} m, nodecode differences {int((out[0][2] != out[1][2]).sum())}")
+    assert de <= 1e-5
