@@ -85,6 +85,7 @@ struct adcgpu_ctx {
   int jcg_grid = 0;
   long long barrier_ticks = 0;  // 0.5 s of the device wall clock
   int jcg_cap = 0;              // resident k_jcg blocks on this device
+  int jcg_halo = -1;            // row tiles each side a tile's rows reference (-1: any)
   int64_t run_iters = 0, run_rebuilds = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // optional timing of the JCG launches with HIP events on the context stream
@@ -133,6 +134,12 @@ int check_launch(const char* what) {
 // per CU although the query reports two; 4- and 8-wave blocks run as many as
 // it reports).  A cooperative grid larger than what is really resident never
 // passes its first grid barrier, so the smaller of the two counts is used.
+// grid barrier counters, then one halo flag per block
+static int barrier_words_for(int grid) {
+  return ((grid + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE +
+         grid * JCG_HALO_STRIDE;
+}
+
 int jcg_blocks_per_cu(int device, int* per_cu_out, int* cus_out) {
   int per_cu = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jcg, JCG_BLOCK, 0));
@@ -163,7 +170,8 @@ int set_jcg_grid(adcgpu_ctx* c) {
 int setup_jcg_grid(adcgpu_ctx* c) {
   int per_cu = 0, cus = 0;
   if (int rc = jcg_blocks_per_cu(c->device, &per_cu, &cus)) return rc;
-  const int bar_cap = (JCG_BAR_WORDS / JCG_BAR_STRIDE - 2) * JCG_BAR_GROUP;
+  int bar_cap = 0;
+  while (barrier_words_for(bar_cap + 1) <= JCG_BAR_WORDS) ++bar_cap;
   const int cap = std::min(per_cu * cus, bar_cap);
   c->jcg_cap = cap;
   int khz = 0;
@@ -186,9 +194,7 @@ int grow_events(adcgpu_ctx* c, size_t n) {
   return 0;
 }
 
-int jcg_barrier_words(const adcgpu_ctx* c) {
-  return ((c->jcg_grid + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE;
-}
+int jcg_barrier_words(const adcgpu_ctx* c) { return barrier_words_for(c->jcg_grid); }
 
 int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
                double* u, int n, int itmax, double zeta, double* eta2, const double* eta1,
@@ -219,6 +225,7 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   a.prescaled = prescaled;
   a.defer_unscal = defer_unscal;
   a.barrier = c->jbar;
+  a.halo = c->jcg_halo;
   a.barrier_ticks = c->barrier_ticks;
   a.flags = c->flags;
   a.eta2 = eta2;
@@ -291,6 +298,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     }
   std::vector<int> nneigh(M->nneigh, M->nneigh + np);
   std::vector<int> col((size_t)mnei * np, -1);
+  int bw = 0;  // matrix bandwidth: the widest |column - row|
   for (int i = 0; i < np; ++i) {
     if (nneigh[i] < 1 || nneigh[i] > mnei) return fail(ADCGPU_E_ARG, "NNEIGH out of range");
     if (M->neitab[i] != i + 1) return fail(ADCGPU_E_ARG, "NEITAB(i,1) must be i");
@@ -298,8 +306,11 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
       int v = M->neitab[i + (size_t)np * j] - 1;
       if (v < 0 || v >= np) return fail(ADCGPU_E_ARG, "NEITAB out of range");
       col[(size_t)j * np + i] = v;
+      bw = std::max(bw, std::abs(v - i));
     }
   }
+  // a JCG block's rows reference rows at most this many blocks away
+  const int halo_blocks = (bw + JCG_CPB * RED_CHUNK - 1) / (JCG_CPB * RED_CHUNK);
   std::vector<int> csrpos(6 * (size_t)ne);
   for (int e = 0; e < ne; ++e)
     for (int k = 0; k < 3; ++k)
@@ -395,6 +406,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   c->device = device;
   c->np = np; c->ne = ne; c->mnei = mnei; c->neta = neta;
   c->nch = (np + RED_CHUNK - 1) / RED_CHUNK;
+  c->jcg_halo = halo_blocks <= JCG_HALO_MAX ? halo_blocks : -1;
   c->prm = *P;
   c->ifnlfa = P->nolifa == 0 ? 0.0 : 1.0;
   c->ifnlct = P->nolica == 0 ? 0.0 : 1.0;
@@ -448,7 +460,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     A(&c->wd.xcnt, nwb);
     A(&c->wd.xval, nwb * WD_XMAX);
   }
-  A(&c->flags, F_COUNT); A(&c->jpart, (size_t)c->nch * 4); A(&c->jsaved, 8);
+  A(&c->flags, F_COUNT); A(&c->jpart, 2 * (size_t)c->nch * 4); A(&c->jsaved, 12);
   A(&c->jbar, JCG_BAR_WORDS);
   if (!rc) {
     hipError_t e = hipStreamCreate(&c->stream);
@@ -1007,7 +1019,7 @@ int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double*
     };
     A(&c->coef, (size_t)width * n); A(&c->cs, (size_t)width * n); A(&c->col, (size_t)width * n);
     A(&c->nneigh, n); A(&c->gwce_lv, n); A(&c->etas, n); A(&c->jd, n); A(&c->jw, n);
-    A(&c->jpart, (size_t)c->nch * 4); A(&c->jsaved, 8); A(&c->jbar, JCG_BAR_WORDS);
+    A(&c->jpart, 2 * (size_t)c->nch * 4); A(&c->jsaved, 12); A(&c->jbar, JCG_BAR_WORDS);
     A(&c->flags, F_COUNT);
     if (!rc) HIPCHK(hipStreamCreate(&c->stream));
     if (!rc) rc = setup_jcg_grid(c);

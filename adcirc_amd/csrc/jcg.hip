@@ -87,6 +87,47 @@ __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& gen, int* err
   return failed == 0;
 }
 
+// Point-to-point wait after the update of the pseudo-residual: this tile
+// publishes its iteration count, then waits until the tiles lo..hi (its rows'
+// neighbours, the mesh bandwidth wide) have published theirs.  Only the
+// neighbours' d is read before the next grid barrier, so this replaces a
+// second grid-wide barrier per iteration.  Same release / acquire pattern
+// and the same bounded wait as grid_sync.
+__device__ __forceinline__ bool halo_sync(unsigned* hflags, int tile, int lo, int hi,
+                                          unsigned gen, int* err, long long ticks) {
+  __shared__ int failed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(hflags + JCG_HALO_STRIDE * tile, gen, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    failed = 0;
+    const long long t0 = wall_clock64();
+    unsigned polls = 0;
+    for (int q = lo; q <= hi && !failed; ++q) {
+      if (q == tile) continue;
+      while (__hip_atomic_load(hflags + JCG_HALO_STRIDE * q, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) < gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if ((++polls & 63u) == 0) {
+          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER) {
+            failed = 1;
+            break;
+          }
+          if (wall_clock64() - t0 > ticks) {
+            atomicOr(err, ERR_BARRIER);
+            failed = 1;
+            break;
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return failed == 0;
+}
+
 // Element access through a 32-bit byte offset from a uniform base: the
 // addresses then take the scalar-base + 32-bit-offset form of the vector
 // memory instructions instead of one 64-bit VGPR pair per row.  Arrays here
@@ -265,7 +306,13 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   const bool eig_block = (int)blockIdx.x == nrb;  // the last block: no rows
   // each group of RED_THREADS threads owns one chunk; rows past NP (including
   // a whole missing chunk in the last block) are skipped like padding
-  const int c = (eig_block ? nrb : xcd_tile(blockIdx.x, nrb)) * JCG_CPB + t / RED_THREADS;
+  const int tile = eig_block ? nrb : xcd_tile(blockIdx.x, nrb);
+  const int c = tile * JCG_CPB + t / RED_THREADS;
+  // halo flags follow the grid barrier's counters
+  unsigned* hflags =
+      a.barrier + ((gridDim.x + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE;
+  const int hlo = a.halo >= 0 ? (tile - a.halo > 0 ? tile - a.halo : 0) : 0;
+  const int hhi = a.halo >= 0 ? (tile + a.halo < nrb - 1 ? tile + a.halo : nrb - 1) : 0;
   const int base = c * RED_CHUNK + t % RED_THREADS;
   unsigned gen = 0;
   int* err = a.flags + F_ERROR;
@@ -408,8 +455,9 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       if (t == 0) {
         L.in = in;
         if (L.adapt) jcg_chgcon(L, tri1, tri2);
-        a.saved[6] = L.cme;
-        a.saved[7] = L.cmold;
+        // by parity: a block may still read the last iteration's pair
+        a.saved[6 + 2 * (in & 1)] = L.cme;
+        a.saved[7 + 2 * (in & 1)] = L.cmold;
       }
     } else {
       // N d, rows in groups of four (bounded register pressure); per row
@@ -474,7 +522,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       __syncthreads();
       if (tt < 3 && c < nch) {
         const double* v = ws + 8 * tt;
-        a.part[(size_t)c * 4 + tt] =
+        a.part[(size_t)(in & 1) * nch * 4 + (size_t)c * 4 + tt] =
             ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
       }
     }
@@ -484,17 +532,18 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
     // the partials
     double cme_in = 0.0, cmold_in = 0.0;
     if (t == 0) {
-      cme_in = a.saved[6];
-      cmold_in = a.saved[7];
+      cme_in = a.saved[6 + 2 * (in & 1)];
+      cmold_in = a.saved[7 + 2 * (in & 1)];
     }
     // reduce the chunk partials (every block, identical tree); the loads of
     // two chunks are issued before their sums
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    const double* part = a.part + (size_t)(in & 1) * nch * 4;
     if (t < RED_XTHREADS) {
       int cc = t;
       for (; cc + RED_XTHREADS < nch; cc += 2 * RED_XTHREADS) {
-        const double* pa = a.part + (size_t)cc * 4;
-        const double* pb = a.part + (size_t)(cc + RED_XTHREADS) * 4;
+        const double* pa = part + (size_t)cc * 4;
+        const double* pb = part + (size_t)(cc + RED_XTHREADS) * 4;
         const double a0 = pa[0], a1 = pa[1], a2 = pa[2];
         const double b0 = pb[0], b1 = pb[1], b2 = pb[2];
         r0 = (r0 + a0) + b0;
@@ -502,7 +551,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         r2 = (r2 + a2) + b2;
       }
       if (cc < nch) {
-        const double* pa = a.part + (size_t)cc * 4;
+        const double* pa = part + (size_t)cc * 4;
         r0 = r0 + pa[0];
         r1 = r1 + pa[1];
         r2 = r2 + pa[2];
@@ -536,7 +585,13 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         if (i < np) __hip_atomic_store(&at(a.d, i), dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+    if (a.halo < 0) {
+      if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+    } else if (!eig_block) {
+      // the partials and the eigenvalue pair are double-buffered by
+      // iteration parity, so only the neighbours' d has to be waited for
+      if (!halo_sync(hflags, tile, hlo, hhi, (unsigned)(in + 1), err, a.barrier_ticks)) return;
+    }
   }
   // the result is the iterate of the last iteration index: u at the halt,
   // or the one before the final update when ITMAX ran out

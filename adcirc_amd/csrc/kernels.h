@@ -18,6 +18,9 @@ constexpr int JCG_BLOCK = RED_THREADS * JCG_CPB;  // threads per block
 constexpr int JCG_BAR_GROUP = 16;        // blocks per first-level barrier counter
 constexpr int JCG_BAR_STRIDE = 64;       // words between barrier counters (256 B)
 constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BAR_STRIDE
+                                          // + blocks * JCG_HALO_STRIDE
+constexpr int JCG_HALO_STRIDE = 16;      // words between per-tile halo flags (64 B)
+constexpr int JCG_HALO_MAX = 2;          // widest halo (tiles each side) waited on by flags
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
 constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
 // ordered wetting updates: candidates are closed in blocks of WD_BLOCK, each
@@ -147,12 +150,14 @@ struct JcgArgs {
   double* u;          // initial guess in (scaled in place), ETAS out
   double* d;          // pseudo-residual of the current iteration
   double* w;          // 1/sqrt(diag)
-  double* part;       // [nch*4] chunk partials (d.d, d.Nd, u.u)
-  double* saved;      // [8] ITPACK SAVE state q1, gamold, rhoold, cmold, cme,
-                      // stptst; [6..7] = cme, cmold of the running iteration
+  double* part;       // [2][nch*4] chunk partials (d.d, d.Nd, u.u), by iteration parity
+  double* saved;      // [12] ITPACK SAVE state q1, gamold, rhoold, cmold, cme,
+                      // stptst; [6..9] = cme, cmold of the running iteration, by parity
   int prescaled;      // rows already scaled (w, u, rhs) by k_rhs_node
   int defer_unscal;   // leave the off-diagonals scaled (k_lhs_rows restores them)
-  unsigned* barrier;  // [JCG_BAR_WORDS] grid barrier counters
+  unsigned* barrier;  // [JCG_BAR_WORDS] grid barrier counters, then per-tile halo flags
+  int halo;           // row tiles each side whose pseudo-residual a tile's rows read;
+                      // -1: wait for all tiles (a grid barrier) after each update
   long long barrier_ticks;  // longest grid-barrier wait, in wall-clock ticks
   int* flags;
   double* eta2;       // optional: ETA2 = NODECODE*(ETAS+0)+ETA1
