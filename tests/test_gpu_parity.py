@@ -291,3 +291,52 @@ def test_warm_start_within_jcg_tolerance_of_plain():
     log(f"600 steps: JCG iterations plain {out[0][0]}, extrapolated {out[1][0]}; "
         f"max |d eta| {de:.3e} m, nodecode differences {int((out[0][2] != out[1][2]).sum())}")
     assert de <= 1e-5
+
+
+def _renumbered(case, order):
+    """The case with its nodes renumbered: new node k is old node order[k]."""
+    import dataclasses
+    m = case.mesh
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.shape[0])
+    ren = lambda ids: (inv[np.asarray(ids, np.int64) - 1] + 1).astype(np.int32)  # noqa: E731
+    mesh = dataclasses.replace(m, x=m.x[order], y=m.y[order], dp=m.dp[order], nm=ren(m.nm),
+                               open_bnd=[ren(s) for s in m.open_bnd],
+                               land_bnd=[(ib, ren(s)) for ib, s in m.land_bnd])
+    return dataclasses.replace(case, mesh=mesh)
+
+
+def _bandwidth(d):
+    npn, mnei = int(d["np"]), int(d["mnei"])
+    nei = np.asarray(d["neitab"], np.int64).reshape(mnei, npn)
+    rows = np.broadcast_to(np.arange(npn), nei.shape)
+    real = nei > 0
+    return int(np.abs(nei[real] - 1 - rows[real]).max())
+
+
+@pytest.mark.parametrize("numbering", ["random", "windowed"])
+def test_renumbered_mesh_bitexact(numbering):
+    """The 20k-node coastal storm with its nodes renumbered, so that the
+    matrix bandwidth sends the JCG iterations down the paths that an
+    RCM-ordered mesh never takes: random numbering (bandwidth above two row
+    blocks of 4096: a second grid barrier after each update) and numbering
+    shuffled within windows of 4096 nodes (a halo of two row blocks each
+    side).  Bit-exact vs oracle."""
+    from gpu_helpers import case_setup
+    case = meshgen.coastal(20000)
+    npn = case.mesh.x.shape[0]
+    rng = np.random.default_rng(5)
+    if numbering == "random":
+        order = rng.permutation(npn)
+    else:
+        order = np.concatenate([w0 + rng.permutation(min(4096, npn - w0))
+                                for w0 in range(0, npn, 4096)])
+    case = _renumbered(case, order)
+    d = case_setup(case)
+    bw = _bandwidth(d)
+    log(f"case coastal {npn} nodes, {numbering} numbering: bandwidth {bw}, 12 steps")
+    if numbering == "random":
+        assert bw > 2 * 4096
+    else:
+        assert 4096 < bw <= 2 * 4096
+    _run_bitexact(d, case.run, 12, check_every=4)
