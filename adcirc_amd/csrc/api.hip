@@ -468,8 +468,17 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   }
   if (!rc) rc = setup_jcg_grid(c);
   if (!rc) {
-    // the wetting propagation stages candidates through LDS
-    hipError_t e = hipFuncSetAttribute((const void*)k_wd_link, hipFuncAttributeMaxDynamicSharedMemorySize,
+    // the LHS row kernel keeps MNEI x 512 doubles of LDS per block; the
+    // wetting propagation stages candidates through LDS
+    hipError_t e = hipFuncSetAttribute((const void*)k_lhs_rows<false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       mnei * RED_THREADS * (int)sizeof(double));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_lhs_rows<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              mnei * RED_THREADS * (int)sizeof(double));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_wd_link, hipFuncAttributeMaxDynamicSharedMemorySize,
                               WD_LINK_LDS);
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)k_wd_close, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -659,30 +668,27 @@ int adcgpu_set_forcing(adcgpu_ctx* c, const double* wsx2, const double* wsy2, co
 static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   const int np = c->np, ne = c->ne;
   const bool lumped = c->prm.ilump != 0;
+  const size_t rows_lds = (size_t)c->mnei * RED_THREADS * sizeof(double);
   const double* rcs = c->coef_scaled ? (const double*)c->cs : nullptr;
   const double* rw = c->coef_scaled ? (const double*)c->jw : nullptr;
-  const dim3 rg(nblk(np, 256)), rb(256);
-  // the row kernel's register width: the smallest instance that holds MNEI
-  if (lumped)
-    hipLaunchKernelGGL((k_lhs_rows<true, 2>), rg, rb, 0, c->stream, c->mesh(), c->state(),
-                       c->flags, rcs, rw);
-  else if (c->mnei <= 10)
-    hipLaunchKernelGGL((k_lhs_rows<false, 10>), rg, rb, 0, c->stream, c->mesh(), c->state(),
-                       c->flags, rcs, rw);
-  else if (c->mnei <= 16)
-    hipLaunchKernelGGL((k_lhs_rows<false, 16>), rg, rb, 0, c->stream, c->mesh(), c->state(),
-                       c->flags, rcs, rw);
-  else
-    hipLaunchKernelGGL((k_lhs_rows<false, LHS_MAXW>), rg, rb, 0, c->stream, c->mesh(),
-                       c->state(), c->flags, rcs, rw);
-  hipLaunchKernelGGL(k_ep_part, dim3(c->nch), dim3(RED_THREADS), 0, c->stream, c->mesh(),
-                     c->state(), c->flags, c->ep_part);
-  if (lumped)
+  // contrib, cx and cy are free until the GWCE right-hand side
+  if (lumped) {
+    hipLaunchKernelGGL(k_lhs_elem<true>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
+    hipLaunchKernelGGL(k_lhs_rows<true>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
+                       rcs, rw);
     hipLaunchKernelGGL(k_lhs_bc<true>, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(),
                        c->state(), c->flags, c->nch, c->ep_part, c->scal);
-  else
+  } else {
+    hipLaunchKernelGGL(k_lhs_elem<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
+    hipLaunchKernelGGL(k_lhs_rows<false>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
+                       rcs, rw);
     hipLaunchKernelGGL(k_lhs_bc<false>, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(),
                        c->state(), c->flags, c->nch, c->ep_part, c->scal);
+  }
   c->coef_scaled = false;
   RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, lumped ? 4.0 : 2.0, lumped ? 0.0 : 1.0,
            lumped ? 1 : 0, c->prm.nws};

@@ -22,7 +22,7 @@ constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BA
 constexpr int JCG_HALO_STRIDE = 16;      // words between per-tile halo flags (64 B)
 constexpr int JCG_HALO_MAX = 2;          // widest halo (tiles each side) waited on by flags
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
-constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI) the row kernel keeps in registers
+constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
 // ordered wetting updates: candidates are closed in blocks of WD_BLOCK, each
 // block naming at most WD_XMAX earlier candidates it depends on
 constexpr int WD_BLOCK = 1024;
@@ -174,10 +174,13 @@ __global__ void k_friction_holland(int np, FricPrm f, double* tk, const double* 
                                    HollandPrm p, double* wsx, double* wsy, double* pr);
 // LUMPED: the lumped GWCE (ILump = 1), whose LHS is the diagonal COEFD only
 // (kept in COEF's diagonal column) and whose solve is explicit
-template <bool LUMPED, int W>
-__global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* restore_cs,
-                           const double* restore_w);
-__global__ void k_ep_part(DevMesh m, DevState s, const int* flags, double* ep_part);
+template <bool LUMPED>
+__global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
+                           double* sb);
+template <bool LUMPED>
+__global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
+                           const double* sa, const double* sb, double* ep_part,
+                           const double* restore_cs, const double* restore_w);
 template <bool LUMPED>
 __global__ void k_lhs_bc(DevMesh m, DevState s, const int* flags, int nch, const double* ep_part,
                          double* scal);

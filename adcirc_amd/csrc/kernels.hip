@@ -107,34 +107,32 @@ __global__ __launch_bounds__(256) void k_friction_holland(int np, FricPrm f, dou
 // of the right-hand-side element kernel, which runs after them and before
 // wet/dry sets the flag again, clears it.
 
-// One incidence's terms of the GWCE matrix: the diagonal and the two
-// off-diagonals that vertex k of element e (code 3e+k) adds to its row
-// (src/gwce.F:538-592; the lumped diagonal COEFD, src/gwce.F:596-635).  The
-// row pass evaluates each element once per vertex; every evaluation performs
-// the same operations in the same order, so the three rows see the values
-// one element pass would have formed.
+// GWCE matrix, element part (src/gwce.F:456-592): for element e and its local
+// vertex k, the diagonal term and the two off-diagonal terms of row NM(e,k),
+// written to the row's incidence slot p = slot[3e+k] (sd, sa, sb).
 template <bool LUMPED>
-__device__ __forceinline__ void lhs_terms(const DevMesh& m, const DevState& s, int code,
-                                          double& vd, double& va, double& vb) {
-  const int e = code / 3, k = code - 3 * e;
+__global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
+                                                  const int* __restrict__ flags,
+                                                  double* __restrict__ sd, double* __restrict__ sa,
+                                                  double* __restrict__ sb) {
+  if (flags[F_NCCHANGE] == 0) return;
+  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
   const double dt = m.dt;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
   const double NCELE = (double)ncele;
   if (LUMPED) {
-    // COEFD: MsFacLOnDiag*SPM*NCELE with OnDiag = 4
+    // COEFD: MsFacLOnDiag*SPM*NCELE with OnDiag = 4 (src/gwce.F:596-635)
     const double area = m.areas[e] * 0.5;
     const double t0avg = (m.tau0[n1] + m.tau0[n2] + m.tau0[n3]) / 3.0;
     const double mson = (4.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
-    vd = mson * 1.0 * NCELE;
-    va = vb = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sd[m.slot[3 * e + k]] = mson * 1.0 * NCELE;
     return;
   }
-  // off-diagonals in the reference's local order: k+1, k+2 (mod 3) map to
-  // CSR_POSITION(e,k,1) and (e,k,2)
-  const int ka = (k == 0) ? 1 : 0, kb = (k == 2) ? 1 : 2;
-  const double fxk = m.fdx[3 * e + k], fxa = m.fdx[3 * e + ka], fxb = m.fdx[3 * e + kb];
-  const double fyk = m.fdy[3 * e + k], fya = m.fdy[3 * e + ka], fyb = m.fdy[3 * e + kb];
+  const double fx[3] = {m.fdx[3 * e], m.fdx[3 * e + 1], m.fdx[3 * e + 2]};
+  const double fy[3] = {m.fdy[3 * e], m.fdy[3 * e + 1], m.fdy[3 * e + 2]};
   const double area2 = m.areas[e];
   const double area = area2 * 0.5;
   const double area4 = area2 * 2.0;
@@ -143,96 +141,90 @@ __device__ __forceinline__ void lhs_terms(const DevMesh& m, const DevState& s, i
   const double t0avg = (m.tau0[n1] + m.tau0[n2] + m.tau0[n3]) / 3.0;
   const double mson = (2.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
   const double msoff = (1.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
-  vd = (mson * 1.0 + ga * (1.0 * fxk * fxk + 1.0 * fyk * fyk)) * NCELE;
-  va = (msoff * 1.0 + ga * (1.0 * fxk * fxa + 1.0 * fyk * fya)) * NCELE;
-  vb = (msoff * 1.0 + ga * (1.0 * fxk * fxb + 1.0 * fyk * fyb)) * NCELE;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    // diagonal: (MsOn + GA*(FDXk*FDXk + FDYk*FDYk))*NCELE; off-diagonals in
+    // the reference's local order (k+1, k+2 mod 3 map to CSR_POSITION(e,k,1)
+    // and (e,k,2))
+    const int ka = (k == 0) ? 1 : 0, kb = (k == 2) ? 1 : 2;
+    const int p = m.slot[3 * e + k];
+    sd[p] = (mson * 1.0 + ga * (1.0 * fx[k] * fx[k] + 1.0 * fy[k] * fy[k])) * NCELE;
+    sa[p] = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[ka] + 1.0 * fy[k] * fy[ka])) * NCELE;
+    sb[p] = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[kb] + 1.0 * fy[k] * fy[kb])) * NCELE;
+  }
 }
+template __global__ void k_lhs_elem<false>(DevMesh, DevState, const int*, double*, double*, double*);
+template __global__ void k_lhs_elem<true>(DevMesh, DevState, const int*, double*, double*, double*);
 
-// GWCE matrix: row i of COEF is the sum of its incident elements' terms in
-// ascending element order (the reference adds into COEF(i,j) element by
-// element, src/gwce.F:456-592).  One thread per row; the row's W columns are
-// registers, each term added to its column by a predicated update (an
-// accumulator that starts at +0 never holds -0, so the untaken adds are exact
-// no-ops).  When the matrix is not rebuilt, the off-diagonals that the last
-// solve left scaled are restored as unscal does (src/itpackv.F:2460-2479).
-template <bool LUMPED, int W>
-__global__ __launch_bounds__(256) void k_lhs_rows(DevMesh m, DevState s,
-                                                  const int* __restrict__ flags,
-                                                  const double* __restrict__ restore_cs,
-                                                  const double* __restrict__ restore_w) {
-  const int np = m.np;
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const int nn = m.nneigh[i];
-  if (flags[F_NCCHANGE] == 0) {
-    if (restore_cs) {
-      const double ri = 1.0 / restore_w[i];
-      for (int j = 1; j < nn; ++j) {
-        const size_t q = (size_t)j * np + i;
-        s.coef[q] = restore_cs[q] * ri * (1.0 / restore_w[m.col[q]]);
-      }
-    }
-    return;
-  }
-  const int p0 = m.inc_ptr[i], p1 = m.inc_ptr[i + 1];
-  if (LUMPED) {
-    // COEFD only; the off-diagonal columns stay zero
-    double diag = 0.0;
-    for (int p = p0; p < p1; ++p) {
-      double vd, va, vb;
-      lhs_terms<true>(m, s, m.inc[p], vd, va, vb);
-      diag = diag + vd;
-    }
-    s.coef[i] = diag;
-    for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = 0.0;
-    return;
-  }
-  double acc[W];
-#pragma unroll
-  for (int j = 0; j < W; ++j) acc[j] = 0.0;
-  double diag = 0.0;
-  for (int p = p0; p < p1; ++p) {
-    const int jab = m.slot_cols[p];
-    const int ja = jab & 0xffff, jb = jab >> 16;
-    double vd, va, vb;
-    lhs_terms<false>(m, s, m.inc[p], vd, va, vb);
-    diag = diag + vd;
-#pragma unroll
-    for (int j = 1; j < W; ++j) {
-      const double xa = acc[j] + va, xb = acc[j] + vb;
-      acc[j] = j == ja ? xa : (j == jb ? xb : acc[j]);
-    }
-  }
-  s.coef[i] = diag;
-#pragma unroll
-  for (int j = 1; j < W; ++j)
-    if (j < nn) s.coef[(size_t)j * np + i] = acc[j];
-}
-template __global__ void k_lhs_rows<true, 2>(DevMesh, DevState, const int*, const double*, const double*);
-template __global__ void k_lhs_rows<false, 10>(DevMesh, DevState, const int*, const double*, const double*);
-template __global__ void k_lhs_rows<false, 16>(DevMesh, DevState, const int*, const double*, const double*);
-template __global__ void k_lhs_rows<false, LHS_MAXW>(DevMesh, DevState, const int*, const double*, const double*);
-
-// EP chunk partials of sum COEF(i,1)^2 (src/gwce.F:646-650) over the rebuilt
-// diagonals, in the fixed blocked tree (thread t of a 2048-row chunk takes
-// rows t, t+512, t+1024, t+1536 in order).
-__global__ __launch_bounds__(RED_THREADS) void k_ep_part(DevMesh m, DevState s,
-                                                         const int* __restrict__ flags,
-                                                         double* __restrict__ ep_part) {
+// GWCE matrix, row part: row i of COEF is the sum of its incidence slots in
+// ascending element order, each row held in LDS while it is formed (the
+// reference adds into COEF(i,j) element by element, src/gwce.F:538-592);
+// also the EP chunk partials of sum COEF(i,1)^2 (src/gwce.F:646-650) in the
+// fixed blocked tree.  Dynamic LDS: MNEI x RED_THREADS doubles, slot-major.
+template <bool LUMPED>
+__global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
+                                                          const int* __restrict__ flags,
+                                                          const double* __restrict__ sd,
+                                                          const double* __restrict__ sa,
+                                                          const double* __restrict__ sb,
+                                                          double* __restrict__ ep_part,
+                                                          const double* __restrict__ restore_cs,
+                                                          const double* __restrict__ restore_w) {
   __shared__ double lds[8];
-  if (flags[F_NCCHANGE] == 0) return;
+  extern __shared__ double racc_lds[];
+  double(*racc)[RED_THREADS] = reinterpret_cast<double(*)[RED_THREADS]>(racc_lds);
+  const int np = m.np;
+  const int t = threadIdx.x;
   const int chunk = xcd_tile(blockIdx.x, gridDim.x);
-  const int base = chunk * RED_CHUNK + threadIdx.x;
+  const int base = chunk * RED_CHUNK + t;
+  if (flags[F_NCCHANGE] == 0) {
+    // no rebuild: the previous solve left the off-diagonals scaled; restore
+    // them as unscal does (src/itpackv.F:2460-2479), same operations
+    if (restore_cs)
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        const int i = base + RED_THREADS * r;
+        if (i >= np) break;
+        const int nn = m.nneigh[i];
+        const double ri = 1.0 / restore_w[i];
+        for (int j = 1; j < nn; ++j) {
+          const size_t q = (size_t)j * np + i;
+          s.coef[q] = restore_cs[q] * ri * (1.0 / restore_w[m.col[q]]);
+        }
+      }
+    return;
+  }
   double acc = 0.0;
   for (int r = 0; r < RED_PER_THREAD; ++r) {
     const int i = base + RED_THREADS * r;
-    if (i >= m.np) break;
-    const double diag = s.coef[i];
+    if (i >= np) break;
+    const int nn = m.nneigh[i];
+    if (LUMPED) {
+      // COEFD only; the off-diagonal columns stay zero
+      double diag = 0.0;
+      for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) diag = diag + sd[p];
+      s.coef[i] = diag;
+      for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = 0.0;
+      acc = acc + diag * diag;
+      continue;
+    }
+    for (int j = 1; j < nn; ++j) racc[j][t] = 0.0;
+    double diag = 0.0;
+    for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
+      const int jab = m.slot_cols[p];
+      const int ja = jab & 0xffff, jb = jab >> 16;
+      diag = diag + sd[p];
+      racc[ja][t] = racc[ja][t] + sa[p];
+      racc[jb][t] = racc[jb][t] + sb[p];
+    }
+    s.coef[i] = diag;
+    for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = racc[j][t];
     acc = acc + diag * diag;
   }
-  const double tot = chunk_tree(acc, lds);
-  if (threadIdx.x == 0) ep_part[chunk] = tot;
+  double tot = chunk_tree(acc, lds);
+  if (t == 0) ep_part[chunk] = tot;
 }
+template __global__ void k_lhs_rows<false>(DevMesh, DevState, const int*, const double*, const double*, const double*, double*, const double*, const double*);
+template __global__ void k_lhs_rows<true>(DevMesh, DevState, const int*, const double*, const double*, const double*, double*, const double*, const double*);
 
 // EP = sqrt(sum of squared diagonals / NP) from the chunk partials, and the
 // elevation-boundary rows (src/gwce.F:700-743): a boundary row becomes EP on
