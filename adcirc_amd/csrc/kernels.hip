@@ -122,6 +122,19 @@ __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
   const double NCELE = (double)ncele;
+  if (ncele == 0) {
+    // dry element: its terms x NCELE are zeros (see k_rhs_elem)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int p = m.slot[3 * e + k];
+      sd[p] = 0.0;
+      if (!LUMPED) {
+        sa[p] = 0.0;
+        sb[p] = 0.0;
+      }
+    }
+    return;
+  }
   if (LUMPED) {
     // COEFD: MsFacLOnDiag*SPM*NCELE with OnDiag = 4 (src/gwce.F:596-635)
     const double area = m.areas[e] * 0.5;
@@ -301,6 +314,15 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
   if (e >= m.ne) return;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
+  if (ncele == 0) {
+    // a dry element's contributions are its (finite) terms x 0: zeros, and a
+    // node sum that starts at +0 never holds -0, so a zero of either sign
+    // leaves it unchanged; the terms are skipped
+    contrib[m.slot[3 * e]] = 0.0;
+    contrib[m.slot[3 * e + 1]] = 0.0;
+    contrib[m.slot[3 * e + 2]] = 0.0;
+    return;
+  }
   const double DT = m.dt, G = m.g, A00 = m.a00, B00 = m.b00, C00 = m.c00;
   const double E0N1 = s.eta1[n1], E0N2 = s.eta1[n2], E0N3 = s.eta1[n3];
   const double E1N1 = s.eta2[n1], E1N2 = s.eta2[n2], E1N3 = s.eta2[n3];
@@ -1034,6 +1056,16 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
   const double THIRD = 1.0 / 3.0, DT = m.dt, GO2 = m.g / 2.0;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
+  if (ncele == 0) {
+    // dry element: every contribution is NCEle*DT x (...) = 0 (see k_rhs_elem)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int sl = m.slot[3 * e + k];
+      cx[sl] = 0.0;
+      cy[sl] = 0.0;
+    }
+    return;
+  }
   const double U1N1 = s.uu1[n1], U1N2 = s.uu1[n2], U1N3 = s.uu1[n3];
   const double V1N1 = s.vv1[n1], V1N2 = s.vv1[n2], V1N3 = s.vv1[n3];
   const double H1N1 = s.h1[n1], H1N2 = s.h1[n2], H1N3 = s.h1[n3];

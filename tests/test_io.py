@@ -193,3 +193,24 @@ def test_global_file_and_maxele(tmp_path):
     h, recs = outputs.read_global(str(tmp_path / "maxele.63"))
     assert h["nsets"] == 2 and len(recs) == 2
     assert np.array_equal(recs[0][2][:, 0], emax) and np.array_equal(recs[1][2][:, 0], t)
+
+
+def test_driver_refuses_what_it_does_not_write(tmp_path):
+    """The case driver refuses hot starts, station output, harmonic analysis,
+    non-ASCII global output and NWS = 8 without vortex parameters, before it
+    touches the GPU."""
+    from adcirc_amd import driver
+    base = fort15.from_case(meshgen.beach(), fort15.Output(1, 0.0, 1.0, 60), fort15.Output())
+    cases = {
+        "IHOT": dataclasses.replace(base, ihot=67),
+        "NHSTAR": dataclasses.replace(base, nhstar=1, nhsinc=100),
+        "NOUTE": dataclasses.replace(base, elev_sta=fort15.Output(1, 0.0, 1.0, 10),
+                                     stations_e=np.array([[100.0, 200.0]])),
+        "NFREQ": dataclasses.replace(base, nfreq=1),
+        "NOUTGE": dataclasses.replace(base, elev=fort15.Output(3, 0.0, 1.0, 60)),
+        "fort.22": base,  # NWS = 8 (the beach storm) and no parameters passed
+    }
+    for word, f in cases.items():
+        with pytest.raises(fort15.Fort15Error, match=word):
+            driver._check(f, None)
+    driver._check(base, meshgen.beach().run.holland)  # accepted
