@@ -48,5 +48,10 @@ def basin_golden():
 
 
 @pytest.fixture(scope="session")
+def basin_linear_golden():
+    return load_golden("basin_linear")
+
+
+@pytest.fixture(scope="session")
 def jcg_golden():
     return load_golden("jcg")

@@ -7,7 +7,7 @@ numpy archives.  Needs the reference build (``make -C oracle ref``), so it runs
 in the build container only; the fixtures it writes are what the tests use
 everywhere else.
 
-    python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [jcg] [tau0]
+    python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [basin_linear] [jcg] [tau0]
 """
 from __future__ import annotations
 
@@ -179,7 +179,7 @@ def make_tau0() -> str:
 
 
 def main(argv):
-    todo = argv or ["basin", "beach", "beach_lumped", "jcg", "tau0"]
+    todo = argv or ["basin", "beach", "beach_lumped", "basin_linear", "jcg", "tau0"]
     if "basin" in todo:
         print(run_case("basin", meshgen.basin(), nsteps=43200, early=10, dump_every=43200))
     if "beach" in todo:
@@ -189,6 +189,13 @@ def main(argv):
         case = meshgen.beach()
         case.run.ilump = 1
         print(run_case("beach_lumped", case, nsteps=3600, early=10, dump_every=600))
+    if "basin_linear" in todo:
+        # the basin with linear friction (NOLIBF = 0, TAU = 1e-4 /s) and the
+        # finite-amplitude and advection terms off (NOLIFA = NOLICA = NOLICAT = 0)
+        case = meshgen.basin()
+        r = case.run
+        r.nolibf, r.cf, r.nolifa, r.nolica, r.nolicat = 0, 1.0e-4, 0, 0, 0
+        print(run_case("basin_linear", case, nsteps=3600, early=10, dump_every=1200))
     if "jcg" in todo:
         print(make_jcg())
     if "tau0" in todo:

@@ -47,6 +47,21 @@ def test_beach_hour_bitexact_and_reference(beach_golden):
     assert np.abs(g.get("vv2") - d[f"s{s}_vv2"]).max() <= 1e-6
 
 
+def test_basin_linear_terms_bitexact_and_reference(basin_linear_golden):
+    """Linear friction (NOLIBF = 0), NOLIFA = NOLICA = NOLICAT = 0 on the
+    basin, 3600 steps: bit-exact vs oracle every 50 steps, and within the
+    reference's bar of the reference build's run."""
+    from test_oracle import basin_linear_run
+    d = basin_linear_golden
+    log("case basin linear: 3600 steps, bit-exact vs oracle")
+    g, o = _run_bitexact(d, basin_linear_run(), 3600, check_every=50)
+    s = 3600
+    assert np.array_equal(g.get("nodecode"), d[f"s{s}_nodecode"])
+    assert np.abs(g.get("eta2") - d[f"s{s}_eta2"]).max() <= 1e-6
+    assert np.abs(g.get("uu2") - d[f"s{s}_uu2"]).max() <= 1e-6
+    assert np.abs(g.get("vv2") - d[f"s{s}_vv2"]).max() <= 1e-6
+
+
 def test_coastal_mesh_bitexact():
     """The product's own preprocessing on a generated 20k-node coastal mesh
     (Delaunay, wet/dry, M2+K1, Holland storm): bit-exact vs oracle."""

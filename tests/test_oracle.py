@@ -55,6 +55,21 @@ def test_beach_lumped_gwce_hour_matches_reference(beach_lumped_golden):
     _run(beach_lumped_golden, run, 3600)
 
 
+def basin_linear_run():
+    """The basin with linear friction (NOLIBF = 0, TAU = 1e-4 /s) and no
+    finite-amplitude or advective terms (NOLIFA = NOLICA = NOLICAT = 0)."""
+    run = meshgen.basin().run
+    run.nolibf, run.cf, run.nolifa, run.nolica, run.nolicat = 0, 1.0e-4, 0, 0, 0
+    return run
+
+
+def test_basin_linear_terms_match_reference(basin_linear_golden):
+    """Linear friction, linear continuity and no advection (the IFLINBF,
+    IFNLFA, IFNLCT and IFNLCAT switches off their default), 3600 steps:
+    bit-identical every step."""
+    _run(basin_linear_golden, basin_linear_run(), 3600)
+
+
 def test_basin_matches_reference_first_3000_steps(basin_golden):
     _run(basin_golden, meshgen.basin().run, 3000)
 
