@@ -22,6 +22,7 @@ constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BA
 constexpr int JCG_HALO_STRIDE = 16;      // words between per-tile halo flags (64 B)
 constexpr int JCG_HALO_MAX = 2;          // widest halo (tiles each side) waited on by flags
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
+constexpr int NODE_STAGE = 2048;   // incidence slots a 256-node block stages in LDS
 constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
 // ordered wetting updates: candidates are closed in blocks of WD_BLOCK, each
 // block naming at most WD_XMAX earlier candidates it depends on

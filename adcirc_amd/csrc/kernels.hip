@@ -427,8 +427,17 @@ template <bool LUMPED>
 __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ contrib,
                            const double* __restrict__ eta_bc, const double* __restrict__ scal,
                            JcgPre pre) {
-  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  // the block's slot range staged in LDS, as in k_mom_node
+  __shared__ double sc[NODE_STAGE];
+  const int r0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+  const int i = r0 + threadIdx.x;
   if (i < pre.barrier_words) pre.barrier[i] = 0u;
+  const int p0 = m.inc_ptr[r0];
+  const int n_st = m.inc_ptr[min(r0 + (int)blockDim.x, m.np)] - p0;
+  const bool staged = n_st <= NODE_STAGE;
+  if (staged)
+    for (int k = threadIdx.x; k < n_st; k += blockDim.x) sc[k] = contrib[p0 + k];
+  __syncthreads();
   if (i >= m.np) return;
   const int bi = m.bnd_index[i];
   double rhs, u;
@@ -441,7 +450,10 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
     rhs = es * (double)s.nodecode[i] * (LUMPED ? s.coef[i] : scal[S_EP]);
   } else {
     double acc = 0.0;
-    for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
+    if (staged)
+      for (int p = m.inc_ptr[i] - p0; p < m.inc_ptr[i + 1] - p0; ++p) acc = acc + sc[p];
+    else
+      for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
     for (int p = m.obcg_ptr[i]; p < (LUMPED ? m.obcg_ptr[i] : m.obcg_ptr[i + 1]); ++p) {
       const int slot = m.obcg[p];
       const int I = slot / m.mnei;
@@ -972,12 +984,23 @@ __global__ __launch_bounds__(256) void k_wd_area(DevMesh m, DevState s, double* 
 __global__ __launch_bounds__(256) void k_wd_d2(DevMesh m, DevState s,
                                                const double* __restrict__ area_slot, int* flags,
                                                double ifnlfa) {
-  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  // the block's slot range staged in LDS, as in k_mom_node
+  __shared__ double sa[NODE_STAGE];
+  const int r0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+  const int i = r0 + threadIdx.x;
+  const int p0 = m.inc_ptr[r0];
+  const int n_st = m.inc_ptr[min(r0 + (int)blockDim.x, m.np)] - p0;
+  const bool staged = n_st <= NODE_STAGE;
+  if (staged)
+    for (int k = threadIdx.x; k < n_st; k += blockDim.x) sa[k] = area_slot[p0 + k];
+  __syncthreads();
   if (i >= m.np) return;
   int mju = 0;
   double ta = 0.0;
+  const int q0 = staged ? p0 : 0;
+  const double* src = staged ? sa : area_slot;
   for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
-    const double a = area_slot[p];
+    const double a = src[p - q0];
     mju += a != 0.0 ? 1 : 0;
     ta = ta + a;
   }
@@ -1124,14 +1147,36 @@ template __global__ void k_mom_elem<true>(DevMesh, DevState, MomPrm, double*, do
 // then the peak-surge maximum (src/write_output.F:6357-6366).
 __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* __restrict__ cx,
                            const double* __restrict__ cy) {
-  int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  // the block's 256 nodes own one contiguous range of incidence slots: it is
+  // read once, coalesced, into LDS, and each node then sums its own slots
+  // from there in order (the same sums as reading them directly)
+  __shared__ double sx[NODE_STAGE], sy[NODE_STAGE];
+  const int r0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+  const int i = r0 + threadIdx.x;
+  const int p0 = m.inc_ptr[r0];
+  const int n_st = m.inc_ptr[min(r0 + (int)blockDim.x, m.np)] - p0;
+  const bool staged = n_st <= NODE_STAGE;
+  if (staged) {
+    for (int k = threadIdx.x; k < n_st; k += blockDim.x) {
+      sx[k] = cx[p0 + k];
+      sy[k] = cy[p0 + k];
+    }
+  }
+  __syncthreads();
   if (i >= m.np) return;
   const double DTO2 = m.dt / 2.0;
   const int nci = s.nodecode[i];
   double mx = 0.0, my = 0.0;
-  for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
-    mx = mx + cx[p];
-    my = my + cy[p];
+  if (staged) {
+    for (int p = m.inc_ptr[i] - p0; p < m.inc_ptr[i + 1] - p0; ++p) {
+      mx = mx + sx[p];
+      my = my + sy[p];
+    }
+  } else {
+    for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
+      mx = mx + cx[p];
+      my = my + cy[p];
+    }
   }
   const double ta = s.totalarea[i];
   if (ta != 0.0) {
