@@ -122,19 +122,6 @@ __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
   const double NCELE = (double)ncele;
-  if (ncele == 0) {
-    // dry element: its terms x NCELE are zeros (see k_rhs_elem)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int p = m.slot[3 * e + k];
-      sd[p] = 0.0;
-      if (!LUMPED) {
-        sa[p] = 0.0;
-        sb[p] = 0.0;
-      }
-    }
-    return;
-  }
   if (LUMPED) {
     // COEFD: MsFacLOnDiag*SPM*NCELE with OnDiag = 4 (src/gwce.F:596-635)
     const double area = m.areas[e] * 0.5;
@@ -314,110 +301,111 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
   if (e >= m.ne) return;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
-  if (ncele == 0) {
-    // a dry element's contributions are its (finite) terms x 0: zeros, and a
-    // node sum that starts at +0 never holds -0, so a zero of either sign
-    // leaves it unchanged; the terms are skipped
-    contrib[m.slot[3 * e]] = 0.0;
-    contrib[m.slot[3 * e + 1]] = 0.0;
-    contrib[m.slot[3 * e + 2]] = 0.0;
-    return;
+  // a dry element's contributions are its (finite) terms x 0: zeros, and a
+  // node sum that starts at +0 never holds -0, so a zero of either sign
+  // leaves it unchanged; the terms are skipped, and all lanes meet again for
+  // one set of stores
+  double R1 = 0.0, R2 = 0.0, R3 = 0.0;
+  if (ncele != 0) {
+    const double DT = m.dt, G = m.g, A00 = m.a00, B00 = m.b00, C00 = m.c00;
+    const double E0N1 = s.eta1[n1], E0N2 = s.eta1[n2], E0N3 = s.eta1[n3];
+    const double E1N1 = s.eta2[n1], E1N2 = s.eta2[n2], E1N3 = s.eta2[n3];
+    const double E1N1SQ = E1N1 * E1N1, E1N2SQ = E1N2 * E1N2, E1N3SQ = E1N3 * E1N3;
+    const double ESN1 = s.etas[n1], ESN2 = s.etas[n2], ESN3 = s.etas[n3];
+    const double U1N1 = s.uu1[n1], U1N2 = s.uu1[n2], U1N3 = s.uu1[n3];
+    const double V1N1 = s.vv1[n1], V1N2 = s.vv1[n2], V1N3 = s.vv1[n3];
+    const double QX1N1 = s.qx1[n1], QX1N2 = s.qx1[n2], QX1N3 = s.qx1[n3];
+    const double QY1N1 = s.qy1[n1], QY1N2 = s.qy1[n2], QY1N3 = s.qy1[n3];
+    const double H1N1 = s.h2[n1], H1N2 = s.h2[n2], H1N3 = s.h2[n3];
+    const double EVMN1 = m.evm[n1], EVMN2 = m.evm[n2], EVMN3 = m.evm[n3];
+    const double T0N1 = m.tau0[n1], T0N2 = m.tau0[n2], T0N3 = m.tau0[n3];
+    double PR1N1 = 0.0, PR1N2 = 0.0, PR1N3 = 0.0, WSXAvg = 0.0, WSYAvg = 0.0;
+    if (q.nws) {
+      PR1N1 = s.pr1[n1]; PR1N2 = s.pr1[n2]; PR1N3 = s.pr1[n3];
+      WSXAvg = (1.0 * s.wsx1[n1] + 1.0 * s.wsx1[n2] + 1.0 * s.wsx1[n3]) / 3.0;
+      WSYAvg = (1.0 * s.wsy1[n1] + 1.0 * s.wsy1[n2] + 1.0 * s.wsy1[n3]) / 3.0;
+    }
+    // TKM(1)=TKM(2)=TK, TKM(3)=0
+    const double tk1 = s.tk[n1], tk2 = s.tk[n2], tk3 = s.tk[n3];
+    const double BSXN1 = tk1 * QX1N1 + 0.0 * QY1N1, BSYN1 = 0.0 * QX1N1 + tk1 * QY1N1;
+    const double BSXN2 = tk2 * QX1N2 + 0.0 * QY1N2, BSYN2 = 0.0 * QX1N2 + tk2 * QY1N2;
+    const double BSXN3 = tk3 * QX1N3 + 0.0 * QY1N3, BSYN3 = 0.0 * QX1N3 + tk3 * QY1N3;
+    const double AreaIE2 = m.areas[e];
+    const double AreaIE = AreaIE2 / 2.0;
+    const double AreaIE4 = 2.0 * AreaIE2;
+    const double FDX1 = m.fdx[3 * e], FDX2 = m.fdx[3 * e + 1], FDX3 = m.fdx[3 * e + 2];
+    const double FDY1 = m.fdy[3 * e], FDY2 = m.fdy[3 * e + 1], FDY3 = m.fdy[3 * e + 2];
+    const double E0XGrad2A = q.lumped ? 0.0 : E0N1 * FDX1 + E0N2 * FDX2 + E0N3 * FDX3;
+    const double E0YGrad2A = q.lumped ? 0.0 : E0N1 * FDY1 + E0N2 * FDY2 + E0N3 * FDY3;
+    const double E1XGrad2A = E1N1 * FDX1 + E1N2 * FDX2 + E1N3 * FDX3;
+    const double E1YGrad2A = E1N1 * FDY1 + E1N2 * FDY2 + E1N3 * FDY3;
+    const double Tau0XGrad2A = 1.0 * (T0N1 * FDX1 + T0N2 * FDX2 + T0N3 * FDX3);
+    const double Tau0YGrad2A = 1.0 * (T0N1 * FDY1 + T0N2 * FDY2 + T0N3 * FDY3);
+    const double EVMXGrad = (EVMN1 * FDX1 + EVMN2 * FDX2 + EVMN3 * FDX3) / AreaIE2;
+    const double EVMYGrad = (EVMN1 * FDY1 + EVMN2 * FDY2 + EVMN3 * FDY3) / AreaIE2;
+    const double EVMAvgODT = ((EVMN1 + EVMN2 + EVMN3) / 3.0) / DT;
+    const double MX = (EVMXGrad * (QX1N1 * FDX1 + QX1N2 * FDX2 + QX1N3 * FDX3) +
+                       EVMYGrad * (QY1N1 * FDX1 + QY1N2 * FDX2 + QY1N3 * FDX3) -
+                       EVMAvgODT * (ESN1 * FDX1 + ESN2 * FDX2 + ESN3 * FDX3)) / AreaIE2;
+    const double MY = (EVMXGrad * (QX1N1 * FDY1 + QX1N2 * FDY2 + QX1N3 * FDY3) +
+                       EVMYGrad * (QY1N1 * FDY1 + QY1N2 * FDY2 + QY1N3 * FDY3) -
+                       EVMAvgODT * (ESN1 * FDY1 + ESN2 * FDY2 + ESN3 * FDY3)) / AreaIE2;
+    const double CorifAvg = (m.corif[n1] + m.corif[n2] + m.corif[n3]) / 3.0;
+    const double Tau0Avg = (T0N1 + T0N2 + T0N3) / 3.0;
+    const double Tau0QXAvg = (T0N1 * QX1N1 + T0N2 * QX1N2 + T0N3 * QX1N3) / 3.0;
+    const double Tau0QYAvg = (T0N1 * QY1N1 + T0N2 * QY1N2 + T0N3 * QY1N3) / 3.0;
+    const double U1Avg = (U1N1 + U1N2 + U1N3) / 3.0;
+    const double V1Avg = (V1N1 + V1N2 + V1N3) / 3.0;
+    const double QX1Avg = (QX1N1 + QX1N2 + QX1N3) / 3.0;
+    const double QY1Avg = (QY1N1 + QY1N2 + QY1N3) / 3.0;
+    const double ESAvg = (ESN1 + ESN2 + ESN3) / 3.0;
+    const double DPAvg = (m.dp[n1] + m.dp[n2] + m.dp[n3]) / 3.0;
+    const double al = s.alphal[e];
+    const double GDP = al * G * DPAvg * 1.0 / AreaIE4;   // both time levels (IFNL_HDP = 0)
+    const double HAvg = (H1N1 + H1N2 + H1N3) / 3.0;
+    const double GHAvgOAreaIE2 = (al * G * HAvg) / AreaIE2;
+    const double BSXAvg = (BSXN1 + BSXN2 + BSXN3) / 3.0;
+    const double BSYAvg = (BSYN1 + BSYN2 + BSYN3) / 3.0;
+    const double MsFacR = AreaIE * (1.0 / DT - Tau0Avg / 2.0) / DT / 12.0;
+    const double GOAreaIE4 = al * G / AreaIE4;
+    const double Tau0SpaVar = (QX1Avg * Tau0XGrad2A + QY1Avg * Tau0YGrad2A) / 6.0;
+    const double A00pB00 = A00 + B00;
+    double JXAvg = CorifAvg * QY1Avg -
+                   q.ifnlfa * GOAreaIE4 * 1.0 * (E1N1SQ * FDX1 + E1N2SQ * FDX2 + E1N3SQ * FDX3) -
+                   GHAvgOAreaIE2 * (PR1N1 * FDX1 + PR1N2 * FDX2 + PR1N3 * FDX3) +
+                   WSXAvg - BSXAvg + MX + Tau0QXAvg;
+    double JYAvg = -CorifAvg * QX1Avg -
+                   q.ifnlfa * GOAreaIE4 * 1.0 * (E1N1SQ * FDY1 + E1N2SQ * FDY2 + E1N3SQ * FDY3) -
+                   GHAvgOAreaIE2 * (PR1N1 * FDY1 + PR1N2 * FDY2 + PR1N3 * FDY3) +
+                   WSYAvg - BSYAvg + MY + Tau0QYAvg;
+    JXAvg = JXAvg -
+            q.ifnlct * (QX1Avg * (U1N1 * FDX1 + U1N2 * FDX2 + U1N3 * FDX3) +
+                        QY1Avg * (U1N1 * FDY1 + U1N2 * FDY2 + U1N3 * FDY3)) / AreaIE2 +
+            q.ifnlcat * U1Avg * ESAvg / DT;
+    JYAvg = JYAvg -
+            q.ifnlct * (QX1Avg * (V1N1 * FDX1 + V1N2 * FDX2 + V1N3 * FDX3) +
+                        QY1Avg * (V1N1 * FDY1 + V1N2 * FDY2 + V1N3 * FDY3)) / AreaIE2 +
+            q.ifnlcat * V1Avg * ESAvg / DT;
+    const double A1 = (q.ondiag * ESN1 * 1.0 + q.offdiag * (ESN2 * 1.0 + ESN3 * 1.0)) * MsFacR -
+                      GDP * (C00 * (1.0 * FDX1 * E0XGrad2A + 1.0 * FDY1 * E0YGrad2A)) -
+                      GDP * (A00pB00 * (1.0 * FDX1 * E1XGrad2A + 1.0 * FDY1 * E1YGrad2A)) +
+                      (1.0 * JXAvg * FDX1 + 1.0 * JYAvg * FDY1) * 0.5 + Tau0SpaVar;
+    const double A2 = (q.ondiag * ESN2 * 1.0 + q.offdiag * (ESN1 * 1.0 + ESN3 * 1.0)) * MsFacR -
+                      GDP * (C00 * (1.0 * FDX2 * E0XGrad2A + 1.0 * FDY2 * E0YGrad2A)) -
+                      GDP * (A00pB00 * (1.0 * FDX2 * E1XGrad2A + 1.0 * FDY2 * E1YGrad2A)) +
+                      (1.0 * JXAvg * FDX2 + 1.0 * JYAvg * FDY2) * 0.5 + Tau0SpaVar;
+    const double A3 = (q.ondiag * ESN3 * 1.0 + q.offdiag * (ESN1 * 1.0 + ESN2 * 1.0)) * MsFacR -
+                      GDP * (C00 * (1.0 * FDX3 * E0XGrad2A + 1.0 * FDY3 * E0YGrad2A)) -
+                      GDP * (A00pB00 * (1.0 * FDX3 * E1XGrad2A + 1.0 * FDY3 * E1YGrad2A)) +
+                      (1.0 * JXAvg * FDX3 + 1.0 * JYAvg * FDY3) * 0.5 + Tau0SpaVar;
+    R1 = A1;
+    R2 = A2;
+    R3 = A3;
   }
-  const double DT = m.dt, G = m.g, A00 = m.a00, B00 = m.b00, C00 = m.c00;
-  const double E0N1 = s.eta1[n1], E0N2 = s.eta1[n2], E0N3 = s.eta1[n3];
-  const double E1N1 = s.eta2[n1], E1N2 = s.eta2[n2], E1N3 = s.eta2[n3];
-  const double E1N1SQ = E1N1 * E1N1, E1N2SQ = E1N2 * E1N2, E1N3SQ = E1N3 * E1N3;
-  const double ESN1 = s.etas[n1], ESN2 = s.etas[n2], ESN3 = s.etas[n3];
-  const double U1N1 = s.uu1[n1], U1N2 = s.uu1[n2], U1N3 = s.uu1[n3];
-  const double V1N1 = s.vv1[n1], V1N2 = s.vv1[n2], V1N3 = s.vv1[n3];
-  const double QX1N1 = s.qx1[n1], QX1N2 = s.qx1[n2], QX1N3 = s.qx1[n3];
-  const double QY1N1 = s.qy1[n1], QY1N2 = s.qy1[n2], QY1N3 = s.qy1[n3];
-  const double H1N1 = s.h2[n1], H1N2 = s.h2[n2], H1N3 = s.h2[n3];
-  const double EVMN1 = m.evm[n1], EVMN2 = m.evm[n2], EVMN3 = m.evm[n3];
-  const double T0N1 = m.tau0[n1], T0N2 = m.tau0[n2], T0N3 = m.tau0[n3];
-  double PR1N1 = 0.0, PR1N2 = 0.0, PR1N3 = 0.0, WSXAvg = 0.0, WSYAvg = 0.0;
-  if (q.nws) {
-    PR1N1 = s.pr1[n1]; PR1N2 = s.pr1[n2]; PR1N3 = s.pr1[n3];
-    WSXAvg = (1.0 * s.wsx1[n1] + 1.0 * s.wsx1[n2] + 1.0 * s.wsx1[n3]) / 3.0;
-    WSYAvg = (1.0 * s.wsy1[n1] + 1.0 * s.wsy1[n2] + 1.0 * s.wsy1[n3]) / 3.0;
-  }
-  // TKM(1)=TKM(2)=TK, TKM(3)=0
-  const double tk1 = s.tk[n1], tk2 = s.tk[n2], tk3 = s.tk[n3];
-  const double BSXN1 = tk1 * QX1N1 + 0.0 * QY1N1, BSYN1 = 0.0 * QX1N1 + tk1 * QY1N1;
-  const double BSXN2 = tk2 * QX1N2 + 0.0 * QY1N2, BSYN2 = 0.0 * QX1N2 + tk2 * QY1N2;
-  const double BSXN3 = tk3 * QX1N3 + 0.0 * QY1N3, BSYN3 = 0.0 * QX1N3 + tk3 * QY1N3;
-  const double AreaIE2 = m.areas[e];
-  const double AreaIE = AreaIE2 / 2.0;
-  const double AreaIE4 = 2.0 * AreaIE2;
-  const double FDX1 = m.fdx[3 * e], FDX2 = m.fdx[3 * e + 1], FDX3 = m.fdx[3 * e + 2];
-  const double FDY1 = m.fdy[3 * e], FDY2 = m.fdy[3 * e + 1], FDY3 = m.fdy[3 * e + 2];
-  const double E0XGrad2A = q.lumped ? 0.0 : E0N1 * FDX1 + E0N2 * FDX2 + E0N3 * FDX3;
-  const double E0YGrad2A = q.lumped ? 0.0 : E0N1 * FDY1 + E0N2 * FDY2 + E0N3 * FDY3;
-  const double E1XGrad2A = E1N1 * FDX1 + E1N2 * FDX2 + E1N3 * FDX3;
-  const double E1YGrad2A = E1N1 * FDY1 + E1N2 * FDY2 + E1N3 * FDY3;
-  const double Tau0XGrad2A = 1.0 * (T0N1 * FDX1 + T0N2 * FDX2 + T0N3 * FDX3);
-  const double Tau0YGrad2A = 1.0 * (T0N1 * FDY1 + T0N2 * FDY2 + T0N3 * FDY3);
-  const double EVMXGrad = (EVMN1 * FDX1 + EVMN2 * FDX2 + EVMN3 * FDX3) / AreaIE2;
-  const double EVMYGrad = (EVMN1 * FDY1 + EVMN2 * FDY2 + EVMN3 * FDY3) / AreaIE2;
-  const double EVMAvgODT = ((EVMN1 + EVMN2 + EVMN3) / 3.0) / DT;
-  const double MX = (EVMXGrad * (QX1N1 * FDX1 + QX1N2 * FDX2 + QX1N3 * FDX3) +
-                     EVMYGrad * (QY1N1 * FDX1 + QY1N2 * FDX2 + QY1N3 * FDX3) -
-                     EVMAvgODT * (ESN1 * FDX1 + ESN2 * FDX2 + ESN3 * FDX3)) / AreaIE2;
-  const double MY = (EVMXGrad * (QX1N1 * FDY1 + QX1N2 * FDY2 + QX1N3 * FDY3) +
-                     EVMYGrad * (QY1N1 * FDY1 + QY1N2 * FDY2 + QY1N3 * FDY3) -
-                     EVMAvgODT * (ESN1 * FDY1 + ESN2 * FDY2 + ESN3 * FDY3)) / AreaIE2;
-  const double CorifAvg = (m.corif[n1] + m.corif[n2] + m.corif[n3]) / 3.0;
-  const double Tau0Avg = (T0N1 + T0N2 + T0N3) / 3.0;
-  const double Tau0QXAvg = (T0N1 * QX1N1 + T0N2 * QX1N2 + T0N3 * QX1N3) / 3.0;
-  const double Tau0QYAvg = (T0N1 * QY1N1 + T0N2 * QY1N2 + T0N3 * QY1N3) / 3.0;
-  const double U1Avg = (U1N1 + U1N2 + U1N3) / 3.0;
-  const double V1Avg = (V1N1 + V1N2 + V1N3) / 3.0;
-  const double QX1Avg = (QX1N1 + QX1N2 + QX1N3) / 3.0;
-  const double QY1Avg = (QY1N1 + QY1N2 + QY1N3) / 3.0;
-  const double ESAvg = (ESN1 + ESN2 + ESN3) / 3.0;
-  const double DPAvg = (m.dp[n1] + m.dp[n2] + m.dp[n3]) / 3.0;
-  const double al = s.alphal[e];
-  const double GDP = al * G * DPAvg * 1.0 / AreaIE4;   // both time levels (IFNL_HDP = 0)
-  const double HAvg = (H1N1 + H1N2 + H1N3) / 3.0;
-  const double GHAvgOAreaIE2 = (al * G * HAvg) / AreaIE2;
-  const double BSXAvg = (BSXN1 + BSXN2 + BSXN3) / 3.0;
-  const double BSYAvg = (BSYN1 + BSYN2 + BSYN3) / 3.0;
-  const double MsFacR = AreaIE * (1.0 / DT - Tau0Avg / 2.0) / DT / 12.0;
-  const double GOAreaIE4 = al * G / AreaIE4;
-  const double Tau0SpaVar = (QX1Avg * Tau0XGrad2A + QY1Avg * Tau0YGrad2A) / 6.0;
-  const double A00pB00 = A00 + B00;
-  double JXAvg = CorifAvg * QY1Avg -
-                 q.ifnlfa * GOAreaIE4 * 1.0 * (E1N1SQ * FDX1 + E1N2SQ * FDX2 + E1N3SQ * FDX3) -
-                 GHAvgOAreaIE2 * (PR1N1 * FDX1 + PR1N2 * FDX2 + PR1N3 * FDX3) +
-                 WSXAvg - BSXAvg + MX + Tau0QXAvg;
-  double JYAvg = -CorifAvg * QX1Avg -
-                 q.ifnlfa * GOAreaIE4 * 1.0 * (E1N1SQ * FDY1 + E1N2SQ * FDY2 + E1N3SQ * FDY3) -
-                 GHAvgOAreaIE2 * (PR1N1 * FDY1 + PR1N2 * FDY2 + PR1N3 * FDY3) +
-                 WSYAvg - BSYAvg + MY + Tau0QYAvg;
-  JXAvg = JXAvg -
-          q.ifnlct * (QX1Avg * (U1N1 * FDX1 + U1N2 * FDX2 + U1N3 * FDX3) +
-                      QY1Avg * (U1N1 * FDY1 + U1N2 * FDY2 + U1N3 * FDY3)) / AreaIE2 +
-          q.ifnlcat * U1Avg * ESAvg / DT;
-  JYAvg = JYAvg -
-          q.ifnlct * (QX1Avg * (V1N1 * FDX1 + V1N2 * FDX2 + V1N3 * FDX3) +
-                      QY1Avg * (V1N1 * FDY1 + V1N2 * FDY2 + V1N3 * FDY3)) / AreaIE2 +
-          q.ifnlcat * V1Avg * ESAvg / DT;
-  const double A1 = (q.ondiag * ESN1 * 1.0 + q.offdiag * (ESN2 * 1.0 + ESN3 * 1.0)) * MsFacR -
-                    GDP * (C00 * (1.0 * FDX1 * E0XGrad2A + 1.0 * FDY1 * E0YGrad2A)) -
-                    GDP * (A00pB00 * (1.0 * FDX1 * E1XGrad2A + 1.0 * FDY1 * E1YGrad2A)) +
-                    (1.0 * JXAvg * FDX1 + 1.0 * JYAvg * FDY1) * 0.5 + Tau0SpaVar;
-  const double A2 = (q.ondiag * ESN2 * 1.0 + q.offdiag * (ESN1 * 1.0 + ESN3 * 1.0)) * MsFacR -
-                    GDP * (C00 * (1.0 * FDX2 * E0XGrad2A + 1.0 * FDY2 * E0YGrad2A)) -
-                    GDP * (A00pB00 * (1.0 * FDX2 * E1XGrad2A + 1.0 * FDY2 * E1YGrad2A)) +
-                    (1.0 * JXAvg * FDX2 + 1.0 * JYAvg * FDY2) * 0.5 + Tau0SpaVar;
-  const double A3 = (q.ondiag * ESN3 * 1.0 + q.offdiag * (ESN1 * 1.0 + ESN2 * 1.0)) * MsFacR -
-                    GDP * (C00 * (1.0 * FDX3 * E0XGrad2A + 1.0 * FDY3 * E0YGrad2A)) -
-                    GDP * (A00pB00 * (1.0 * FDX3 * E1XGrad2A + 1.0 * FDY3 * E1YGrad2A)) +
-                    (1.0 * JXAvg * FDX3 + 1.0 * JYAvg * FDY3) * 0.5 + Tau0SpaVar;
   const double nce = (double)ncele;
-  contrib[m.slot[3 * e]] = A1 * nce;
-  contrib[m.slot[3 * e + 1]] = A2 * nce;
-  contrib[m.slot[3 * e + 2]] = A3 * nce;
+  contrib[m.slot[3 * e]] = R1 * nce;
+  contrib[m.slot[3 * e + 1]] = R2 * nce;
+  contrib[m.slot[3 * e + 2]] = R3 * nce;
 }
 
 // Node gather of GWCE_LV, the time-level shift of ETA and the elevation
@@ -1079,63 +1067,60 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
   const double THIRD = 1.0 / 3.0, DT = m.dt, GO2 = m.g / 2.0;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
-  if (ncele == 0) {
-    // dry element: every contribution is NCEle*DT x (...) = 0 (see k_rhs_elem)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int sl = m.slot[3 * e + k];
-      cx[sl] = 0.0;
-      cy[sl] = 0.0;
+  // dry element: every contribution is NCEle*DT x (...) = 0 (see k_rhs_elem)
+  double tx[3] = {0.0, 0.0, 0.0}, ty[3] = {0.0, 0.0, 0.0};
+  if (ncele != 0) {
+    const double U1N1 = s.uu1[n1], U1N2 = s.uu1[n2], U1N3 = s.uu1[n3];
+    const double V1N1 = s.vv1[n1], V1N2 = s.vv1[n2], V1N3 = s.vv1[n3];
+    const double H1N1 = s.h1[n1], H1N2 = s.h1[n2], H1N3 = s.h1[n3];
+    const double AreaIE2 = m.areas[e];
+    const double AreaIE = 0.5 * AreaIE2;
+    const double FDX1 = m.fdx[3 * e], FDX2 = m.fdx[3 * e + 1], FDX3 = m.fdx[3 * e + 2];
+    const double FDY1 = m.fdy[3 * e], FDY2 = m.fdy[3 * e + 1], FDY3 = m.fdy[3 * e + 2];
+    const double U1Avg = THIRD * (U1N1 + U1N2 + U1N3);
+    const double V1Avg = THIRD * (V1N1 + V1N2 + V1N3);
+    const double DU1DXA = (U1N1 * FDX1 + U1N2 * FDX2 + U1N3 * FDX3) * 0.5;
+    const double DU1DYA = (U1N1 * FDY1 + U1N2 * FDY2 + U1N3 * FDY3) * 0.5;
+    const double DV1DXA = (V1N1 * FDX1 + V1N2 * FDX2 + V1N3 * FDX3) * 0.5;
+    const double DV1DYA = (V1N1 * FDY1 + V1N2 * FDY2 + V1N3 * FDY3) * 0.5;
+    const double EVMEle = THIRD * (m.evm[n1] + m.evm[n2] + m.evm[n3]);
+    double B1N1 = al * s.eta1[n1], B2N1 = al * s.eta2[n1];
+    double B1N2 = al * s.eta1[n2], B2N2 = al * s.eta2[n2];
+    double B1N3 = al * s.eta1[n3], B2N3 = al * s.eta2[n3];
+    if (q.nws) {
+      B1N1 = B1N1 + s.pr1[n1]; B2N1 = B2N1 + s.pr2[n1];
+      B1N2 = B1N2 + s.pr1[n2]; B2N2 = B2N2 + s.pr2[n2];
+      B1N3 = B1N3 + s.pr1[n3]; B2N3 = B2N3 + s.pr2[n3];
     }
-    return;
+    const double DBTPDX1A = B1N1 * FDX1 + B1N2 * FDX2 + B1N3 * FDX3;
+    const double DBTPDX2A = B2N1 * FDX1 + B2N2 * FDX2 + B2N3 * FDX3;
+    const double DBTPDY1A = B1N1 * FDY1 + B1N2 * FDY2 + B1N3 * FDY3;
+    const double DBTPDY2A = B2N1 * FDY1 + B2N2 * FDY2 + B2N3 * FDY3;
+    const double DBTPDXA = 0.5 * (1.0 * DBTPDX1A + 1.0 * DBTPDX2A);
+    const double DBTPDYA = 0.5 * (1.0 * DBTPDY1A + 1.0 * DBTPDY2A);
+    const double UDU = U1Avg * DU1DXA, VDU = V1Avg * DU1DYA;
+    const double UDV = U1Avg * DV1DXA, VDV = V1Avg * DV1DYA;
+    const double DU1DX = DU1DXA / AreaIE, DU1DY = DU1DYA / AreaIE;
+    const double DV1DX = DV1DXA / AreaIE, DV1DY = DV1DYA / AreaIE;
+    const double NCDT = (double)ncele * DT;
+    const double advx = -q.ifnlct * (UDU + VDU) - GO2 * DBTPDXA - 0.0;
+    const double advy = -q.ifnlct * (UDV + VDV) - GO2 * DBTPDYA - 0.0;
+    const double Hs[3] = {H1N1, H1N2, H1N3};
+    const double FX[3] = {FDX1, FDX2, FDX3}, FY[3] = {FDY1, FDY2, FDY3};
+  #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double eh = EVMEle * Hs[k];
+      const double lxx = eh * DU1DX, lxy = eh * DU1DY, lyx = eh * DV1DX, lyy = eh * DV1DY;
+      // TANPHI = 0: the AreaIE*TANPHI*(...) terms are signed zeros
+      tx[k] = NCDT * (advx - (1.5 * (lxx * FX[k] + lxy * FY[k]) - AreaIE * 0.0 * (lxy + lyx)) / Hs[k]);
+      ty[k] = NCDT * (advy - (1.5 * (lyx * FX[k] + lyy * FY[k]) + AreaIE * 0.0 * (lxx - lyy)) / Hs[k]);
+    }
   }
-  const double U1N1 = s.uu1[n1], U1N2 = s.uu1[n2], U1N3 = s.uu1[n3];
-  const double V1N1 = s.vv1[n1], V1N2 = s.vv1[n2], V1N3 = s.vv1[n3];
-  const double H1N1 = s.h1[n1], H1N2 = s.h1[n2], H1N3 = s.h1[n3];
-  const double AreaIE2 = m.areas[e];
-  const double AreaIE = 0.5 * AreaIE2;
-  const double FDX1 = m.fdx[3 * e], FDX2 = m.fdx[3 * e + 1], FDX3 = m.fdx[3 * e + 2];
-  const double FDY1 = m.fdy[3 * e], FDY2 = m.fdy[3 * e + 1], FDY3 = m.fdy[3 * e + 2];
-  const double U1Avg = THIRD * (U1N1 + U1N2 + U1N3);
-  const double V1Avg = THIRD * (V1N1 + V1N2 + V1N3);
-  const double DU1DXA = (U1N1 * FDX1 + U1N2 * FDX2 + U1N3 * FDX3) * 0.5;
-  const double DU1DYA = (U1N1 * FDY1 + U1N2 * FDY2 + U1N3 * FDY3) * 0.5;
-  const double DV1DXA = (V1N1 * FDX1 + V1N2 * FDX2 + V1N3 * FDX3) * 0.5;
-  const double DV1DYA = (V1N1 * FDY1 + V1N2 * FDY2 + V1N3 * FDY3) * 0.5;
-  const double EVMEle = THIRD * (m.evm[n1] + m.evm[n2] + m.evm[n3]);
-  double B1N1 = al * s.eta1[n1], B2N1 = al * s.eta2[n1];
-  double B1N2 = al * s.eta1[n2], B2N2 = al * s.eta2[n2];
-  double B1N3 = al * s.eta1[n3], B2N3 = al * s.eta2[n3];
-  if (q.nws) {
-    B1N1 = B1N1 + s.pr1[n1]; B2N1 = B2N1 + s.pr2[n1];
-    B1N2 = B1N2 + s.pr1[n2]; B2N2 = B2N2 + s.pr2[n2];
-    B1N3 = B1N3 + s.pr1[n3]; B2N3 = B2N3 + s.pr2[n3];
-  }
-  const double DBTPDX1A = B1N1 * FDX1 + B1N2 * FDX2 + B1N3 * FDX3;
-  const double DBTPDX2A = B2N1 * FDX1 + B2N2 * FDX2 + B2N3 * FDX3;
-  const double DBTPDY1A = B1N1 * FDY1 + B1N2 * FDY2 + B1N3 * FDY3;
-  const double DBTPDY2A = B2N1 * FDY1 + B2N2 * FDY2 + B2N3 * FDY3;
-  const double DBTPDXA = 0.5 * (1.0 * DBTPDX1A + 1.0 * DBTPDX2A);
-  const double DBTPDYA = 0.5 * (1.0 * DBTPDY1A + 1.0 * DBTPDY2A);
-  const double UDU = U1Avg * DU1DXA, VDU = V1Avg * DU1DYA;
-  const double UDV = U1Avg * DV1DXA, VDV = V1Avg * DV1DYA;
-  const double DU1DX = DU1DXA / AreaIE, DU1DY = DU1DYA / AreaIE;
-  const double DV1DX = DV1DXA / AreaIE, DV1DY = DV1DYA / AreaIE;
-  const double NCDT = (double)ncele * DT;
-  const double advx = -q.ifnlct * (UDU + VDU) - GO2 * DBTPDXA - 0.0;
-  const double advy = -q.ifnlct * (UDV + VDV) - GO2 * DBTPDYA - 0.0;
-  const double Hs[3] = {H1N1, H1N2, H1N3};
-  const double FX[3] = {FDX1, FDX2, FDX3}, FY[3] = {FDY1, FDY2, FDY3};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double eh = EVMEle * Hs[k];
-    const double lxx = eh * DU1DX, lxy = eh * DU1DY, lyx = eh * DV1DX, lyy = eh * DV1DY;
-    // TANPHI = 0: the AreaIE*TANPHI*(...) terms are signed zeros
-    const double ta = NCDT * (advx - (1.5 * (lxx * FX[k] + lxy * FY[k]) - AreaIE * 0.0 * (lxy + lyx)) / Hs[k]);
-    const double tb = NCDT * (advy - (1.5 * (lyx * FX[k] + lyy * FY[k]) + AreaIE * 0.0 * (lxx - lyy)) / Hs[k]);
     const int sl = m.slot[3 * e + k];
-    cx[sl] = ta;
-    cy[sl] = tb;
+    cx[sl] = tx[k];
+    cy[sl] = ty[k];
   }
 }
 
