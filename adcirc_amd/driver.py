@@ -8,9 +8,12 @@ The steps between two output times run as one resident launch sequence
 (``adcgpu_run``): state stays on the device, and only the output steps copy
 ETA2, UU2, VV2 and NODECODE back.
 
+Elevation and velocity stations (NOUTE/NOUTV with NSTAE/NSTAV stations) are
+written to fort.61 and fort.62 at their own output steps.
+
 Not handled, and refused with an error: hot starts (IHOT != 0, NHSTAR != 0),
-station output (NSTAE/NSTAV/NSTAM > 0 with output on), harmonic analysis
-(NFREQ > 0), output formats other than full ASCII (NOUTGE/NOUTGV other than
+met-station output (NOUTM with NSTAM > 0), harmonic analysis (NFREQ > 0),
+output formats other than full ASCII (NOUTGE/NOUTGV/NOUTE/NOUTV other than
 0, 1, -1).  fort.22 is not read: with NWS = 8 the caller passes the
 parametric vortex's parameters (the 13 values ``Model.holland`` takes).
 
@@ -37,13 +40,12 @@ def _check(f: fort15.Fort15, holland) -> None:
         bad.append(f"IHOT = {f.ihot} (hot start)")
     if f.nhstar != 0:
         bad.append(f"NHSTAR = {f.nhstar} (hot-start output)")
-    for o, st, name in ((f.elev_sta, f.stations_e, "NOUTE"), (f.vel_sta, f.stations_v, "NOUTV"),
-                        (f.met_sta, f.stations_m, "NOUTM")):
-        if o.nout != 0 and st.shape[0] > 0:
-            bad.append(f"{name} = {o.nout} with stations (station output)")
+    if f.met_sta.nout != 0 and f.stations_m.shape[0] > 0:
+        bad.append(f"NOUTM = {f.met_sta.nout} with stations (met-station output)")
     if f.nfreq != 0:
         bad.append(f"NFREQ = {f.nfreq} (harmonic analysis)")
-    for o, name in ((f.elev, "NOUTGE"), (f.vel, "NOUTGV")):
+    for o, name in ((f.elev, "NOUTGE"), (f.vel, "NOUTGV"), (f.elev_sta, "NOUTE"),
+                    (f.vel_sta, "NOUTV")):
         if o.nout not in (0, 1, -1):
             bad.append(f"{name} = {o.nout} (only full ASCII output, 1 or -1)")
     if f.nws != 0 and holland is None:
@@ -89,8 +91,21 @@ def run_directory(case_dir: str, out_dir: Optional[str] = None,
             f64 = outputs.GlobalFile(p, f.rundes, f.runid, agrid, f.vel.datasets(run), npn,
                                      run.dtdp, f.vel.nspool, 2)
             files.append(p)
+        sta = []  # (output control, file, interpolation, fields)
+        for o, xy, name, rtype, fields in ((f.elev_sta, f.stations_e, "fort.61", 1, ("eta2",)),
+                                           (f.vel_sta, f.stations_v, "fort.62", 2,
+                                            ("uu2", "vv2"))):
+            if o.nout and xy.shape[0] > 0:
+                el, fac = outputs.station_factors(tables["x"], tables["y"], tables["nm"],
+                                                  tables["areas"], xy[:, 0], xy[:, 1])
+                p = os.path.join(out_dir, name)
+                sf = outputs.StationFile(p, f.rundes, f.runid, agrid, o.datasets(run),
+                                         xy.shape[0], run.dtdp, o.nspool, rtype)
+                sta.append((o, sf, el, fac, fields))
+                files.append(p)
         stops = sorted({it for it in range(1, nt + 1)
-                        if f.elev.due(run, it) or f.vel.due(run, it)} | {nt})
+                        if f.elev.due(run, it) or f.vel.due(run, it)
+                        or any(o.due(run, it) for o, *_ in sta)} | {nt})
         t0 = time.time()
         it, iters = 0, 0
         for stop in stops:
@@ -102,6 +117,11 @@ def run_directory(case_dir: str, out_dir: Optional[str] = None,
                 f63.write(timeloc, it, g.get("nodecode"), g.get("eta2"))
             if f64 is not None and f.vel.due(run, it):
                 f64.write(timeloc, it, g.get("nodecode"), g.get("uu2"), g.get("vv2"))
+            for o, sf, el, fac, fields in sta:
+                if o.due(run, it):
+                    nc = g.get("nodecode")
+                    sf.write_values(timeloc, it, *(outputs.station_values(
+                        g.get(k), nc, tables["nm"], el, fac) for k in fields))
         wall = time.time() - t0
         if f.elev.nout:
             p = os.path.join(out_dir, "maxele.63")

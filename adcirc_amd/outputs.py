@@ -1,6 +1,6 @@
-"""Full-domain ASCII output files in the reference's formats: fort.63
-(elevation), fort.64 (depth-averaged velocity) and maxele.63 (peak elevation
-and its time).
+"""ASCII output files in the reference's formats: fort.63 (elevation),
+fort.64 (depth-averaged velocity), maxele.63 (peak elevation and its time),
+and the station files fort.61 (elevation) and fort.62 (velocity).
 
 Formats follow the reference's serial writers:
 
@@ -19,6 +19,13 @@ Formats follow the reference's serial writers:
 maxele.63 holds two records: ETAMAX, then ETAMAX_Time, the time of each
 node's maximum (-99999.0 where it never rose; src/write_output.F:1220-1237,
 4330-4400).
+
+Station files have the same layout with an ``I5`` NSPOOL in header line 2
+(src/globalio.F:963-1003) and one line per station.  A station's value is
+the linear interpolation d1*f1 + d2*f2 + d3*f3 over the element that holds
+it, with the reference's interpolation factors
+(src/read_input.F:6246-6277), and -99999.0 when a node of that element is
+dry (src/write_output.F:2000-2060, StatPartWetFix off).
 """
 from __future__ import annotations
 
@@ -61,10 +68,12 @@ def e20(v: float) -> str:
 
 
 def header(rundes: str, runid: str, agrid: str, nsets: int, np_: int, dt_spool: float,
-           nspool: int, rtype: int) -> str:
+           nspool: int, rtype: int, spool_width: int = 8) -> str:
+    """Header lines 1-2; ``spool_width`` is 8 for the global files, 5 for
+    the station files."""
     l1 = " " + rundes[:32].ljust(32) + "  " + runid[:24].ljust(24) + "  " + agrid[:24].ljust(24)
-    l2 = (f" {nsets:10d} {np_:10d} {_fmt_e(dt_spool, 15, 7, 3)} {nspool:8d} {rtype:5d} "
-          f"FileFmtVersion: {FILE_FMT_VERSION:10d}")
+    l2 = (f" {nsets:10d} {np_:10d} {_fmt_e(dt_spool, 15, 7, 3)} {nspool:{spool_width}d} "
+          f"{rtype:5d} FileFmtVersion: {FILE_FMT_VERSION:10d}")
     return l1 + "\n" + l2 + "\n"
 
 
@@ -85,10 +94,11 @@ class GlobalFile:
     to ``write``."""
 
     def __init__(self, path: str, rundes: str, runid: str, agrid: str, nsets: int, np_: int,
-                 dtdp: float, nspool: int, rtype: int):
+                 dtdp: float, nspool: int, rtype: int, spool_width: int = 8):
         self.path, self.rtype = path, rtype
         with open(path, "w") as f:
-            f.write(header(rundes, runid, agrid, nsets, np_, dtdp * nspool, nspool, rtype))
+            f.write(header(rundes, runid, agrid, nsets, np_, dtdp * nspool, nspool, rtype,
+                           spool_width))
 
     def write(self, timeloc: float, it: int, nodecode: np.ndarray, *cols: np.ndarray) -> None:
         dry = np.asarray(nodecode) == 0
@@ -133,3 +143,59 @@ def etamax_time(etamax_it: np.ndarray, dtdp: float, statim: float) -> np.ndarray
     it = np.asarray(etamax_it, np.int64)
     t = it * dtdp + statim * 86400.0
     return np.where(it > 0, t, DRY_VALUE)
+
+
+def station_factors(x: np.ndarray, y: np.ndarray, nm: np.ndarray, areas: np.ndarray,
+                    xs: np.ndarray, ys: np.ndarray):
+    """For each station (xs, ys): the 0-based element that holds it (-1 if
+    none) and the reference's interpolation factors f1, f2, f3
+    (ComputeInterpolatingFactors, src/read_input.F:6246-6277; AREAS is twice
+    the element area; NM 1-based).  Of the elements that hold a station on a
+    shared edge or vertex, the one with the largest smallest factor is taken."""
+    nm0 = np.asarray(nm, np.int64).reshape(-1, 3) - 1
+    X1, X2, X3 = (x[nm0[:, k]] for k in range(3))
+    Y1, Y2, Y3 = (y[nm0[:, k]] for k in range(3))
+    ns = len(xs)
+    el = np.full(ns, -1, np.int64)
+    f = np.zeros((ns, 3))
+    for j in range(ns):
+        X4, Y4 = float(xs[j]), float(ys[j])
+        f1 = ((X4 - X3) * (Y2 - Y3) + (X2 - X3) * (Y3 - Y4)) / areas
+        f2 = ((X4 - X1) * (Y3 - Y1) - (Y4 - Y1) * (X3 - X1)) / areas
+        f3 = (-(X4 - X1) * (Y2 - Y1) + (Y4 - Y1) * (X2 - X1)) / areas
+        worst = np.minimum(np.minimum(f1, f2), f3)
+        e = int(np.argmax(worst))
+        if worst[e] >= -1.0e-9:
+            el[j] = e
+            f[j] = (f1[e], f2[e], f3[e])
+    return el, f
+
+
+def station_values(vals: np.ndarray, nodecode: np.ndarray, nm: np.ndarray, el: np.ndarray,
+                   f: np.ndarray) -> np.ndarray:
+    """d1*f1 + d2*f2 + d3*f3 at each station, -99999.0 where a node of its
+    element is dry or the station lies outside the mesh (NM 1-based)."""
+    nm0 = np.asarray(nm, np.int64).reshape(-1, 3) - 1
+    out = np.full(el.shape[0], DRY_VALUE)
+    for j, e in enumerate(el):
+        if e < 0:
+            continue
+        n1, n2, n3 = nm0[e]
+        if nodecode[n1] * nodecode[n2] * nodecode[n3] == 0:
+            continue
+        out[j] = vals[n1] * f[j, 0] + vals[n2] * f[j, 1] + vals[n3] * f[j, 2]
+    return out
+
+
+class StationFile(GlobalFile):
+    """fort.61 (rtype 1) or fort.62 (rtype 2): one line per station."""
+
+    def __init__(self, path: str, rundes: str, runid: str, agrid: str, nsets: int, nsta: int,
+                 dtdp: float, nspool: int, rtype: int):
+        super().__init__(path, rundes, runid, agrid, nsets, nsta, dtdp, nspool, rtype,
+                         spool_width=5)
+
+    def write_values(self, timeloc: float, it: int, *cols: np.ndarray) -> None:
+        with open(self.path, "a") as f:
+            f.write(record_header(timeloc, it))
+            f.write(node_lines(*cols))

@@ -14,11 +14,15 @@ from adcirc_amd.model import Model, cold_wetdry_state
 pytestmark = pytest.mark.gpu
 
 
-def _write_case(d, case, nsteps, elev, vel):
+def _write_case(d, case, nsteps, elev, vel, sta=None, stations=None):
     case.run.nsteps = nsteps
     os.makedirs(d, exist_ok=True)
     fort14.write_fort14(os.path.join(d, "fort.14"), case.mesh)
-    fort15.write_fort15(os.path.join(d, "fort.15"), fort15.from_case(case, elev, vel))
+    f = fort15.from_case(case, elev, vel)
+    if sta is not None:
+        f.elev_sta, f.stations_e = sta, stations
+        f.vel_sta, f.stations_v = sta, stations
+    fort15.write_fort15(os.path.join(d, "fort.15"), f)
 
 
 def _per_step_model(d, holland):
@@ -52,7 +56,14 @@ def test_driver_outputs_equal_per_step_state(tmp_path, name):
         case, n = meshgen.beach(), 900
         hol = case.run.holland
         elev, vel = fort15.Output(1, 0.0, 1.0, 150), fort15.Output(1, 0.0, 1.0, 450)
-    _write_case(d, case, n, elev, vel)
+    # stations: inside the mesh, one near the shore (wets and dries on the
+    # beach), one outside
+    x, y = case.mesh.x, case.mesh.y
+    stations = np.array([[0.5 * (x.min() + x.max()), 0.5 * (y.min() + y.max())],
+                         [x.min() + 0.1 * (x.max() - x.min()), 0.37 * (y.max() - y.min())],
+                         [x.max() + 1.0e4, y.max()]])
+    sta = fort15.Output(1, 0.0, 1.0, 100)
+    _write_case(d, case, n, elev, vel, sta, stations)
     log(f"case {name}: driver over {n} steps")
     res = driver.run_directory(d, holland=hol, log=log)
     assert res["steps"] == n
@@ -62,6 +73,16 @@ def test_driver_outputs_equal_per_step_state(tmp_path, name):
     assert [r[1] for r in e63] == list(range(elev.nspool, n + 1, elev.nspool))
     assert [r[1] for r in e64] == list(range(vel.nspool, n + 1, vel.nspool))
     assert h64["rtype"] == 2 and hm["nsets"] == 2
+    _, e61 = outputs.read_global(os.path.join(d, "fort.61"))
+    _, e62 = outputs.read_global(os.path.join(d, "fort.62"))
+    assert [r[1] for r in e61] == list(range(100, n + 1, 100))
+    want61 = {it: v for _, it, v in e61}
+    want62 = {it: v for _, it, v in e62}
+    tabs = prep.build_tables(fort14.read_fort14(os.path.join(d, "fort.14")), case.run.nolifa,
+                             case.run.h0, case.run.anginn)
+    el, fac = outputs.station_factors(tabs["x"], tabs["y"], tabs["nm"], tabs["areas"],
+                                      stations[:, 0], stations[:, 1])
+    assert el[-1] == -1 and (el[:-1] >= 0).all()
     g, run = _per_step_model(d, hol)
     want63 = {it: v for _, it, v in e63}
     want64 = {it: v for _, it, v in e64}
@@ -79,6 +100,12 @@ def test_driver_outputs_equal_per_step_state(tmp_path, name):
         if it in want64:
             assert np.array_equal(want64[it][:, 0], _expect(g.get("uu2"), nc)), it
             assert np.array_equal(want64[it][:, 1], _expect(g.get("vv2"), nc)), it
+        if it in want61:
+            sv = outputs.station_values(g.get("eta2"), nc, tabs["nm"], el, fac)
+            assert np.array_equal(want61[it][:, 0], [float(outputs.e20(v)) for v in sv]), it
+            for c, k in enumerate(("uu2", "vv2")):
+                sv = outputs.station_values(g.get(k), nc, tabs["nm"], el, fac)
+                assert np.array_equal(want62[it][:, c], [float(outputs.e20(v)) for v in sv]), it
     tmax = outputs.etamax_time(g.get("etamax_it"), run.dtdp, run.statim)
     assert np.array_equal(emx[0][2][:, 0], [float(outputs.e20(v)) for v in g.get("etamax")])
     assert np.array_equal(emx[1][2][:, 0], [float(outputs.e20(v)) for v in tmax])

@@ -196,15 +196,17 @@ def test_global_file_and_maxele(tmp_path):
 
 
 def test_driver_refuses_what_it_does_not_write(tmp_path):
-    """The case driver refuses hot starts, station output, harmonic analysis,
-    non-ASCII global output and NWS = 8 without vortex parameters, before it
-    touches the GPU."""
+    """The case driver refuses hot starts, met-station output, harmonic
+    analysis, non-ASCII output and NWS = 8 without vortex parameters, before
+    it touches the GPU."""
     from adcirc_amd import driver
     base = fort15.from_case(meshgen.beach(), fort15.Output(1, 0.0, 1.0, 60), fort15.Output())
     cases = {
         "IHOT": dataclasses.replace(base, ihot=67),
         "NHSTAR": dataclasses.replace(base, nhstar=1, nhsinc=100),
-        "NOUTE": dataclasses.replace(base, elev_sta=fort15.Output(1, 0.0, 1.0, 10),
+        "NOUTM": dataclasses.replace(base, met_sta=fort15.Output(1, 0.0, 1.0, 10),
+                                     stations_m=np.array([[100.0, 200.0]])),
+        "NOUTE": dataclasses.replace(base, elev_sta=fort15.Output(2, 0.0, 1.0, 10),
                                      stations_e=np.array([[100.0, 200.0]])),
         "NFREQ": dataclasses.replace(base, nfreq=1),
         "NOUTGE": dataclasses.replace(base, elev=fort15.Output(3, 0.0, 1.0, 60)),
@@ -214,3 +216,30 @@ def test_driver_refuses_what_it_does_not_write(tmp_path):
         with pytest.raises(fort15.Fort15Error, match=word):
             driver._check(f, None)
     driver._check(base, meshgen.beach().run.holland)  # accepted
+
+
+def test_station_interpolation_reproduces_linear_fields():
+    """The reference's interpolation factors (src/read_input.F:6246-6277)
+    reproduce a linear field exactly at any point of the element that holds
+    it, sum to one, and are 1/0/0 at a vertex; a point outside the mesh gets
+    -99999.  The beach mesh, with its own AREAS."""
+    from adcirc_amd import prep
+    case = meshgen.beach()
+    t = prep.build_tables(case.mesh, case.run.nolifa, case.run.h0, case.run.anginn)
+    x, y, nm = t["x"], t["y"], np.asarray(t["nm"]).reshape(-1, 3)
+    rng = np.random.default_rng(2)
+    xs = rng.uniform(x.min(), x.max(), 25)
+    ys = rng.uniform(y.min(), y.max(), 25)
+    xs = np.append(xs, [x[nm[7, 1] - 1], x.max() + 1.0e4])
+    ys = np.append(ys, [y[nm[7, 1] - 1], y.max() + 1.0e4])
+    el, f = outputs.station_factors(x, y, nm, t["areas"], xs, ys)
+    assert (el[:-1] >= 0).all() and el[-1] == -1
+    assert np.allclose(f[:-1].sum(axis=1), 1.0, atol=1e-12)
+    assert np.allclose(sorted(f[-2]), [0.0, 0.0, 1.0], atol=1e-12)
+    lin = 0.3 + 2.0e-4 * x - 5.0e-5 * y
+    v = outputs.station_values(lin, np.ones(x.shape[0], np.int32), nm, el, f)
+    assert np.allclose(v[:-1], 0.3 + 2.0e-4 * xs[:-1] - 5.0e-5 * ys[:-1], rtol=0, atol=1e-12)
+    assert v[-1] == -99999.0
+    dry = np.ones(x.shape[0], np.int32)
+    dry[nm[el[0], 0] - 1] = 0
+    assert outputs.station_values(lin, dry, nm, el, f)[0] == -99999.0
