@@ -17,9 +17,9 @@ ip_ = C.POINTER(C.c_int32)
 FP64, MIXED = 0, 1
 FIELDS = dict(eta1=0, eta2=1, etas=2, uu1=3, vv1=4, uu2=5, vv2=6, qx1=7, qy1=8, qx2=9, qy2=10,
               h1=11, h2=12, tk=13, etamax=14, totalarea=15, wsx1=16, wsy1=17, wsx2=18,
-              wsy2=19, pr1=20, pr2=21, gwce_lv=22, nodecode=100, noff=101, mju=102, etamax_it=103,
+              wsy2=19, pr1=20, pr2=21, gwce_lv=22, umax=23, nodecode=100, noff=101, mju=102, etamax_it=103, umax_it=104,
               coef=200, alphal=300)
-INT_FIELDS = ("nodecode", "noff", "mju", "etamax_it")
+INT_FIELDS = ("nodecode", "noff", "mju", "etamax_it", "umax_it")
 
 # every symbol include/adcgpu.h declares
 EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_field",

@@ -69,8 +69,8 @@ struct adcgpu_ctx {
   // state (pointers rotate between time levels)
   double *eta1, *eta2, *etas, *uu1, *vv1, *uu2, *vv2, *qx1, *qy1, *qx2, *qy2, *h1, *h2, *tk;
   double *etamax, *totalarea, *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2, *gwce_lv, *alphal;
-  double *coef, *obc, *fric, *etas_prev = nullptr;
-  int *nodecode, *nnodecode, *noff, *noffold, *mju, *etamax_it;
+  double *coef, *obc, *fric, *etas_prev = nullptr, *umax = nullptr;
+  int *nodecode, *nnodecode, *noff, *noffold, *mju, *etamax_it, *umax_it;
   int cur_it = 0;  // step opened by adcgpu_begin_step
   // work
   double *contrib = nullptr, *cx = nullptr, *cy = nullptr, *ep_part = nullptr, *scal = nullptr;
@@ -114,7 +114,7 @@ struct adcgpu_ctx {
     s.wsx2 = wsx2; s.wsy2 = wsy2; s.pr1 = pr1; s.pr2 = pr2; s.gwce_lv = gwce_lv;
     s.alphal = alphal; s.coef = coef; s.obc = obc; s.fric = fric; s.nodecode = nodecode;
     s.nnodecode = nnodecode; s.noff = noff; s.noffold = noffold; s.mju = mju;
-    s.etamax_it = etamax_it;
+    s.etamax_it = etamax_it; s.umax = umax; s.umax_it = umax_it;
     return s;
   }
 };
@@ -448,6 +448,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   A(&c->obc, (size_t)std::max(neta, 1) * mnei);
   A(&c->nodecode, np); A(&c->nnodecode, np); A(&c->noff, ne); A(&c->noffold, ne);
   A(&c->mju, np); A(&c->etamax_it, np); A(&c->etas_prev, np);
+  A(&c->umax, np); A(&c->umax_it, np);
   A(&c->contrib, 3 * (size_t)ne); A(&c->cx, 3 * (size_t)ne); A(&c->cy, 3 * (size_t)ne);
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
   {
@@ -492,6 +493,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     (void)hipMemcpy(c->pr1, pr.data(), np * sizeof(double), hipMemcpyHostToDevice);
     (void)hipMemcpy(c->pr2, pr.data(), np * sizeof(double), hipMemcpyHostToDevice);
     (void)hipMemcpy(c->etamax, emax.data(), np * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->umax, emax.data(), np * sizeof(double), hipMemcpyHostToDevice);
     std::vector<int> one_i(std::max(np, ne), 1);
     (void)hipMemcpy(c->nodecode, one_i.data(), np * sizeof(int), hipMemcpyHostToDevice);
     (void)hipMemcpy(c->noff, one_i.data(), ne * sizeof(int), hipMemcpyHostToDevice);
@@ -548,6 +550,7 @@ static double** field_ptr(adcgpu_ctx* c, int f) {
     case ADCGPU_PR1: return &c->pr1;
     case ADCGPU_PR2: return &c->pr2;
     case ADCGPU_GWCE_LV: return &c->gwce_lv;
+    case ADCGPU_UMAX: return &c->umax;
     case ADCGPU_COEF: return &c->coef;
     case ADCGPU_ALPHAL: return &c->alphal;
   }
@@ -560,6 +563,7 @@ static int** ifield_ptr(adcgpu_ctx* c, int f) {
     case ADCGPU_NOFF: return &c->noff;
     case ADCGPU_MJU: return &c->mju;
     case ADCGPU_ETAMAX_IT: return &c->etamax_it;
+    case ADCGPU_UMAX_IT: return &c->umax_it;
   }
   return nullptr;
 }

@@ -88,6 +88,8 @@ struct DevState {
   double *gwce_lv, *alphal, *coef, *obc, *fric;
   int *nodecode, *nnodecode, *noff, *noffold, *mju;
   int *etamax_it;  // step of the last ETAMAX rise (0: never)
+  double *umax;    // peak depth-averaged speed at wet nodes
+  int *umax_it;    // step of the last UMAX rise (0: never)
 };
 
 struct RhsPrm {

@@ -1234,6 +1234,14 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
     s.etamax[i] = e2;
     s.etamax_it[i] = q.it;
   }
+  // peak speed and its step (src/write_output.F:6367-6376)
+  if (nci == 1) {
+    const double ut = __builtin_sqrt(u2 * u2 + v2 * v2);
+    if (ut > s.umax[i]) {
+      s.umax[i] = ut;
+      s.umax_it[i] = q.it;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, unsigned v, int n) {

@@ -2,7 +2,7 @@
 global outputs, the way the reference's serial executable runs the same
 inputs: cold start (src/cstart.F), NT = floor(RNDAY*86400/DTDP) steps of the
 2DDI timestep, fort.63/fort.64 records at the steps the NOUTGE/NOUTGV lines
-select, and maxele.63 at the end.
+select, and maxele.63/maxvel.63 at the end.
 
 The steps between two output times run as one resident launch sequence
 (``adcgpu_run``): state stays on the device, and only the output steps copy
@@ -128,6 +128,12 @@ def run_directory(case_dir: str, out_dir: Optional[str] = None,
             timeloc = nt * run.dtdp + run.statim * 86400.0
             outputs.write_maxele(p, f.rundes, f.runid, agrid, timeloc, nt, g.get("etamax"),
                                  outputs.etamax_time(g.get("etamax_it"), run.dtdp, run.statim))
+            files.append(p)
+        if f.vel.nout:
+            p = os.path.join(out_dir, "maxvel.63")
+            timeloc = nt * run.dtdp + run.statim * 86400.0
+            outputs.write_maxele(p, f.rundes, f.runid, agrid, timeloc, nt, g.get("umax"),
+                                 outputs.etamax_time(g.get("umax_it"), run.dtdp, run.statim))
             files.append(p)
         log(f"{nt} steps in {wall:.2f} s ({nt * run.dtdp / max(wall, 1e-9):.1f} sim-s/wall-s), "
             f"{iters} JCG iterations; wrote {', '.join(os.path.basename(p) for p in files)}")

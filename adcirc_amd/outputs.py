@@ -18,7 +18,7 @@ Formats follow the reference's serial writers:
 
 maxele.63 holds two records: ETAMAX, then ETAMAX_Time, the time of each
 node's maximum (-99999.0 where it never rose; src/write_output.F:1220-1237,
-4330-4400).
+4330-4400); maxvel.63 the same for the peak speed UMAX.
 
 Station files have the same layout with an ``I5`` NSPOOL in header line 2
 (src/globalio.F:963-1003) and one line per station.  A station's value is
@@ -110,6 +110,8 @@ class GlobalFile:
 
 def write_maxele(path: str, rundes: str, runid: str, agrid: str, timeloc: float, it: int,
                  etamax: np.ndarray, etamax_time: np.ndarray) -> None:
+    """maxele.63 (ETAMAX, ETAMAX_Time) or, with UMAX and UMAX_Time, maxvel.63
+    (src/write_output.F:1205-1268): the same two-record layout."""
     np_ = etamax.shape[0]
     with open(path, "w") as f:
         f.write(header(rundes, runid, agrid, 2, np_, 1.0, 1, 1))
@@ -137,7 +139,7 @@ def read_global(path: str):
 
 
 def etamax_time(etamax_it: np.ndarray, dtdp: float, statim: float) -> np.ndarray:
-    """ETAMAX_Time from the step of each node's last ETAMAX rise:
+    """ETAMAX_Time (or UMAX_Time) from the step of each node's last rise:
     TimeLoc = IT*DTDP + STATIM*86400 (src/timestep.F:257), -99999.0 where
     ETAMAX never rose."""
     it = np.asarray(etamax_it, np.int64)

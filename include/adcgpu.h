@@ -92,10 +92,13 @@ enum adcgpu_field {
   ADCGPU_VV2, ADCGPU_QX1, ADCGPU_QY1, ADCGPU_QX2, ADCGPU_QY2, ADCGPU_H1, ADCGPU_H2,
   ADCGPU_TK, ADCGPU_ETAMAX, ADCGPU_TOTALAREA, ADCGPU_WSX1, ADCGPU_WSY1, ADCGPU_WSX2,
   ADCGPU_WSY2, ADCGPU_PR1, ADCGPU_PR2, ADCGPU_GWCE_LV,
+  ADCGPU_UMAX,      /* peak sqrt(UU2^2+VV2^2) at wet nodes, -99999 before any
+                       (src/write_output.F:6367-6376) */
   ADCGPU_NODECODE = 100, ADCGPU_NOFF, ADCGPU_MJU,
   ADCGPU_ETAMAX_IT, /* step IT of each node's last ETAMAX rise, 0 if none: the
                        reference's ETAMAX_Time is IT*DTDP + STATIM*86400
                        (src/write_output.F:6357-6364) */
+  ADCGPU_UMAX_IT,   /* step IT of each node's last UMAX rise, 0 if none (UMAX_Time) */
   ADCGPU_COEF = 200, /* COEF(NP,MNEI) column-major */
   ADCGPU_ALPHAL = 300
 };

@@ -104,9 +104,9 @@ void adco_destroy(adco_model *m);
 int adco_step(adco_model *m, int it, const double *wsx2, const double *wsy2,
               const double *pr2);
 /* Access to state arrays by name ("eta1","eta2","etas","uu2","vv2","qx2",
- * "qy2","tk","etamax","totalarea","coef","gwce_lv","h2","alphal"). */
+ * "qy2","tk","etamax","umax","totalarea","coef","gwce_lv","h2","alphal"). */
 double *adco_array(adco_model *m, const char *name);
-int *adco_iarray(adco_model *m, const char *name); /* nodecode, noff, mju, etamax_it */
+int *adco_iarray(adco_model *m, const char *name); /* nodecode, noff, mju, etamax_it, umax_it */
 int adco_numitr(const adco_model *m);
 int adco_ncchange(const adco_model *m);
 uint64_t adco_state_hash(const adco_model *m);

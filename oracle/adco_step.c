@@ -28,7 +28,8 @@ struct adco_model {
   double *h1, *h2, *tk, *tkm, *gwce_lv, *mom_lv_x, *mom_lv_y;
   double *wsx1, *wsy1, *wsx2, *wsy2, *pr1, *pr2, *etamax, *totalarea, *auv;
   double *tk_tmp;
-  int *nodecode, *nnodecode, *mju, *nibcnt, *etamax_it;
+  int *nodecode, *nnodecode, *mju, *nibcnt, *etamax_it, *umax_it;
+  double *umax;
   /* element state */
   int *noff, *noffold;
   double *alphal;
@@ -84,6 +85,8 @@ adco_model *adco_create(const adco_setup *s, int order) {
   m->nnodecode = ialloc(np);
   m->mju = ialloc(np);
   m->etamax_it = ialloc(np);
+  m->umax_it = ialloc(np);
+  m->umax = dalloc(np);
   m->nibcnt = ialloc(np);
   m->noff = ialloc(ne);
   m->noffold = ialloc(ne);
@@ -125,6 +128,7 @@ adco_model *adco_create(const adco_setup *s, int order) {
     m->pr1[i] = s->prbckgrnd_mh2o;
     m->pr2[i] = s->prbckgrnd_mh2o;
     m->etamax[i] = -99999.0;
+    m->umax[i] = -99999.0;
   }
   for (int ie = 0; ie < ne; ++ie) {
     m->noff[ie] = s->noff0[ie];
@@ -155,7 +159,7 @@ void adco_destroy(adco_model *m) {
                m->vv2, m->qx1, m->qy1, m->qx2, m->qy2, m->h1, m->h2, m->tk, m->gwce_lv,
                m->mom_lv_x, m->mom_lv_y, m->wsx1, m->wsy1, m->wsx2, m->wsy2, m->pr1,
                m->pr2, m->etamax, m->totalarea, m->tk_tmp, m->tkm, m->auv, m->nodecode,
-               m->nnodecode, m->mju, m->etamax_it, m->nibcnt, m->noff, m->noffold, m->alphal, m->coef,
+               m->nnodecode, m->mju, m->etamax_it, m->umax_it, m->umax, m->nibcnt, m->noff, m->noffold, m->alphal, m->coef,
                m->obccoef, m->wksp};
   for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
   free(m);
@@ -932,6 +936,15 @@ int adco_step(adco_model *m, int it, const double *wsx2, const double *wsy2,
       m->etamax[i] = m->eta2[i];
       m->etamax_it[i] = it;
     }
+  /* the peak speed at wet nodes (src/write_output.F:6367-6376) */
+  for (int i = 0; i < np; ++i)
+    if (m->nodecode[i] == 1) {
+      const double ut = sqrt(m->uu2[i] * m->uu2[i] + m->vv2[i] * m->vv2[i]);
+      if (ut > m->umax[i]) {
+        m->umax[i] = ut;
+        m->umax_it[i] = it;
+      }
+    }
   return ier;
 }
 
@@ -939,7 +952,7 @@ double *adco_array(adco_model *m, const char *name) {
   struct { const char *n; double *p; } t[] = {
       {"eta1", m->eta1}, {"eta2", m->eta2}, {"etas", m->etas}, {"uu2", m->uu2},
       {"vv2", m->vv2}, {"qx2", m->qx2}, {"qy2", m->qy2}, {"tk", m->tk},
-      {"etamax", m->etamax}, {"totalarea", m->totalarea}, {"coef", m->coef},
+      {"etamax", m->etamax}, {"umax", m->umax}, {"totalarea", m->totalarea}, {"coef", m->coef},
       {"gwce_lv", m->gwce_lv}, {"h2", m->h2}, {"h1", m->h1}, {"alphal", m->alphal},
       {"uu1", m->uu1}, {"vv1", m->vv1}, {"wsx2", m->wsx2}, {"wsy2", m->wsy2},
       {"pr2", m->pr2}, {"wsx1", m->wsx1}, {"wsy1", m->wsy1}, {"pr1", m->pr1},
@@ -955,6 +968,7 @@ int *adco_iarray(adco_model *m, const char *name) {
   if (!strcmp(name, "noff")) return m->noff;
   if (!strcmp(name, "mju")) return m->mju;
   if (!strcmp(name, "etamax_it")) return m->etamax_it;
+  if (!strcmp(name, "umax_it")) return m->umax_it;
   if (!strcmp(name, "neitab")) return m->neitab;
   return NULL;
 }

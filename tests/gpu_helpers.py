@@ -10,8 +10,8 @@ import numpy as np
 import oracle_lib
 from adcirc_amd.model import Model
 
-STATE_F = ("eta2", "etas", "uu2", "vv2", "tk", "etamax")
-STATE_I = ("nodecode", "noff", "etamax_it")
+STATE_F = ("eta2", "etas", "uu2", "vv2", "tk", "etamax", "umax")
+STATE_I = ("nodecode", "noff", "etamax_it", "umax_it")
 
 
 def golden_tables(d):

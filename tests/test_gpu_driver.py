@@ -109,6 +109,10 @@ def test_driver_outputs_equal_per_step_state(tmp_path, name):
     tmax = outputs.etamax_time(g.get("etamax_it"), run.dtdp, run.statim)
     assert np.array_equal(emx[0][2][:, 0], [float(outputs.e20(v)) for v in g.get("etamax")])
     assert np.array_equal(emx[1][2][:, 0], [float(outputs.e20(v)) for v in tmax])
+    _, vmx = outputs.read_global(os.path.join(d, "maxvel.63"))
+    umt = outputs.etamax_time(g.get("umax_it"), run.dtdp, run.statim)
+    assert np.array_equal(vmx[0][2][:, 0], [float(outputs.e20(v)) for v in g.get("umax")])
+    assert np.array_equal(vmx[1][2][:, 0], [float(outputs.e20(v)) for v in umt])
     if name == "beach_storm":
         assert dry_seen > 0
     log(f"  records identical; dry node-records {dry_seen}")

@@ -246,3 +246,31 @@ def test_friction_closed_form():
     h = d["dp"] + eta
     expect = np.minimum(c.run.cf * np.sqrt(u * u + v * v) / h, 2.0 / c.run.dtdp)
     assert np.allclose(tk, expect, rtol=1e-14, atol=0)
+
+
+def test_peak_surge_and_speed_tracking(beach_golden):
+    """ETAMAX/UMAX and the steps of their last rise (collectMinMaxData,
+    src/write_output.F:6343-6405) against running maxima formed here from
+    each step's state over 120 steps of the wet/dry beach storm."""
+    d = beach_golden
+    m = oracle_lib.OracleModel(d, meshgen.beach().run, oracle_lib.SEQUENTIAL)
+    npn = int(d["np"])
+    emax = np.full(npn, -99999.0)
+    umax = np.full(npn, -99999.0)
+    eit = np.zeros(npn, np.int64)
+    uit = np.zeros(npn, np.int64)
+    for it in range(1, 121):
+        m.step(it)
+        wet = m.iarray("nodecode") == 1
+        e2 = m.array("eta2")
+        u, v = m.array("uu2"), m.array("vv2")
+        sp = np.sqrt(u * u + v * v)
+        up_e = wet & (e2 > emax)
+        emax[up_e], eit[up_e] = e2[up_e], it
+        up_u = wet & (sp > umax)
+        umax[up_u], uit[up_u] = sp[up_u], it
+    assert np.array_equal(m.array("etamax"), emax)
+    assert np.array_equal(m.array("umax"), umax)
+    assert np.array_equal(m.iarray("etamax_it"), eit)
+    assert np.array_equal(m.iarray("umax_it"), uit)
+    assert (uit > 0).any() and (eit > 0).any()
