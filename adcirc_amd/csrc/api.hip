@@ -616,6 +616,26 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
   if (!c || !eta2 || !nodecode || !noff) return fail(ADCGPU_E_ARG, "null argument");
   HIPCHK(hipSetDevice(c->device));
   const size_t np = c->np, ne = c->ne;
+  // everything else starts cold too, so that a context can be cold-started
+  // again: velocities, fluxes, increments, friction, forcing, the running
+  // maxima, the JCG start history and ITPACK's saved state
+  double* zero_d[] = {c->uu1, c->vv1, c->uu2, c->vv2, c->qx1, c->qy1, c->qx2, c->qy2,
+                      c->etas, c->etas_prev, c->tk, c->gwce_lv, c->h1, c->h2,
+                      c->wsx1, c->wsy1, c->wsx2, c->wsy2};
+  for (double* p : zero_d) HIPCHK(hipMemsetAsync(p, 0, np * 8, c->stream));
+  HIPCHK(hipMemsetAsync(c->etamax_it, 0, np * 4, c->stream));
+  HIPCHK(hipMemsetAsync(c->umax_it, 0, np * 4, c->stream));
+  HIPCHK(hipMemsetAsync(c->jsaved, 0, 12 * sizeof(double), c->stream));
+  {
+    std::vector<double> pr(np, c->prm.prbckgrnd_mh2o), low(np, -99999.0), ones(ne, 1.0);
+    HIPCHK(hipMemcpyAsync(c->pr1, pr.data(), np * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->pr2, pr.data(), np * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->etamax, low.data(), np * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->umax, low.data(), np * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->alphal, ones.data(), ne * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->coef_scaled = false;
   HIPCHK(hipMemcpyAsync(c->eta2, eta2, np * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->eta1, eta2, np * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->nodecode, nodecode, np * 4, hipMemcpyHostToDevice, c->stream));

@@ -116,7 +116,10 @@ int adcgpu_set_field(adcgpu_ctx *ctx, int field, const void *host, int64_t count
 int adcgpu_get_field(adcgpu_ctx *ctx, int field, void *host, int64_t count);
 /* Cold start: eta2 (= eta1), nodecode and noff from the reference's
  * setColdWetDryStateAndWaterSurfaceElevation, TotalArea/MJU computed here
- * (src/mesh.F:3360 totalAreaCalc), NCChange = 1. */
+ * (src/mesh.F:3360 totalAreaCalc), NCChange = 1.  All other state starts
+ * cold as well (velocities, fluxes, ETAS, TK, forcing at rest, ALPHAL = 1,
+ * the peak-surge and peak-speed maxima, the JCG saved state), so a context
+ * may be cold-started again. */
 int adcgpu_cold_start(adcgpu_ctx *ctx, const double *eta2, const int32_t *nodecode,
                       const int32_t *noff);
 

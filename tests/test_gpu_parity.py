@@ -355,3 +355,39 @@ def test_renumbered_mesh_bitexact(numbering):
     else:
         assert 4096 < bw <= 2 * 4096
     _run_bitexact(d, case.run, 12, check_every=4)
+
+
+def test_cold_start_again_equals_fresh_context():
+    """A context that has run 60 steps of the wet/dry beach storm and is cold
+    started again steps exactly like a fresh one (state, COEF, maxima, JCG
+    iteration counts)."""
+    from gpu_helpers import case_setup, golden_tables
+    from adcirc_amd.model import Model
+    case = meshgen.beach()
+    d = case_setup(case)
+    run = case.run
+    t = golden_tables(d)
+    nb = run.nbfr
+
+    def fresh():
+        g = Model(t, run, tau0var=d["tau0var"], evm=d["evm"], fric=d["fric"],
+                  corif=np.full(t["np"], run.cori),
+                  emo=np.asarray(d["emo"]).reshape(t["neta"], nb),
+                  efa=np.asarray(d["efa"]).reshape(t["neta"], nb))
+        g.cold_start(d["eta2_0"], d["nodecode0"], d["noff0"])
+        g.holland(0.0, run.holland)
+        return g
+
+    log("case beach: 60 steps, cold start again, 60 steps vs a fresh context")
+    g1 = fresh()
+    g1.run_resident(1, 60, holland=run.holland)
+    g1.cold_start(d["eta2_0"], d["nodecode0"], d["noff0"])
+    g1.holland(0.0, run.holland)
+    it1 = g1.run_resident(1, 60, holland=run.holland)
+    g2 = fresh()
+    it2 = g2.run_resident(1, 60, holland=run.holland)
+    assert it1 == it2
+    for k in ("eta1", "eta2", "etas", "uu2", "vv2", "qx2", "qy2", "tk", "etamax", "umax", "coef",
+              "alphal", "nodecode", "noff", "etamax_it", "umax_it"):
+        a, b = g1.get(k), g2.get(k)
+        assert np.array_equal(a, b), (k, int((a != b).sum()))
