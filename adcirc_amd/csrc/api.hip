@@ -781,11 +781,9 @@ static int wetdry_device(adcgpu_ctx* c, bool alpha_pass) {
                      c->flags);
   hipLaunchKernelGGL(k_wd_link, dim3(1), dim3(256), WD_LINK_LDS, c->stream, c->wd, c->flags,
                      c->tk);
-  hipLaunchKernelGGL(k_wd_final, dim3(std::min(nb, 1024)), dim3(256), 0, c->stream, c->wd,
-                     c->flags, c->tk);
   // cx is free until the momentum step: it holds NCEle*AREAS/2 per incidence slot
-  hipLaunchKernelGGL(k_wd_area, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), c->cx);
+  hipLaunchKernelGGL(k_wd_final_area, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), c->wd, c->flags, c->tk, c->cx);
   hipLaunchKernelGGL(k_wd_d2, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->cx, c->flags, c->ifnlfa);
   if (alpha_pass)

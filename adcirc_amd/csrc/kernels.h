@@ -208,8 +208,8 @@ __global__ void k_wd_index(const int* bcnt, int nb, int* bofs, int* flags);
 __global__ void k_wd_pred(DevMesh m, DevState s, WdWork w);
 __global__ void k_wd_close(WdWork w, int* flags);
 __global__ void k_wd_link(WdWork w, int* flags, double* tk);
-__global__ void k_wd_final(WdWork w, const int* flags, double* tk);
-__global__ void k_wd_area(DevMesh m, DevState s, double* area_slot);
+__global__ void k_wd_final_area(DevMesh m, DevState s, WdWork w, const int* flags, double* tk,
+                                double* area_slot);
 __global__ void k_wd_d2(DevMesh m, DevState s, const double* area_slot, int* flags, double ifnlfa);
 __global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim);
 __global__ void k_h2(int np, const double* dp, const double* eta2, double ifnlfa, double* h2);

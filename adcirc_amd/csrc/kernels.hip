@@ -930,32 +930,35 @@ __global__ __launch_bounds__(256) void k_wd_link(WdWork w, int* flags, double* _
 }
 
 // Final values and the TK of each node from its last candidate.
-__global__ __launch_bounds__(256) void k_wd_final(WdWork w, const int* flags,
-                                                  double* __restrict__ tk) {
+__global__ __launch_bounds__(256) void k_wd_final_area(DevMesh m, DevState s, WdWork w,
+                                                       const int* flags,
+                                                       double* __restrict__ tk,
+                                                       double* __restrict__ area_slot) {
+  // (1) each candidate's M_k, written as TK of its two wet nodes (skipped
+  // when the serial pass already did it)
   const int n = flags[F_NCAND];
-  if (flags[F_WDSERIAL]) return;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    double v = w.cv[k];
-    const double* Xb = w.xval + (k / WD_BLOCK) * WD_XMAX;
+  if (!flags[F_WDSERIAL]) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+      double v = w.cv[k];
+      const double* Xb = w.xval + (k / WD_BLOCK) * WD_XMAX;
 #pragma unroll
-    for (int i = 0; i < WD_XW; ++i) {
-      unsigned long long E = w.cE[(size_t)k * WD_XW + i];
-      while (E) {
-        v = fmax(v, Xb[64 * i + __ffsll((long long)E) - 1]);
-        E &= E - 1;
+      for (int i = 0; i < WD_XW; ++i) {
+        unsigned long long E = w.cE[(size_t)k * WD_XW + i];
+        while (E) {
+          v = fmax(v, Xb[64 * i + __ffsll((long long)E) - 1]);
+          E &= E - 1;
+        }
       }
+      const int2 x = w.wnode[k];
+      if (x.x >= 0) tk[x.x] = v;
+      if (x.y >= 0) tk[x.y] = v;
     }
-    const int2 x = w.wnode[k];
-    if (x.x >= 0) tk[x.x] = v;
-    if (x.y >= 0) tk[x.y] = v;
   }
-}
-
-// D2, element part: NCEle (NNODECODE of the three nodes times NOFF) and its
-// area term NCEle*AREAS/2, written to the incidence slots of the element's
-// three nodes (src/wetdry.F:1361-1380).
-__global__ __launch_bounds__(256) void k_wd_area(DevMesh m, DevState s, double* __restrict__ area_slot) {
-  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  // (2) D2, element part: NCEle (NNODECODE of the three nodes times NOFF,
+  // both final since the candidate pass) and its area term NCEle*AREAS/2,
+  // written to the incidence slots of the element's three nodes
+  // (src/wetdry.F:1361-1380).  Independent of (1): one launch for both.
+  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const int ncele = s.nnodecode[m.nm[3 * e]] * s.nnodecode[m.nm[3 * e + 1]] *
                     s.nnodecode[m.nm[3 * e + 2]] * s.noff[e];
