@@ -355,7 +355,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
       obcl[nb].push_back(I * mnei + (J - 2));
     }
   }
-  // the COEF positions that involve a boundary node, for k_lhs_bc
+  // the COEF positions that involve a boundary node, for the matrix's boundary pass
   std::vector<int> obc_q, obc_s;
   for (int j = 1; j < mnei; ++j)
     for (int i = 0; i < np; ++i) {
@@ -702,22 +702,21 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
     hipLaunchKernelGGL(k_lhs_rows<true>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
                        c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
                        rcs, rw);
-    hipLaunchKernelGGL(k_lhs_bc<true>, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(),
-                       c->state(), c->flags, c->nch, c->ep_part, c->scal);
   } else {
     hipLaunchKernelGGL(k_lhs_elem<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
                        c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
     hipLaunchKernelGGL(k_lhs_rows<false>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
                        c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
                        rcs, rw);
-    hipLaunchKernelGGL(k_lhs_bc<false>, dim3(1), dim3(RED_XTHREADS), 0, c->stream, c->mesh(),
-                       c->state(), c->flags, c->nch, c->ep_part, c->scal);
   }
   c->coef_scaled = false;
   RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, lumped ? 4.0 : 2.0, lumped ? 0.0 : 1.0,
            lumped ? 1 : 0, c->prm.nws};
-  hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), q, c->flags, c->contrib);
+  // its last block finishes the matrix (EP, elevation-boundary rows) and
+  // clears NCChange
+  static_assert(RED_XTHREADS == 256, "the matrix-finishing block is a 256-thread block");
+  hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256) + 1), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), q, c->flags, c->contrib, c->nch, c->ep_part, c->scal);
   // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
   std::swap(c->eta1, c->eta2);
   std::swap(c->h1, c->h2);

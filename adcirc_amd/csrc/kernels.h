@@ -184,11 +184,10 @@ template <bool LUMPED>
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
                            const double* sa, const double* sb, double* ep_part,
                            const double* restore_cs, const double* restore_w);
-template <bool LUMPED>
-__global__ void k_lhs_bc(DevMesh m, DevState s, const int* flags, int nch, const double* ep_part,
-                         double* scal);
 __global__ void k_coef_restore(DevMesh m, DevState s, const double* cs, const double* w);
-__global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags, double* contrib);
+// one block more than the elements need: the last finishes the GWCE matrix
+__global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags, double* contrib, int nch,
+                           const double* ep_part, double* scal);
 // Jacobi scaling of the GWCE rows ahead of k_jcg (its first scaling pass)
 struct JcgPre {
   double* w;          // 1/sqrt(diag)

@@ -103,9 +103,9 @@ __global__ __launch_bounds__(256) void k_friction_holland(int np, FricPrm f, dou
 
 // --------------------------------------------------------------- GWCE LHS
 // The matrix is rebuilt when NCChange > 0 and NCChange is then cleared
-// (src/gwce.F:427-428): the LHS kernels test F_NCCHANGE, and the first thread
+// (src/gwce.F:427-428): the LHS kernels test F_NCCHANGE, and the last block
 // of the right-hand-side element kernel, which runs after them and before
-// wet/dry sets the flag again, clears it.
+// wet/dry sets the flag again, finishes the matrix and clears it.
 
 // GWCE matrix, element part (src/gwce.F:456-592): for element e and its local
 // vertex k, the diagonal term and the two off-diagonal terms of row NM(e,k),
@@ -233,12 +233,10 @@ template __global__ void k_lhs_rows<true>(DevMesh, DevState, const int*, const d
 // listed positions are touched: one block.  Zero diagonals of dry rows are
 // set to EP by k_rhs_node.
 template <bool LUMPED>
-__global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, const int* flags, int nch,
-                                                const double* __restrict__ ep_part,
-                                                double* __restrict__ scal) {
+__device__ void lhs_bc_block(const DevMesh& m, DevState s, int nch,
+                             const double* __restrict__ ep_part, double* __restrict__ scal) {
   __shared__ double lds[4];
   __shared__ double ep_s;
-  if (flags[F_NCCHANGE] == 0) return;
   const int np = m.np, t = threadIdx.x;
   double acc = 0.0;
   for (int c = t; c < nch; c += RED_XTHREADS) acc = acc + ep_part[c];
@@ -268,8 +266,6 @@ __global__ __launch_bounds__(256) void k_lhs_bc(DevMesh m, DevState s, const int
     }
   }
 }
-template __global__ void k_lhs_bc<false>(DevMesh, DevState, const int*, int, const double*, double*);
-template __global__ void k_lhs_bc<true>(DevMesh, DevState, const int*, int, const double*, double*);
 
 // The off-diagonals of a solve whose unscaling was deferred, restored as
 // unscal does (src/itpackv.F:2460-2479).
@@ -292,12 +288,30 @@ __global__ __launch_bounds__(256) void k_coef_restore(DevMesh m, DevState s,
 // configuration (sponge layer, dispersion, baroclinic, tidal potential, water
 // level offset) are omitted; they only add signed zeros.
 __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags,
-                                                  double* __restrict__ contrib) {
-  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e == 0) {
-    if (flags[F_NCCHANGE] > 0) flags[F_NREBUILD] += 1;
-    flags[F_NCCHANGE] = 0;
+                                                  double* __restrict__ contrib, int nch,
+                                                  const double* __restrict__ ep_part,
+                                                  double* __restrict__ scal) {
+  // The last block finishes the GWCE matrix instead (EP and the elevation-
+  // boundary rows, when it was rebuilt), beside the element blocks, which read
+  // neither; it then clears NCChange (src/gwce.F:427-428), after every thread
+  // of the block has read it.
+  const int nbe = (int)gridDim.x - 1;
+  if ((int)blockIdx.x == nbe) {
+    const int ncc = flags[F_NCCHANGE];
+    if (ncc > 0) {
+      if (q.lumped)
+        lhs_bc_block<true>(m, s, nch, ep_part, scal);
+      else
+        lhs_bc_block<false>(m, s, nch, ep_part, scal);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (ncc > 0) flags[F_NREBUILD] += 1;
+      flags[F_NCCHANGE] = 0;
+    }
+    return;
   }
+  int e = xcd_tile(blockIdx.x, nbe) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
