@@ -9,8 +9,9 @@ at 100k nodes, so the two sides run separately:
 
 The GPU side steps through the per-step C ABI (adcgpu_timestep) with the
 oracle's host Holland forcing, so both sides see identical inputs; every
-simulated hour it compares ETA2, UU2, VV2, TK, ETAS, ETAMAX, NODECODE, NOFF
-and NUMITR and exits non-zero on any difference.
+simulated hour it compares ETA2, UU2, VV2, TK, ETAS, ETAMAX, UMAX, NODECODE,
+NOFF, the steps of the last ETAMAX/UMAX rise and NUMITR, and exits non-zero
+on any difference.
 """
 from __future__ import annotations
 
@@ -28,8 +29,8 @@ import oracle_lib  # noqa: E402
 from gpu_helpers import case_setup, golden_tables  # noqa: E402
 from adcirc_amd import meshgen  # noqa: E402
 
-F = ("eta2", "uu2", "vv2", "tk", "etas", "etamax")
-I = ("nodecode", "noff")
+F = ("eta2", "uu2", "vv2", "tk", "etas", "etamax", "umax")
+I = ("nodecode", "noff", "etamax_it", "umax_it")
 EVERY = 3600
 
 
