@@ -17,7 +17,8 @@ ip_ = C.POINTER(C.c_int32)
 FP64, MIXED = 0, 1
 FIELDS = dict(eta1=0, eta2=1, etas=2, uu1=3, vv1=4, uu2=5, vv2=6, qx1=7, qy1=8, qx2=9, qy2=10,
               h1=11, h2=12, tk=13, etamax=14, totalarea=15, wsx1=16, wsy1=17, wsx2=18,
-              wsy2=19, pr1=20, pr2=21, gwce_lv=22, umax=23, nodecode=100, noff=101, mju=102, etamax_it=103, umax_it=104,
+              wsy2=19, pr1=20, pr2=21, gwce_lv=22, umax=23, etas_prev=24, nodecode=100, noff=101,
+              mju=102, etamax_it=103, umax_it=104,
               coef=200, alphal=300)
 INT_FIELDS = ("nodecode", "noff", "mju", "etamax_it", "umax_it")
 
@@ -27,7 +28,14 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_set_forcing", "adcgpu_solve_gwce", "adcgpu_wetdry", "adcgpu_momentum",
            "adcgpu_timestep", "adcgpu_last_numitr", "adcgpu_synchronize", "adcgpu_run",
            "adcgpu_run_iterations", "adcgpu_run_rebuilds", "adcgpu_elevation_bc", "adcgpu_jcg", "adcgpu_holland",
-           "adcgpu_profile", "adcgpu_profile_read", "adcgpu_jcg_shape")
+           "adcgpu_profile", "adcgpu_profile_read", "adcgpu_jcg_shape", "adcgpu_run_stats",
+           "adcgpu_set_wait_limit", "adcgpu_ml_params",
+           "adcgpu_ml_collect", "adcgpu_ml_samples", "adcgpu_ml_train", "adcgpu_ml_grad",
+           "adcgpu_ml_features", "adcgpu_ml_correction", "adcgpu_state_bytes",
+           "adcgpu_state_save", "adcgpu_state_load")
+ML_NPARAM = 22913
+# error codes (ADCGPU_E_*)
+E_ARG, E_UNSUPPORTED, E_HIP, E_DIAG, E_CAPACITY, E_NONFINITE, E_ERRORELEV = range(-1, -8, -1)
 
 
 class Mesh(C.Structure):
@@ -46,7 +54,8 @@ class Params(C.Structure):
                 ("nolifa", C.c_int32), ("nolica", C.c_int32), ("nolicat", C.c_int32),
                 ("nolibf", C.c_int32), ("itmax", C.c_int32), ("nws", C.c_int32),
                 ("precision", C.c_int32), ("tau0var", dp_), ("evm", dp_), ("fric", dp_),
-                ("corif", dp_), ("ilump", C.c_int32), ("warm_start", C.c_int32)]
+                ("corif", dp_), ("ilump", C.c_int32), ("warm_start", C.c_int32),
+                ("errorelev", C.c_double)]
 
 
 _lib = None
@@ -88,6 +97,21 @@ def load() -> C.CDLL:
     lib.adcgpu_profile.argtypes = [vp, C.c_int]
     lib.adcgpu_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.adcgpu_jcg_shape.argtypes = [vp, ip_]
+    lib.adcgpu_run_stats.argtypes = [vp, C.POINTER(C.c_int64)]
+    lib.adcgpu_set_wait_limit.argtypes = [C.c_double]
+    lib.adcgpu_state_bytes.argtypes = [vp]
+    lib.adcgpu_state_bytes.restype = C.c_int64
+    lib.adcgpu_state_save.argtypes = [vp, vp, C.c_int64]
+    lib.adcgpu_state_load.argtypes = [vp, vp, C.c_int64]
+    fp_ = C.POINTER(C.c_float)
+    lib.adcgpu_ml_params.argtypes = [vp, fp_, fp_]
+    lib.adcgpu_ml_collect.argtypes = [vp, C.c_int]
+    lib.adcgpu_ml_samples.argtypes = [vp]
+    lib.adcgpu_ml_samples.restype = C.c_int64
+    lib.adcgpu_ml_train.argtypes = [vp, C.c_int, C.c_double, C.c_uint64, dp_]
+    lib.adcgpu_ml_grad.argtypes = [vp, C.c_int, fp_, fp_, fp_, fp_, dp_]
+    lib.adcgpu_ml_features.argtypes = [vp, fp_, dp_]
+    lib.adcgpu_ml_correction.argtypes = [vp, dp_]
     _lib = lib
     return lib
 

@@ -11,16 +11,25 @@
 #include <vector>
 
 #include "../../include/adcgpu.h"
+#include "jcg_host.h"
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace adcgpu;
 
+// message of the last failed call on this thread: a fixed buffer, so that
+// recording an error allocates nothing
+thread_local char g_err[512];
+
+void adcgpu::set_last_error(const char* msg) { std::snprintf(g_err, sizeof(g_err), "%s", msg); }
+
 namespace {
 
-thread_local std::string g_err;
+// bound of every wait inside the persistent JCG kernel (adcgpu_set_wait_limit)
+double g_wait_seconds = 0.5;
 
 int fail(int code, const std::string& msg) {
-  g_err = msg;
+  set_last_error(msg.c_str());
   return code;
 }
 
@@ -57,7 +66,7 @@ struct adcgpu_ctx {
   int np = 0, ne = 0, mnei = 0, neta = 0, nch = 0;
   adcgpu_params prm{};
   double ifnlfa = 1, ifnlct = 1, ifnlcat = 1, iflinbf = 0, ifnlbf = 1, tkmax = 1;
-  double habsmin = 0, hoff = 0;
+  double habsmin = 0, hoff = 0, errorelev = 1000.0;
   // mesh
   int *nm = nullptr, *inc_ptr = nullptr, *inc = nullptr, *slot = nullptr, *nneigh = nullptr;
   int *col = nullptr, *slot_cols = nullptr;
@@ -83,11 +92,34 @@ struct adcgpu_ctx {
   double *jpart = nullptr, *jsaved = nullptr;
   unsigned* jbar = nullptr;
   int jcg_grid = 0;
-  long long barrier_ticks = 0;  // 0.5 s of the device wall clock
+  int clock_khz = 0;            // device wall-clock rate: the JCG wait bound in ticks
   int jcg_cap = 0;              // resident k_jcg blocks on this device
   int jcg_halo = -1;            // row tiles each side a tile's rows reference (-1: any)
-  int64_t run_iters = 0, run_rebuilds = 0;
+  int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
+  // the GWCE matrix's unchanging terms per incidence slot, and each element's
+  // NCEle as the next rebuild sees it (written by the momentum element pass;
+  // recomputed when the flags were changed otherwise)
+  double *lhs_td = nullptr, *lhs_ta = nullptr, *lhs_tb = nullptr;
+  unsigned char* ncele = nullptr;
+  bool ncele_fresh = false;
+  // learned JCG start (set up on first use): FP32 master parameters and Adam
+  // moments, the kernels' weight layouts, the per-step correction and second
+  // start, training samples and the training work arrays
+  bool ml_ready = false;
+  float *ml_master = nullptr, *ml_m1 = nullptr, *ml_m2 = nullptr;
+  MlWeights mlw{};
+  double *ml_corr = nullptr, *ml_u = nullptr, *jd2 = nullptr;
+  MlSamples smp{};
+  int64_t smp_total = 0;  // samples collected so far (the ring keeps the last cap)
+  int ml_collect = 0;     // samples per step while collecting (0: off)
+  int ml_step = 0;        // Adam steps taken
+  bf16 *tx = nullptr, *txt = nullptr, *ta1 = nullptr, *ta1t = nullptr, *tdz2 = nullptr,
+       *tdz2t = nullptr, *tdz1t = nullptr;
+  float *tz1 = nullptr, *tz2 = nullptr, *tda1 = nullptr, *tgw1 = nullptr, *tgw2 = nullptr,
+        *tpo = nullptr, *tpz = nullptr, *tloss = nullptr, *ty = nullptr, *tw = nullptr,
+        *twt = nullptr, *tmeanw = nullptr;
+  int* tperm = nullptr;
   // optional timing of the JCG launches with HIP events on the context stream
   int profile = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> jcg_events;  // pool, created ahead
@@ -114,7 +146,7 @@ struct adcgpu_ctx {
     s.wsx2 = wsx2; s.wsy2 = wsy2; s.pr1 = pr1; s.pr2 = pr2; s.gwce_lv = gwce_lv;
     s.alphal = alphal; s.coef = coef; s.obc = obc; s.fric = fric; s.nodecode = nodecode;
     s.nnodecode = nnodecode; s.noff = noff; s.noffold = noffold; s.mju = mju;
-    s.etamax_it = etamax_it; s.umax = umax; s.umax_it = umax_it;
+    s.etamax_it = etamax_it; s.umax = umax; s.umax_it = umax_it; s.ncele = ncele;
     return s;
   }
 };
@@ -127,38 +159,20 @@ int check_launch(const char* what) {
   return 0;
 }
 
-// Blocks of k_jcg that are resident on one CU at once.  The occupancy query
-// counts wave slots per CU, but the dispatcher places the waves of a
-// workgroup on the four SIMDs in turn, so a block of W waves takes ceil(W/4)
-// slots of one SIMD (measured on gfx950: 5-wave blocks with 164 VGPRs run one
-// per CU although the query reports two; 4- and 8-wave blocks run as many as
-// it reports).  A cooperative grid larger than what is really resident never
-// passes its first grid barrier, so the smaller of the two counts is used.
-// grid barrier counters, then one halo flag per block
-static int barrier_words_for(int grid) {
-  return ((grid + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE +
-         grid * JCG_HALO_STRIDE;
-}
-
-int jcg_blocks_per_cu(int device, int* per_cu_out, int* cus_out) {
-  int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jcg, JCG_BLOCK, 0));
-  hipFuncAttributes fa;
-  HIPCHK(hipFuncGetAttributes(&fa, (const void*)k_jcg));
-  const int vgpr = std::max((fa.numRegs + 7) & ~7, 8);  // allocation granule 8
-  const int per_simd = std::min(8, 512 / vgpr);
-  const int waves = (JCG_BLOCK + 63) / 64;
-  per_cu = std::min(per_cu, per_simd / ((waves + 3) / 4));
-  hipDeviceProp_t prop;
-  HIPCHK(hipGetDeviceProperties(&prop, device));
-  *per_cu_out = per_cu;
-  *cus_out = prop.multiProcessorCount;
+// The sticky error conditions of the device flag words, as a status code.
+int flag_errors(const int* f) {
+  const int e = f[F_ERROR];
+  const std::string at = " (first at step " + std::to_string(f[F_ERRSTEP]) + ")";
+  if (e & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG wait timed out");
+  if (e & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
+  if (e & ERR_NONFINITE) return fail(ADCGPU_E_NONFINITE, "non-finite velocity" + at);
+  if (e & ERR_ELEV) return fail(ADCGPU_E_ERRORELEV, "wet-node elevation above ErrorElev" + at);
   return 0;
 }
 
 // JCG_CPB chunks of 2048 rows per block plus the eigenvalue block, all resident.
 int set_jcg_grid(adcgpu_ctx* c) {
-  const int grid = (c->nch + JCG_CPB - 1) / JCG_CPB + 1;
+  const int grid = jcg_grid_for(c->np);
   if (grid > c->jcg_cap)
     return fail(ADCGPU_E_CAPACITY, "system too large for the register-resident JCG (" +
                                        std::to_string(c->nch) + " chunks of 2048 rows, " +
@@ -168,15 +182,10 @@ int set_jcg_grid(adcgpu_ctx* c) {
 }
 
 int setup_jcg_grid(adcgpu_ctx* c) {
-  int per_cu = 0, cus = 0;
-  if (int rc = jcg_blocks_per_cu(c->device, &per_cu, &cus)) return rc;
-  int bar_cap = 0;
-  while (barrier_words_for(bar_cap + 1) <= JCG_BAR_WORDS) ++bar_cap;
-  const int cap = std::min(per_cu * cus, bar_cap);
-  c->jcg_cap = cap;
+  HIPCHK(jcg_resident_cap(c->device, &c->jcg_cap));
   int khz = 0;
   HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
-  c->barrier_ticks = (long long)std::max(khz, 1000) * 500;
+  c->clock_khz = std::max(khz, 1000);
   return set_jcg_grid(c);
 }
 
@@ -194,24 +203,15 @@ int grow_events(adcgpu_ctx* c, size_t n) {
   return 0;
 }
 
-int jcg_barrier_words(const adcgpu_ctx* c) { return barrier_words_for(c->jcg_grid); }
+int jcg_barrier_words(const adcgpu_ctx* c) { return jcg_barrier_words_for(c->jcg_grid); }
 
 int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
                double* u, int n, int itmax, double zeta, double* eta2, const double* eta1,
-               const int* nodecode, bool prescaled, bool defer_unscal) {
+               const int* nodecode, bool prescaled, bool defer_unscal,
+               const double* u2 = nullptr) {
   JcgArgs a;
   a.n = n;
-  a.itmax = itmax;
-  a.maxfn = std::max(itmax, 50);
-  // zeta is raised to 512*srelpr if smaller (src/itpackv.F:223-232)
-  const double srelpr = std::ldexp(1.0, -52);
-  a.zeta = zeta >= 512.0 * srelpr ? zeta : 512.0 * srelpr;
-  // eigvss: nsig = max(int(-log10|zeta|), 4); zbrent: t = 10**(-nsig)
-  int itmp = (int)(-std::log10(std::fabs(a.zeta)));
-  int nsig = std::max(itmp, 4);
-  double p10 = 1.0;
-  for (int i = 0; i < nsig; ++i) p10 *= 10.0;
-  a.tz = 1.0 / p10;
+  jcg_tolerances(a, itmax, zeta);
   a.coef = coef;
   a.cs = c->cs;
   a.col = col;
@@ -226,11 +226,14 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   a.defer_unscal = defer_unscal;
   a.barrier = c->jbar;
   a.halo = c->jcg_halo;
-  a.barrier_ticks = c->barrier_ticks;
+  a.barrier_ticks = (long long)((double)c->clock_khz * 1000.0 * g_wait_seconds);
   a.flags = c->flags;
   a.eta2 = eta2;
   a.eta1 = eta1;
   a.nodecode = nodecode;
+  // the learned second start, formed by the pass before (learned mode)
+  a.u2 = u2;
+  a.d2 = c->jd2;
   // a prescaling pass (k_rhs_node) has already cleared the barrier words
   if (!prescaled) {
     const int bar_words = jcg_barrier_words(c);
@@ -243,17 +246,151 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
       if (int rc = grow_events(c, c->jcg_events.size() + 256)) return rc;
     HIPCHK(hipEventRecord(c->jcg_events[c->ev_used].first, c->stream));
   }
-  HIPCHK(hipLaunchCooperativeKernel((const void*)k_jcg, dim3(c->jcg_grid), dim3(JCG_BLOCK), args,
-                                    0, c->stream));
+  const void* fn = u2 ? (const void*)k_jcg<true> : (const void*)k_jcg<false>;
+  HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->jcg_grid), dim3(JCG_BLOCK), args, 0, c->stream));
   if (c->profile) HIPCHK(hipEventRecord(c->jcg_events[c->ev_used++].second, c->stream));
   return 0;
+}
+
+// ---------------------------------------------------------------- learned start
+constexpr int ML_CAP = 1 << 20;  // training samples kept (node snapshots)
+
+// Host conversions to and from bfloat16 (round to nearest even).
+uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float bf16_to_f32(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Initial parameters: W1, W2 uniform in +-sqrt(6/(fan_in+fan_out)) (Glorot &
+// Bengio 2010) from a fixed-seed generator, biases zero, and the output layer
+// zero, so the untrained network's correction is zero: the reference's start.
+std::vector<float> ml_initial_params() {
+  std::vector<float> p(ML_NPARAM, 0.0f);
+  uint64_t st = 0x9e3779b97f4a7c15ull;
+  auto uni = [&]() {  // splitmix64 -> [0, 1)
+    st += 0x9e3779b97f4a7c15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const double a1 = std::sqrt(6.0 / (ML_F + ML_H)), a2 = std::sqrt(6.0 / (ML_H + ML_H));
+  for (int k = 0; k < ML_W1; ++k) p[ML_OFF_W1 + k] = (float)((2.0 * uni() - 1.0) * a1);
+  for (int k = 0; k < ML_W2; ++k) p[ML_OFF_W2 + k] = (float)((2.0 * uni() - 1.0) * a2);
+  return p;
+}
+
+int ml_setup(adcgpu_ctx* c) {
+  if (c->ml_ready) return 0;
+  HIPCHK(hipSetDevice(c->device));
+  int rc = 0;
+  auto A = [&](auto** p, size_t n) {
+    if (!rc) rc = dmalloc(p, n);
+    if (!rc) {
+      c->allocs.push_back((void*)*p);
+      (void)hipMemset(*p, 0, n * sizeof(**p));
+    }
+  };
+  A(&c->ml_master, ML_NPARAM); A(&c->ml_m1, ML_NPARAM); A(&c->ml_m2, ML_NPARAM);
+  A(&c->mlw.w1p, ML_W1); A(&c->mlw.w2p, ML_W2); A(&c->mlw.w1t, ML_W1); A(&c->mlw.w2t, ML_W2);
+  A(&c->mlw.w2n, ML_W2); A(&c->mlw.b1p, 128); A(&c->mlw.b2p, 128); A(&c->mlw.w3p, 128);
+  A(&c->mlw.b3, 1);
+  A(&c->ml_corr, c->np); A(&c->ml_u, c->np); A(&c->jd2, c->np);
+  c->smp.cap = ML_CAP;
+  A(&c->smp.x, (size_t)ML_CAP * ML_F); A(&c->smp.s, ML_CAP); A(&c->smp.e0, ML_CAP);
+  A(&c->smp.node, ML_CAP); A(&c->smp.y, ML_CAP);
+  const size_t B = ML_BATCH;
+  A(&c->tx, B * ML_F); A(&c->txt, B * ML_F); A(&c->ta1, B * ML_H); A(&c->ta1t, B * ML_H);
+  A(&c->tdz2, B * ML_H); A(&c->tdz2t, B * ML_H); A(&c->tdz1t, B * ML_H);
+  A(&c->tz1, B * ML_H); A(&c->tz2, B * ML_H); A(&c->tda1, B * ML_H); A(&c->tgw1, ML_W1);
+  A(&c->tgw2, ML_W2); A(&c->tpo, (size_t)ML_TRAIN_BLOCKS * ML_PART_OUT);
+  A(&c->tpz, (size_t)ML_TRAIN_BLOCKS * ML_H); A(&c->tloss, ML_TRAIN_BLOCKS + 1); A(&c->ty, B);
+  A(&c->tw, B); A(&c->twt, ML_CAP); A(&c->tmeanw, 1); A(&c->tperm, B);
+  if (rc) return rc;
+  c->ml_ready = true;
+  const std::vector<float> p = ml_initial_params();
+  HIPCHK(hipMemcpy(c->ml_master, p.data(), p.size() * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_ml_pack, dim3(nblk(ML_W2 * 2, 256)), dim3(256), 0, c->stream,
+                     (const float*)c->ml_master, c->mlw);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return check_launch("learned start set-up");
+}
+
+// The step's feature inputs; `eta` is eta^k.
+MlIn ml_in(const adcgpu_ctx* c, const double* eta) {
+  MlIn in;
+  in.np = c->np;
+  in.nneigh = c->nneigh;
+  in.col = c->col;
+  in.bnd_index = c->bnd_index;
+  in.eta = eta;
+  in.etas = c->etas;
+  in.etasp = c->etas_prev;
+  in.dp = c->dp;
+  in.wsx2 = c->wsx2;
+  in.wsy2 = c->wsy2;
+  in.pr1 = c->pr1;
+  in.pr2 = c->pr2;
+  in.tk = c->tk;
+  in.nodecode = c->nodecode;
+  in.dt = c->prm.dtdp;
+  return in;
+}
+
+// One minibatch through the network and back; `perm` (device) names its
+// ML_BATCH sample slots (-1 pads).  With apply, one Adam step follows.
+int ml_minibatch(adcgpu_ctx* c, float inv_n, float* gout, bool apply, float lr, float* lossacc) {
+  const hipStream_t st = c->stream;
+  const int B = ML_BATCH;
+  hipLaunchKernelGGL(k_ml_batch, dim3(nblk(B * ML_F, 256)), dim3(256), 0, st, c->smp,
+                     (const float*)c->twt, (const int*)c->tperm, c->tx, c->txt, c->ty, c->tw);
+  // Z1 = X W1
+  hipLaunchKernelGGL(k_ml_gemm_nt, dim3(B / 32, ML_H / 32), dim3(64), 0, st, B, ML_H, ML_F,
+                     (const bf16*)c->tx, ML_F, (const bf16*)c->mlw.w1t, ML_F, c->tz1, ML_H);
+  hipLaunchKernelGGL(k_ml_act1, dim3(nblk(B * ML_H, 256)), dim3(256), 0, st, c->tz1,
+                     (const float*)c->ml_master, c->ta1, c->ta1t);
+  // Z2 = A1 W2
+  hipLaunchKernelGGL(k_ml_gemm_nt, dim3(B / 32, ML_H / 32), dim3(64), 0, st, B, ML_H, ML_H,
+                     (const bf16*)c->ta1, ML_H, (const bf16*)c->mlw.w2t, ML_H, c->tz2, ML_H);
+  hipLaunchKernelGGL(k_ml_out, dim3(ML_TRAIN_BLOCKS), dim3(ML_H), 0, st, (const float*)c->tz2,
+                     (const float*)c->ml_master, (const float*)c->ty, (const float*)c->tw, inv_n,
+                     c->tdz2, c->tdz2t, c->tpo, c->tloss);
+  // dW2 = A1^T dZ2 ; dA1 = dZ2 W2^T
+  hipLaunchKernelGGL(k_ml_gemm_nt, dim3(ML_H / 32, ML_H / 32), dim3(64), 0, st, ML_H, ML_H, B,
+                     (const bf16*)c->ta1t, B, (const bf16*)c->tdz2t, B, c->tgw2, ML_H);
+  hipLaunchKernelGGL(k_ml_gemm_nt, dim3(B / 32, ML_H / 32), dim3(64), 0, st, B, ML_H, ML_H,
+                     (const bf16*)c->tdz2, ML_H, (const bf16*)c->mlw.w2n, ML_H, c->tda1, ML_H);
+  hipLaunchKernelGGL(k_ml_dz1, dim3(ML_TRAIN_BLOCKS), dim3(ML_H), 0, st, (const float*)c->tda1,
+                     (const float*)c->tz1, c->tdz1t, c->tpz);
+  // dW1 = X^T dZ1 (48 rows)
+  hipLaunchKernelGGL(k_ml_gemm_nt, dim3((ML_F + 31) / 32, ML_H / 32), dim3(64), 0, st, ML_F,
+                     ML_H, B, (const bf16*)c->txt, B, (const bf16*)c->tdz1t, B, c->tgw1, ML_H);
+  if (apply) ++c->ml_step;
+  hipLaunchKernelGGL(k_ml_adam, dim3(nblk(ML_NPARAM, 256)), dim3(256), 0, st, c->ml_master,
+                     c->ml_m1, c->ml_m2, (const float*)c->tgw1, (const float*)c->tgw2,
+                     (const float*)c->tpo, (const float*)c->tpz, (const float*)c->tloss, lossacc,
+                     gout, apply ? 1 : 0, c->ml_step, lr);
+  if (apply)
+    hipLaunchKernelGGL(k_ml_pack, dim3(nblk(ML_W2 * 2, 256)), dim3(256), 0, st,
+                       (const float*)c->ml_master, c->mlw);
+  return check_launch("training minibatch");
 }
 
 }  // namespace
 
 extern "C" {
 
-const char* adcgpu_last_error(void) { return g_err.c_str(); }
+const char* adcgpu_last_error(void) { return g_err; }
 
 int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcgpu_ctx** out) {
   if (!M || !P || !out) return fail(ADCGPU_E_ARG, "null argument");
@@ -267,7 +404,9 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   if (P->nolibf < 0 || P->nolibf > 3) return fail(ADCGPU_E_ARG, "NOLIBF");
   if (P->itmax <= 0 || P->itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
   if (P->precision != ADCGPU_FP64) return fail(ADCGPU_E_UNSUPPORTED, "only the FP64 path");
-  if (P->warm_start != 0 && P->warm_start != 1) return fail(ADCGPU_E_ARG, "warm_start must be 0 or 1");
+  if (P->warm_start < 0 || P->warm_start > 2) return fail(ADCGPU_E_ARG, "warm_start must be 0, 1 or 2");
+  if (P->warm_start == 2 && P->ilump != 0)
+    return fail(ADCGPU_E_UNSUPPORTED, "the learned start needs the JCG solve (ILump = 0)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
     return fail(ADCGPU_E_HIP, "no HIP device " + std::to_string(device));
@@ -416,6 +555,11 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   c->tkmax = (double)2.0f / P->dtdp;
   c->habsmin = 0.8 * P->h0;
   c->hoff = 1.2 * P->h0;
+  if (P->errorelev < 0.0) {
+    delete c;
+    return fail(ADCGPU_E_ARG, "ErrorElev must not be negative");
+  }
+  c->errorelev = P->errorelev > 0.0 ? P->errorelev : 1000.0;
   int rc = 0;
   auto U = [&](auto** p, const auto& v) {
     if (!rc) rc = upload(p, v);
@@ -450,6 +594,8 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   A(&c->mju, np); A(&c->etamax_it, np); A(&c->etas_prev, np);
   A(&c->umax, np); A(&c->umax_it, np);
   A(&c->contrib, 3 * (size_t)ne); A(&c->cx, 3 * (size_t)ne); A(&c->cy, 3 * (size_t)ne);
+  A(&c->lhs_td, 3 * (size_t)ne); A(&c->lhs_ta, 3 * (size_t)ne); A(&c->lhs_tb, 3 * (size_t)ne);
+  A(&c->ncele, ne);
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
   {
     const size_t nb = (size_t)(ne + 255) / 256;
@@ -500,9 +646,18 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     int f[F_COUNT] = {0};
     f[F_NCCHANGE] = 1;
     (void)hipMemcpy(c->flags, f, sizeof(f), hipMemcpyHostToDevice);
-    hipError_t e = hipDeviceSynchronize();
+    // the GWCE matrix's unchanging terms, once
+    if (P->ilump != 0)
+      hipLaunchKernelGGL(k_lhs_terms<true>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
+                         c->mesh(), c->lhs_td, c->lhs_ta, c->lhs_tb);
+    else
+      hipLaunchKernelGGL(k_lhs_terms<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
+                         c->mesh(), c->lhs_td, c->lhs_ta, c->lhs_tb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
   }
+  if (!rc && P->warm_start == 2) rc = ml_setup(c);
   if (rc) {
     adcgpu_destroy(c);
     return rc;
@@ -550,6 +705,7 @@ static double** field_ptr(adcgpu_ctx* c, int f) {
     case ADCGPU_PR1: return &c->pr1;
     case ADCGPU_PR2: return &c->pr2;
     case ADCGPU_GWCE_LV: return &c->gwce_lv;
+    case ADCGPU_ETAS_PREV: return &c->etas_prev;
     case ADCGPU_UMAX: return &c->umax;
     case ADCGPU_COEF: return &c->coef;
     case ADCGPU_ALPHAL: return &c->alphal;
@@ -588,8 +744,14 @@ int adcgpu_set_field(adcgpu_ctx* c, int f, const void* host, int64_t count) {
   } else {
     return fail(ADCGPU_E_ARG, "unknown field");
   }
+  // new wet/dry flags: the next matrix rebuild forms NCEle afresh; an ALPHAL
+  // from the host may differ from 1, so the matrix terms are recomputed from
+  // it until the next wet/dry pass
+  if (f == ADCGPU_NODECODE || f == ADCGPU_NOFF) c->ncele_fresh = false;
+  if (f == ADCGPU_ALPHAL)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_ALPHA, 1, 1);
   HIPCHK(hipStreamSynchronize(c->stream));
-  return 0;
+  return check_launch("set_field");
 }
 
 static int restore_coef(adcgpu_ctx* c);
@@ -636,6 +798,7 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   c->coef_scaled = false;
+  c->ncele_fresh = false;
   HIPCHK(hipMemcpyAsync(c->eta2, eta2, np * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->eta1, eta2, np * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->nodecode, nodecode, np * 4, hipMemcpyHostToDevice, c->stream));
@@ -695,19 +858,26 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   const size_t rows_lds = (size_t)c->mnei * RED_THREADS * sizeof(double);
   const double* rcs = c->coef_scaled ? (const double*)c->cs : nullptr;
   const double* rw = c->coef_scaled ? (const double*)c->jw : nullptr;
+  // a rebuild multiplies the cached terms by each element's NCEle, which the
+  // momentum element pass left unless the flags changed since
+  if (!c->ncele_fresh) {
+    hipLaunchKernelGGL(k_ncele, dim3(nblk(ne, 256)), dim3(256), 0, c->stream, c->mesh(),
+                       c->state());
+    c->ncele_fresh = true;
+  }
+  const LhsTerms lt{c->lhs_td, c->lhs_ta, c->lhs_tb};
   // contrib, cx and cy are free until the GWCE right-hand side
   if (lumped) {
-    hipLaunchKernelGGL(k_lhs_elem<true>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
-                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
     hipLaunchKernelGGL(k_lhs_rows<true>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
-                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
-                       rcs, rw);
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, lt,
+                       c->ep_part, rcs, rw);
   } else {
+    // the element pass only runs while some ALPHAL differs from 1
     hipLaunchKernelGGL(k_lhs_elem<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
                        c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
     hipLaunchKernelGGL(k_lhs_rows<false>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
-                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, c->ep_part,
-                       rcs, rw);
+                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, lt,
+                       c->ep_part, rcs, rw);
   }
   c->coef_scaled = false;
   RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, lumped ? 4.0 : 2.0, lumped ? 0.0 : 1.0,
@@ -720,7 +890,22 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
   std::swap(c->eta1, c->eta2);
   std::swap(c->h1, c->h2);
-  JcgPre pre{c->jw, c->jbar, jcg_barrier_words(c), c->flags, c->etas_prev, c->prm.warm_start};
+  // the learned start's correction and the training samples are formed from
+  // the state before the solve (eta^k now in ETA1)
+  const bool learned = c->prm.warm_start == 2 && !lumped;
+  const bool collect = c->ml_collect > 0 && c->ml_ready && !lumped;
+  const int slot0 = collect ? (int)(c->smp_total % c->smp.cap) : 0;
+  if (learned || collect) {
+    const MlIn in = ml_in(c, c->eta1);
+    if (learned)
+      hipLaunchKernelGGL(k_ml_infer, dim3(std::min(nblk(np, 128), 4 * 256)), dim3(256), 0,
+                         c->stream, in, c->mlw, c->ml_corr);
+    if (collect)
+      hipLaunchKernelGGL(k_ml_collect, dim3(nblk(c->ml_collect, 128)), dim3(256), 0, c->stream,
+                         in, c->smp, slot0, c->ml_collect, c->cur_it);
+  }
+  JcgPre pre{c->jw, c->jbar, jcg_barrier_words(c), c->flags, c->etas_prev, c->prm.warm_start,
+             (c->prm.warm_start != 0 || c->ml_ready) ? 1 : 0, c->ml_corr, c->ml_u};
   if (lumped) {
     // the explicit solve of the lumped GWCE ends the node pass; no JCG
     hipLaunchKernelGGL(k_rhs_node<true>, dim3(nblk(np, 256)), dim3(256), 0, c->stream,
@@ -732,8 +917,16 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   int rc = check_launch("gwce assembly");
   if (rc) return rc;
   rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, np, c->prm.itmax,
-                  c->prm.convcr, c->eta2, c->eta1, c->nodecode, true, deferred);
+                  c->prm.convcr, c->eta2, c->eta1, c->nodecode, true, deferred,
+                  learned ? c->ml_u : nullptr);
   if (!rc && deferred) c->coef_scaled = true;
+  if (!rc && collect) {
+    // targets from this step's solution
+    hipLaunchKernelGGL(k_ml_target, dim3(nblk(c->ml_collect, 256)), dim3(256), 0, c->stream,
+                       c->smp, slot0, c->ml_collect, (const double*)c->etas);
+    c->smp_total += c->ml_collect;
+    rc = check_launch("training samples");
+  }
   return rc;
 }
 
@@ -761,6 +954,8 @@ int adcgpu_solve_gwce(adcgpu_ctx* c, int it, const double* eta_bc) {
 // element pass (the resident run).
 static int wetdry_device(adcgpu_ctx* c, bool alpha_pass) {
   const int np = c->np, ne = c->ne;
+  // the flags change here; the momentum element pass records NCEle again
+  c->ncele_fresh = false;
   if (c->prm.nolifa != 2) {
     hipLaunchKernelGGL(k_h2, dim3(nblk(np, 256)), dim3(256), 0, c->stream, np, c->dp, c->eta2,
                        c->ifnlfa, c->h2);
@@ -799,7 +994,7 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
 
 // Momentum; with `alpha` the element pass forms ALPHAL first.
 static int momentum_device(adcgpu_ctx* c, bool alpha) {
-  MomPrm q{c->ifnlct, c->prm.nws, c->cur_it};
+  MomPrm q{c->ifnlct, c->prm.nws, c->cur_it, c->errorelev, c->flags};
   if (alpha)
     hipLaunchKernelGGL(k_mom_elem<true>, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream,
                        c->mesh(), c->state(), q, c->cx, c->cy, c->flags);
@@ -808,6 +1003,7 @@ static int momentum_device(adcgpu_ctx* c, bool alpha) {
                        c->mesh(), c->state(), q, c->cx, c->cy, c->flags);
   hipLaunchKernelGGL(k_mom_node, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), q, c->cx, c->cy);
+  c->ncele_fresh = true;
   return check_launch("momentum");
 }
 
@@ -834,9 +1030,7 @@ int adcgpu_last_numitr(adcgpu_ctx* c, int* numitr, int* ier) {
   HIPCHK(hipStreamSynchronize(c->stream));
   if (numitr) *numitr = f[F_NUMITR];
   if (ier) *ier = f[F_IER];
-  if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
-  if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
-  return 0;
+  return flag_errors(f);
 }
 
 int adcgpu_synchronize(adcgpu_ctx* c) {
@@ -857,8 +1051,9 @@ int adcgpu_holland(adcgpu_ctx* c, double t, const double* prm13) {
 
 int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
                const double* holland, int use_graph) {
-  (void)use_graph;
   if (!c || nsteps < 0) return fail(ADCGPU_E_ARG, "invalid argument");
+  if (use_graph != 0)
+    return fail(ADCGPU_E_UNSUPPORTED, "graph capture of the step is not implemented");
   if (c->prm.nws != 0 && !holland) return fail(ADCGPU_E_ARG, "met forcing required");
   HIPCHK(hipSetDevice(c->device));
   if (c->neta > 0) {
@@ -874,6 +1069,8 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   }
   // F_ITSUM and F_NREBUILD are adjacent: one fill clears both
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_ITSUM, 0, 2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_NITMAX, 0, 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_MLUSED, 0, 1);
   for (int s = 0; s < nsteps; ++s) {
     const int it = it0 + s;
     int rc = adcgpu_begin_step(c, it);
@@ -901,7 +1098,14 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
       int f[F_COUNT];
       HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
-      if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
+      if (f[F_ERROR]) {
+        c->run_iters = f[F_ITSUM];
+        c->run_rebuilds = f[F_NREBUILD];
+        c->run_itmax = f[F_NITMAX];
+        c->run_errstep = f[F_ERRSTEP];
+        c->run_mlused = f[F_MLUSED];
+        return flag_errors(f);
+      }
     }
   }
   if (int rc = restore_coef(c)) return rc;
@@ -910,8 +1114,233 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   HIPCHK(hipStreamSynchronize(c->stream));
   c->run_iters = f[F_ITSUM];
   c->run_rebuilds = f[F_NREBUILD];
-  if (f[F_ERROR] & ERR_BARRIER) return fail(ADCGPU_E_HIP, "JCG grid barrier timed out");
-  if (f[F_ERROR] & ERR_DIAG) return fail(ADCGPU_E_DIAG, "GWCE diagonal <= 0");
+  c->run_itmax = f[F_NITMAX];
+  c->run_errstep = f[F_ERRSTEP];
+  c->run_mlused = f[F_MLUSED];
+  return flag_errors(f);
+}
+
+int adcgpu_run_stats(adcgpu_ctx* c, int64_t* out5) {
+  if (!c || !out5) return fail(ADCGPU_E_ARG, "null argument");
+  out5[0] = c->run_iters;
+  out5[1] = c->run_rebuilds;
+  out5[2] = c->run_itmax;
+  out5[3] = c->run_errstep;
+  out5[4] = c->run_mlused;
+  return 0;
+}
+
+// ---- learned start
+int adcgpu_ml_params(adcgpu_ctx* c, const float* in, float* out) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  if (int rc = ml_setup(c)) return rc;
+  if (in) {
+    HIPCHK(hipMemcpyAsync(c->ml_master, in, (size_t)ML_NPARAM * 4, hipMemcpyHostToDevice,
+                          c->stream));
+    // a new starting point for training: the Adam moments restart
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ML_NPARAM, 256)), dim3(256), 0, c->stream,
+                       reinterpret_cast<int*>(c->ml_m1), 0, ML_NPARAM);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ML_NPARAM, 256)), dim3(256), 0, c->stream,
+                       reinterpret_cast<int*>(c->ml_m2), 0, ML_NPARAM);
+    c->ml_step = 0;
+    hipLaunchKernelGGL(k_ml_pack, dim3(nblk(ML_W2 * 2, 256)), dim3(256), 0, c->stream,
+                       (const float*)c->ml_master, c->mlw);
+  }
+  if (out)
+    HIPCHK(hipMemcpyAsync(out, c->ml_master, (size_t)ML_NPARAM * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return check_launch("ml params");
+}
+
+int adcgpu_ml_collect(adcgpu_ctx* c, int per_step) {
+  if (!c || per_step < 0 || per_step > ML_CAP) return fail(ADCGPU_E_ARG, "invalid argument");
+  if (c->prm.ilump != 0) return fail(ADCGPU_E_UNSUPPORTED, "training samples need ILump = 0");
+  if (int rc = ml_setup(c)) return rc;
+  c->ml_collect = per_step;
+  return 0;
+}
+
+int64_t adcgpu_ml_samples(adcgpu_ctx* c) {
+  if (!c) return -1;
+  return std::min<int64_t>(c->smp_total, c->ml_ready ? c->smp.cap : 0);
+}
+
+int adcgpu_ml_train(adcgpu_ctx* c, int epochs, double lr, uint64_t seed, double* loss_out) {
+  if (!c || epochs < 0 || !(lr > 0.0)) return fail(ADCGPU_E_ARG, "invalid argument");
+  if (int rc = ml_setup(c)) return rc;
+  const int ns = (int)std::min<int64_t>(c->smp_total, c->smp.cap);
+  if (ns <= 0) return fail(ADCGPU_E_ARG, "no training samples collected");
+  hipLaunchKernelGGL(k_ml_weights, dim3(1), dim3(1024), 0, c->stream, c->smp, ns, c->twt,
+                     c->tmeanw);
+  const int nb = (ns + ML_BATCH - 1) / ML_BATCH;
+  std::vector<int> perm((size_t)nb * ML_BATCH, -1);
+  uint64_t st = seed ^ 0x243f6a8885a308d3ull;
+  auto next = [&]() {  // splitmix64
+    st += 0x9e3779b97f4a7c15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  float* lossacc = c->tloss + ML_TRAIN_BLOCKS;
+  double last = 0.0;
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (int k = 0; k < ns; ++k) perm[k] = k;
+    for (int k = ns - 1; k > 0; --k) std::swap(perm[k], perm[next() % (uint64_t)(k + 1)]);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream,
+                       reinterpret_cast<int*>(lossacc), 0, 1);
+    for (int b = 0; b < nb; ++b) {
+      HIPCHK(hipMemcpyAsync(c->tperm, perm.data() + (size_t)b * ML_BATCH, ML_BATCH * 4,
+                            hipMemcpyHostToDevice, c->stream));
+      if (int rc = ml_minibatch(c, 1.0f / ML_BATCH, nullptr, true, (float)lr, lossacc)) return rc;
+    }
+    float l = 0.0f;
+    HIPCHK(hipMemcpyAsync(&l, lossacc, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    last = (double)l / nb;
+  }
+  if (loss_out) *loss_out = last;
+  return 0;
+}
+
+int adcgpu_ml_grad(adcgpu_ctx* c, int n, const float* x, const float* y, const float* w,
+                   float* grad, double* loss) {
+  if (!c || n <= 0 || n > ML_BATCH || !x || !y || !w || !grad)
+    return fail(ADCGPU_E_ARG, "invalid argument");
+  if (int rc = ml_setup(c)) return rc;
+  std::vector<uint16_t> xb((size_t)n * ML_F);
+  for (size_t k = 0; k < xb.size(); ++k) xb[k] = f32_to_bf16(x[k]);
+  std::vector<int> node(n, 0), perm(ML_BATCH, -1);
+  for (int k = 0; k < n; ++k) perm[k] = k;
+  HIPCHK(hipMemcpyAsync(c->smp.x, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->smp.y, y, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->smp.node, node.data(), (size_t)n * 4, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(c->twt, w, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->tperm, perm.data(), ML_BATCH * 4, hipMemcpyHostToDevice, c->stream));
+  float* lossacc = c->tloss + ML_TRAIN_BLOCKS;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<int*>(lossacc),
+                     0, 1);
+  float* gdev = c->tda1;  // free once the minibatch's dZ1 pass has read it
+  if (int rc = ml_minibatch(c, 1.0f / n, gdev, false, 0.0f, lossacc)) return rc;
+  float l = 0.0f;
+  HIPCHK(hipMemcpyAsync(grad, gdev, (size_t)ML_NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&l, lossacc, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (loss) *loss = l;
+  return 0;
+}
+
+int adcgpu_ml_features(adcgpu_ctx* c, float* x_out, double* s_out) {
+  if (!c || !x_out) return fail(ADCGPU_E_ARG, "null argument");
+  if (int rc = ml_setup(c)) return rc;
+  if (c->np > c->smp.cap) return fail(ADCGPU_E_CAPACITY, "more nodes than the sample ring holds");
+  // the state at the start of a step's solve: eta^k is ETA2 between steps
+  const MlIn in = ml_in(c, c->eta2);
+  hipLaunchKernelGGL(k_ml_collect, dim3(nblk(c->np, 128)), dim3(256), 0, c->stream, in, c->smp,
+                     0, c->np, -1);
+  std::vector<uint16_t> xb((size_t)c->np * ML_F);
+  std::vector<float> s(c->np);
+  HIPCHK(hipMemcpyAsync(xb.data(), c->smp.x, xb.size() * 2, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(s.data(), c->smp.s, (size_t)c->np * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < xb.size(); ++k) x_out[k] = bf16_to_f32(xb[k]);
+  if (s_out)
+    for (int i = 0; i < c->np; ++i) s_out[i] = s[i];
+  return check_launch("ml features");
+}
+
+int adcgpu_ml_correction(adcgpu_ctx* c, double* corr_out) {
+  if (!c || !corr_out) return fail(ADCGPU_E_ARG, "null argument");
+  if (int rc = ml_setup(c)) return rc;
+  const MlIn in = ml_in(c, c->eta2);
+  hipLaunchKernelGGL(k_ml_infer, dim3(std::min(nblk(c->np, 128), 4 * 256)), dim3(256), 0,
+                     c->stream, in, c->mlw, c->ml_corr);
+  HIPCHK(hipMemcpyAsync(corr_out, c->ml_corr, (size_t)c->np * 8, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return check_launch("ml correction");
+}
+
+// ---- whole-state snapshot (between steps)
+namespace {
+struct StatePart {
+  void* dev;
+  size_t bytes;
+};
+std::vector<StatePart> state_parts(adcgpu_ctx* c) {
+  const size_t np = c->np, ne = c->ne, mn = (size_t)c->mnei * np;
+  std::vector<StatePart> v;
+  double* nodal[] = {c->eta1, c->eta2, c->etas, c->uu1, c->vv1, c->uu2, c->vv2, c->qx1, c->qy1,
+                     c->qx2, c->qy2, c->h1, c->h2, c->tk, c->etamax, c->totalarea, c->wsx1,
+                     c->wsy1, c->wsx2, c->wsy2, c->pr1, c->pr2, c->gwce_lv, c->etas_prev,
+                     c->umax};
+  for (double* p : nodal) v.push_back({p, np * 8});
+  int* inodal[] = {c->nodecode, c->nnodecode, c->mju, c->etamax_it, c->umax_it};
+  for (int* p : inodal) v.push_back({p, np * 4});
+  v.push_back({c->noff, ne * 4});
+  v.push_back({c->noffold, ne * 4});
+  v.push_back({c->alphal, ne * 8});
+  v.push_back({c->coef, mn * 8});
+  v.push_back({c->obc, (size_t)std::max(c->neta, 1) * c->mnei * 8});
+  v.push_back({c->scal, S_COUNT * 8});
+  v.push_back({c->jsaved, 12 * 8});
+  v.push_back({c->flags, F_COUNT * 4});
+  return v;
+}
+}  // namespace
+
+int64_t adcgpu_state_bytes(adcgpu_ctx* c) {
+  if (!c) return -1;
+  int64_t n = 16;  // header: NP, NE, MNEI, the step of the snapshot
+  for (const StatePart& p : state_parts(c)) n += (int64_t)p.bytes;
+  return n;
+}
+
+int adcgpu_state_save(adcgpu_ctx* c, void* buf, int64_t bytes) {
+  if (!c || !buf) return fail(ADCGPU_E_ARG, "null argument");
+  if (bytes != adcgpu_state_bytes(c)) return fail(ADCGPU_E_ARG, "snapshot size");
+  HIPCHK(hipSetDevice(c->device));
+  // the off-diagonals a deferred solve left scaled are restored first (the
+  // same operations the next matrix pass would apply)
+  if (int rc = restore_coef(c)) return rc;
+  char* o = static_cast<char*>(buf);
+  const int32_t hdr[4] = {c->np, c->ne, c->mnei, c->cur_it};
+  std::memcpy(o, hdr, 16);
+  o += 16;
+  for (const StatePart& p : state_parts(c)) {
+    HIPCHK(hipMemcpyAsync(o, p.dev, p.bytes, hipMemcpyDeviceToHost, c->stream));
+    o += p.bytes;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int adcgpu_state_load(adcgpu_ctx* c, const void* buf, int64_t bytes) {
+  if (!c || !buf) return fail(ADCGPU_E_ARG, "null argument");
+  if (bytes != adcgpu_state_bytes(c)) return fail(ADCGPU_E_ARG, "snapshot size");
+  const char* o = static_cast<const char*>(buf);
+  int32_t hdr[4];
+  std::memcpy(hdr, o, 16);
+  if (hdr[0] != c->np || hdr[1] != c->ne || hdr[2] != c->mnei)
+    return fail(ADCGPU_E_ARG, "snapshot of another mesh");
+  o += 16;
+  HIPCHK(hipSetDevice(c->device));
+  for (const StatePart& p : state_parts(c)) {
+    HIPCHK(hipMemcpyAsync(p.dev, o, p.bytes, hipMemcpyHostToDevice, c->stream));
+    o += p.bytes;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->cur_it = hdr[3];
+  c->coef_scaled = false;
+  c->ncele_fresh = false;
+  return 0;
+}
+
+int adcgpu_set_wait_limit(double seconds) {
+  if (!(seconds >= 0.0 && seconds <= 3600.0)) return fail(ADCGPU_E_ARG, "wait limit");
+  g_wait_seconds = seconds;
   return 0;
 }
 
@@ -1061,10 +1490,21 @@ int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double*
   HIPCHK(hipMemcpyAsync(c->nneigh, nn.data(), n * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->gwce_lv, rhs, n * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->etas, u, n * 8, hipMemcpyHostToDevice, c->stream));
+  // this call's outcome only: a failed earlier call must not leave its
+  // error, iteration count or ier behind
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags, 0, F_COUNT);
   // the device matrix is stored with leading dimension n
   int rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, n, itmax, rparm[0],
                       nullptr, nullptr, nullptr, false, false);
   if (rc) return rc;
+  {
+    int fe[F_COUNT];
+    HIPCHK(hipMemcpyAsync(fe, c->flags, sizeof(fe), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // the device copies are left as the failed launch left them; the
+    // caller's arrays are untouched
+    if (int erc = flag_errors(fe)) return erc;
+  }
   double saved[8];
   int f[F_COUNT];
   HIPCHK(hipMemcpyAsync(hc.data(), c->coef, hc.size() * 8, hipMemcpyDeviceToHost, c->stream));

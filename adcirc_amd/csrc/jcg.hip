@@ -65,7 +65,14 @@ __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& gen, int* err
     }
     const long long t0 = wall_clock64();
     unsigned polls = 0;
-    while (__hip_atomic_load(genw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
+    // a bound of zero allows no wait at all: every barrier fails at once (the
+    // host forces the error path this way)
+    if (ticks <= 0) {
+      atomicOr(err, ERR_BARRIER);
+      failed = 1;
+    }
+    while (!failed &&
+           __hip_atomic_load(genw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
       __builtin_amdgcn_s_sleep(1);
       if ((++polls & 63u) == 0) {
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER) {
@@ -104,6 +111,10 @@ __device__ __forceinline__ bool halo_sync(unsigned* hflags, int tile, int lo, in
     failed = 0;
     const long long t0 = wall_clock64();
     unsigned polls = 0;
+    if (ticks <= 0) {
+      atomicOr(err, ERR_BARRIER);
+      failed = 1;
+    }
     for (int q = lo; q <= hi && !failed; ++q) {
       if (q == tile) continue;
       while (__hip_atomic_load(hflags + JCG_HALO_STRIDE * q, __ATOMIC_RELAXED,
@@ -295,6 +306,11 @@ __device__ void jcg_parcon(ItLocal& L, double dtnrm) {
 }
 
 
+// DUAL: two initial guesses, a.u (the reference's) and a.u2 (the learned
+// one); the pseudo-residuals of both are formed in the scaling pass and the
+// iteration starts from the one with the smaller norm (the reference's on a
+// tie).  Everything after that choice is the same ITPACK iteration.
+template <bool DUAL>
 __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   __shared__ ItLocal L;
   __shared__ double lds[12];
@@ -381,6 +397,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
 
   // ---- scal, part 2: scale off-diagonals; initial pseudo-residual
   // d0 = rhs - N u - u (src/itpackv.F:2394-2400, 346-353)
+  double u2c[DUAL ? RED_PER_THREAD : 1], d2c[DUAL ? RED_PER_THREAD : 1];
   if (!eig_block) {
     double wi[RED_PER_THREAD];
 #pragma unroll
@@ -388,9 +405,17 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       const int i = base + RED_THREADS * r;
       wi[r] = 0.0;
       dc[r] = 0.0;
+      if (DUAL) {
+        u2c[r] = 0.0;
+        d2c[r] = 0.0;
+      }
       if (i < np) {
         wi[r] = at(a.w, i);
         dc[r] = at(a.rhs, i);
+        if (DUAL) {
+          u2c[r] = at(a.u2, i);
+          d2c[r] = dc[r];
+        }
       }
     }
     // the next slot's column index and coefficient are loaded while this
@@ -432,10 +457,12 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
             const double cs = kc[r - h] * wi[r] * at(a.w, cc[r - h]);
             at(a.cs, q0 + RED_THREADS * r) = cs;
             dc[r] = dc[r] - cs * at(a.u, cc[r - h]);
+            if (DUAL) d2c[r] = d2c[r] - cs * at(a.u2, cc[r - h]);
           }
         }
       }
     }
+    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int r = 0; r < RED_PER_THREAD; ++r) {
       const int i = base + RED_THREADS * r;
@@ -443,9 +470,68 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       dp[r] = 0.0;
       up[r] = 0.0;
       if (i < np) at(a.d, i) = dc[r];
+      if (DUAL) {
+        d2c[r] = d2c[r] - u2c[r];
+        if (i < np) {
+          at(a.d2, i) = d2c[r];
+          s0 = s0 + dc[r] * dc[r];
+          s1 = s1 + d2c[r] * d2c[r];
+        }
+      }
+    }
+    if (DUAL) {
+      // chunk values of |d0|^2 for both starts, in the inner-product tree,
+      // in the second parity slot of the partials (the first iteration
+      // writes the first)
+      s0 = wave_sum(s0);
+      s1 = wave_sum(s1);
+      __shared__ double wd[JCG_CPB][16];
+      const int tt = t % RED_THREADS;
+      double* ws = wd[t / RED_THREADS];
+      if ((tt & 63) == 0) {
+        ws[tt >> 6] = s0;
+        ws[8 + (tt >> 6)] = s1;
+      }
+      __syncthreads();
+      if (tt < 2 && c < nch) {
+        const double* v = ws + 8 * tt;
+        a.part[(size_t)nch * 4 + (size_t)c * 4 + tt] =
+            ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+      }
     }
   }
   if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+  // the start with the smaller initial pseudo-residual
+  double* dv = a.d;
+  if (DUAL) {
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    const double* part = a.part + (size_t)nch * 4;
+    if (t < RED_XTHREADS)
+      for (int cc = t; cc < nch; cc += RED_XTHREADS) {
+        r0 = r0 + part[(size_t)cc * 4];
+        r1 = r1 + part[(size_t)cc * 4 + 1];
+      }
+    block_tree3(r0, r1, r2, lds);
+    __shared__ int use2;
+    if (t == 0) {
+      use2 = r1 < r0 ? 1 : 0;
+      if (blockIdx.x == 0) {
+        a.saved[10] = r0;
+        a.saved[11] = r1;
+        if (use2) a.flags[F_MLUSED] += 1;
+      }
+    }
+    __syncthreads();
+    if (use2) {
+      dv = a.d2;
+      if (!eig_block)
+#pragma unroll
+        for (int r = 0; r < RED_PER_THREAD; ++r) {
+          uc[r] = u2c[r];
+          dc[r] = d2c[r];
+        }
+    }
+  }
 
   // ---- iterations
   for (int in = 0; in <= a.itmax; ++in) {
@@ -497,7 +583,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
           }
 #pragma unroll
           for (int r = h; r < h + 4; ++r)
-            if (j < NN(r)) nd[r] = nd[r] - sc[r - h] * at(a.d, cc[r - h]);
+            if (j < NN(r)) nd[r] = nd[r] - sc[r - h] * at(dv, cc[r - h]);
         }
       }
       double add = 0.0, adt = 0.0, auu = 0.0;
@@ -582,7 +668,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         dc[r] = dn;
         // written through to memory: the next grid barrier's release then
         // has no pseudo-residual lines to write back from this XCD's L2
-        if (i < np) __hip_atomic_store(&at(a.d, i), dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (i < np) __hip_atomic_store(&at(dv, i), dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     if (a.halo < 0) {
@@ -642,7 +728,11 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
     a.saved[4] = L.cme;
     a.saved[5] = L.stptst;
     a.flags[F_ITSUM] += in_final;
+    if (!halted) a.flags[F_NITMAX] += 1;
   }
 }
+
+template __global__ void k_jcg<false>(JcgArgs);
+template __global__ void k_jcg<true>(JcgArgs);
 
 }  // namespace adcgpu

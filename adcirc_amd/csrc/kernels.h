@@ -42,8 +42,13 @@ constexpr int WD_LINK_LDS = WD_LINK_BLOCKS * WD_XMAX * 8 > WD_SERIAL_LDS
 
 // device flag slots
 enum { F_NCCHANGE = 0, F_NUMITR = 2, F_IER, F_ERROR, F_NCAND, F_ITSUM, F_NREBUILD, F_WDSERIAL,
+       F_ERRSTEP,  // first step that raised ERR_ELEV or ERR_NONFINITE (0: none)
+       F_NITMAX,   // solves that reached ITMAX (ier = 13)
+       F_ALPHA,    // some element's ALPHAL differs from 1 (the cached matrix
+                   // terms assume ALPHAL = 1; the element pass recomputes them)
+       F_MLUSED,   // solves that started from the learned guess
        F_COUNT = 16 };
-enum { ERR_DIAG = 1, ERR_BARRIER = 4 };
+enum { ERR_DIAG = 1, ERR_BARRIER = 4, ERR_ELEV = 8, ERR_NONFINITE = 16 };
 // device scalar slots
 enum { S_EP = 0, S_COUNT = 8 };
 
@@ -90,6 +95,14 @@ struct DevState {
   int *etamax_it;  // step of the last ETAMAX rise (0: never)
   double *umax;    // peak depth-averaged speed at wet nodes
   int *umax_it;    // step of the last UMAX rise (0: never)
+  unsigned char* ncele;  // [ne] NCEle of each element for the next GWCE matrix
+};
+
+// The GWCE matrix terms of every (element, vertex) slot that do not change
+// from step to step (geometry, depth, Tau0, ALPHAL = 1), kept in slot order;
+// a rebuild multiplies them by the element's NCEle.
+struct LhsTerms {
+  const double *td, *ta, *tb;  // [3 ne] diagonal and the two off-diagonal terms
 };
 
 struct RhsPrm {
@@ -102,7 +115,9 @@ struct RhsPrm {
 struct MomPrm {
   double ifnlct;
   int nws;
-  int it;  // this step, recorded where ETAMAX rises
+  int it;            // this step, recorded where ETAMAX rises
+  double errorelev;  // ErrorElev: |ETA2| above it at a wet node stops the run
+  int* flags;        // run-time error flags (ERR_ELEV, ERR_NONFINITE, F_ERRSTEP)
 };
 
 struct WdPrm {
@@ -166,6 +181,8 @@ struct JcgArgs {
   double* eta2;       // optional: ETA2 = NODECODE*(ETAS+0)+ETA1
   const double* eta1;
   const int* nodecode;
+  const double* u2;   // k_jcg<true>: the second (learned) start, scaled like u
+  double* d2;         //              and its initial pseudo-residual
 };
 
 __global__ void k_init_h(int np, const double* dp, const double* eta2, double ifnlfa, double* h1,
@@ -182,8 +199,14 @@ __global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, 
                            double* sb);
 template <bool LUMPED>
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
-                           const double* sa, const double* sb, double* ep_part,
+                           const double* sa, const double* sb, LhsTerms lt, double* ep_part,
                            const double* restore_cs, const double* restore_w);
+// the cached terms (once per context): k_lhs_elem's terms with NCEle = 1 and
+// ALPHAL = 1
+template <bool LUMPED>
+__global__ void k_lhs_terms(DevMesh m, double* td, double* ta, double* tb);
+// NCEle of every element from the current NODECODE and NOFF
+__global__ void k_ncele(DevMesh m, DevState s);
 __global__ void k_coef_restore(DevMesh m, DevState s, const double* cs, const double* w);
 // one block more than the elements need: the last finishes the GWCE matrix
 __global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags, double* contrib, int nch,
@@ -194,12 +217,18 @@ struct JcgPre {
   unsigned* barrier;  // grid barrier words of the coming k_jcg launch, cleared here
   int barrier_words;
   int* flags;
-  double* etas_prev;  // the previous step's ETAS (warm start only)
-  int warm;           // 1: start JCG from 2*ETAS - ETAS_prev instead of ETAS
+  double* etas_prev;  // the previous step's ETAS (kept when keep_prev != 0)
+  int warm;           // 1: start JCG from 2*ETAS - ETAS_prev instead of ETAS;
+                      // 2: also form the learned start ETAS + ml_corr into ml_u
+  int keep_prev;      // record this step's starting ETAS in etas_prev
+  const double* ml_corr;  // warm == 2: the network's correction per node
+  double* ml_u;           // warm == 2: the learned start, scaled
 };
 template <bool LUMPED>
 __global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const double* eta_bc,
                            const double* scal, JcgPre pre);
+// DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2
+template <bool DUAL>
 __global__ void k_jcg(JcgArgs a);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);

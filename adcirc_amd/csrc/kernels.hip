@@ -110,25 +110,26 @@ __global__ __launch_bounds__(256) void k_friction_holland(int np, FricPrm f, dou
 // GWCE matrix, element part (src/gwce.F:456-592): for element e and its local
 // vertex k, the diagonal term and the two off-diagonal terms of row NM(e,k),
 // written to the row's incidence slot p = slot[3e+k] (sd, sa, sb).
+// The three terms of element e's vertex k before the NCEle factor: diagonal
+// (MsOn + GA*(FDXk*FDXk + FDYk*FDYk)) and the off-diagonals in the
+// reference's local order (k+1, k+2 mod 3 map to CSR_POSITION(e,k,1) and
+// (e,k,2)); the lumped LHS has the diagonal MsFacLOnDiag*SPM with OnDiag = 4
+// only (src/gwce.F:596-635).  `al` is the element's ALPHAL.
 template <bool LUMPED>
-__global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
-                                                  const int* __restrict__ flags,
-                                                  double* __restrict__ sd, double* __restrict__ sa,
-                                                  double* __restrict__ sb) {
-  if (flags[F_NCCHANGE] == 0) return;
-  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e >= m.ne) return;
+__device__ __forceinline__ void lhs_elem_terms(const DevMesh& m, int e, double al, double td[3],
+                                               double ta[3], double tb[3]) {
   const double dt = m.dt;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
-  const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
-  const double NCELE = (double)ncele;
   if (LUMPED) {
-    // COEFD: MsFacLOnDiag*SPM*NCELE with OnDiag = 4 (src/gwce.F:596-635)
     const double area = m.areas[e] * 0.5;
     const double t0avg = (m.tau0[n1] + m.tau0[n2] + m.tau0[n3]) / 3.0;
     const double mson = (4.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) sd[m.slot[3 * e + k]] = mson * 1.0 * NCELE;
+    for (int k = 0; k < 3; ++k) {
+      td[k] = mson * 1.0;
+      ta[k] = 0.0;
+      tb[k] = 0.0;
+    }
     return;
   }
   const double fx[3] = {m.fdx[3 * e], m.fdx[3 * e + 1], m.fdx[3 * e + 2]};
@@ -137,24 +138,76 @@ __global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
   const double area = area2 * 0.5;
   const double area4 = area2 * 2.0;
   const double dpavg = (m.dp[n1] + m.dp[n2] + m.dp[n3]) / 3.0;
-  const double ga = s.alphal[e] * m.a00 * m.g * dpavg * 1.0 / area4;
+  const double ga = al * m.a00 * m.g * dpavg * 1.0 / area4;
   const double t0avg = (m.tau0[n1] + m.tau0[n2] + m.tau0[n3]) / 3.0;
   const double mson = (2.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
   const double msoff = (1.0 * area * (1.0 / dt + t0avg / 2.0)) / dt / 12.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    // diagonal: (MsOn + GA*(FDXk*FDXk + FDYk*FDYk))*NCELE; off-diagonals in
-    // the reference's local order (k+1, k+2 mod 3 map to CSR_POSITION(e,k,1)
-    // and (e,k,2))
     const int ka = (k == 0) ? 1 : 0, kb = (k == 2) ? 1 : 2;
+    td[k] = mson * 1.0 + ga * (1.0 * fx[k] * fx[k] + 1.0 * fy[k] * fy[k]);
+    ta[k] = msoff * 1.0 + ga * (1.0 * fx[k] * fx[ka] + 1.0 * fy[k] * fy[ka]);
+    tb[k] = msoff * 1.0 + ga * (1.0 * fx[k] * fx[kb] + 1.0 * fy[k] * fy[kb]);
+  }
+}
+
+// Element pass of a rebuild, used only while some ALPHAL differs from 1
+// (F_ALPHA); otherwise the row pass multiplies the cached terms by NCEle.
+template <bool LUMPED>
+__global__ __launch_bounds__(256) void k_lhs_elem(DevMesh m, DevState s,
+                                                  const int* __restrict__ flags,
+                                                  double* __restrict__ sd, double* __restrict__ sa,
+                                                  double* __restrict__ sb) {
+  if (flags[F_NCCHANGE] == 0 || flags[F_ALPHA] == 0) return;
+  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
+  const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
+  const double NCELE = (double)ncele;
+  double td[3], ta[3], tb[3];
+  lhs_elem_terms<LUMPED>(m, e, s.alphal[e], td, ta, tb);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
     const int p = m.slot[3 * e + k];
-    sd[p] = (mson * 1.0 + ga * (1.0 * fx[k] * fx[k] + 1.0 * fy[k] * fy[k])) * NCELE;
-    sa[p] = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[ka] + 1.0 * fy[k] * fy[ka])) * NCELE;
-    sb[p] = (msoff * 1.0 + ga * (1.0 * fx[k] * fx[kb] + 1.0 * fy[k] * fy[kb])) * NCELE;
+    sd[p] = td[k] * NCELE;
+    if (!LUMPED) {
+      sa[p] = ta[k] * NCELE;
+      sb[p] = tb[k] * NCELE;
+    }
   }
 }
 template __global__ void k_lhs_elem<false>(DevMesh, DevState, const int*, double*, double*, double*);
 template __global__ void k_lhs_elem<true>(DevMesh, DevState, const int*, double*, double*, double*);
+
+// The terms with NCEle = 1 and ALPHAL = 1, in slot order, once per context.
+// Multiplying one by NCEle (0 or 1) later gives exactly the element pass's
+// value: x * 1 = x, and x * 0 is a zero that leaves every node sum unchanged
+// (the sums start at +0).
+template <bool LUMPED>
+__global__ __launch_bounds__(256) void k_lhs_terms(DevMesh m, double* __restrict__ td,
+                                                   double* __restrict__ ta,
+                                                   double* __restrict__ tb) {
+  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double d[3], a[3], b[3];
+  lhs_elem_terms<LUMPED>(m, e, 1.0, d, a, b);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int p = m.slot[3 * e + k];
+    td[p] = d[k];
+    ta[p] = a[k];
+    tb[p] = b[k];
+  }
+}
+template __global__ void k_lhs_terms<false>(DevMesh, double*, double*, double*);
+template __global__ void k_lhs_terms<true>(DevMesh, double*, double*, double*);
+
+__global__ __launch_bounds__(256) void k_ncele(DevMesh m, DevState s) {
+  const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
+  s.ncele[e] = (unsigned char)(s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e]);
+}
 
 // GWCE matrix, row part: row i of COEF is the sum of its incidence slots in
 // ascending element order, each row held in LDS while it is formed (the
@@ -167,6 +220,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
                                                           const double* __restrict__ sd,
                                                           const double* __restrict__ sa,
                                                           const double* __restrict__ sb,
+                                                          LhsTerms lt,
                                                           double* __restrict__ ep_part,
                                                           const double* __restrict__ restore_cs,
                                                           const double* __restrict__ restore_w) {
@@ -193,6 +247,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
       }
     return;
   }
+  // the cached terms times NCEle, unless some ALPHAL differs from 1 (then
+  // the element pass has written the slot values); the lumped matrix has no
+  // ALPHAL term
+  const bool cached = LUMPED || flags[F_ALPHA] == 0;
   double acc = 0.0;
   for (int r = 0; r < RED_PER_THREAD; ++r) {
     const int i = base + RED_THREADS * r;
@@ -201,7 +259,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
     if (LUMPED) {
       // COEFD only; the off-diagonal columns stay zero
       double diag = 0.0;
-      for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) diag = diag + sd[p];
+      for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p)
+        diag = diag + lt.td[p] * (double)s.ncele[(unsigned)m.inc[p] / 3u];
       s.coef[i] = diag;
       for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = 0.0;
       acc = acc + diag * diag;
@@ -212,9 +271,20 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
     for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
       const int jab = m.slot_cols[p];
       const int ja = jab & 0xffff, jb = jab >> 16;
-      diag = diag + sd[p];
-      racc[ja][t] = racc[ja][t] + sa[p];
-      racc[jb][t] = racc[jb][t] + sb[p];
+      double vd, va, vb;
+      if (cached) {
+        const double nce = (double)s.ncele[(unsigned)m.inc[p] / 3u];
+        vd = lt.td[p] * nce;
+        va = lt.ta[p] * nce;
+        vb = lt.tb[p] * nce;
+      } else {
+        vd = sd[p];
+        va = sa[p];
+        vb = sb[p];
+      }
+      diag = diag + vd;
+      racc[ja][t] = racc[ja][t] + va;
+      racc[jb][t] = racc[jb][t] + vb;
     }
     s.coef[i] = diag;
     for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = racc[j][t];
@@ -223,8 +293,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
   double tot = chunk_tree(acc, lds);
   if (t == 0) ep_part[chunk] = tot;
 }
-template __global__ void k_lhs_rows<false>(DevMesh, DevState, const int*, const double*, const double*, const double*, double*, const double*, const double*);
-template __global__ void k_lhs_rows<true>(DevMesh, DevState, const int*, const double*, const double*, const double*, double*, const double*, const double*);
+template __global__ void k_lhs_rows<false>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
+template __global__ void k_lhs_rows<true>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
 
 // EP = sqrt(sum of squared diagonals / NP) from the chunk partials, and the
 // elevation-boundary rows (src/gwce.F:700-743): a boundary row becomes EP on
@@ -446,6 +516,7 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
   if (bi >= 0) {
     const double e2 = eta_bc[bi];
     const double es = e2 - s.eta1[i];
+    if (pre.keep_prev) pre.etas_prev[i] = s.etas[i];
     s.eta2[i] = e2;
     u = es;
     // lumped: the boundary row's own COEFD (src/gwce.F:1856-1862)
@@ -465,13 +536,13 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
     }
     rhs = acc;
     u = s.etas[i];
-    if (pre.warm) {
+    const double u_ref = u;
+    if (pre.warm == 1) {
       // the increment extrapolated from the last two: ETA2 extrapolated to
       // second order instead of the first order the previous increment gives
-      const double up = pre.etas_prev[i];
-      pre.etas_prev[i] = u;
-      u = 2.0 * u - up;
+      u = 2.0 * u - pre.etas_prev[i];
     }
+    if (pre.keep_prev) pre.etas_prev[i] = u_ref;
     s.eta2[i] = 0.0;
   }
   // a zero diagonal (a dry row) becomes EP (src/gwce.F:737-741); then the
@@ -500,6 +571,8 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
   const double sq = __builtin_sqrt(dg);
   const double w = 1.0 / sq;
   s.etas[i] = u * sq;
+  // the learned start (zero correction on boundary and dry rows), scaled
+  if (pre.warm == 2) pre.ml_u[i] = (u + pre.ml_corr[i]) * sq;
   pre.w[i] = w;
   s.gwce_lv[i] = rhs * w;
 }
@@ -510,6 +583,8 @@ template __global__ void k_rhs_node<true>(DevMesh, DevState, const double*, cons
 // D1 (src/wetdry.F:234-285); also NNODECODE := NODECODE for the W1 writes.
 __global__ __launch_bounds__(256) void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin) {
   int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  // this step's ALPHAL pass sets it again where needed
+  if (i == 0) flags[F_ALPHA] = 0;
   if (i >= m.np) return;
   int nc = s.nodecode[i];
   if (nc == 1) {
@@ -1054,6 +1129,8 @@ __device__ __forceinline__ double wd_alpha(const DevMesh& m, const DevState& s, 
     }
   }
   s.alphal[e] = al;
+  // the cached GWCE matrix terms hold for ALPHAL = 1 only
+  if (al != 1.0) atomicOr(&flags[F_ALPHA], 1);
   return al;
 }
 
@@ -1084,6 +1161,8 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
   const double THIRD = 1.0 / 3.0, DT = m.dt, GO2 = m.g / 2.0;
   const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
   const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
+  // the flags are final for this step: NCEle as the next GWCE matrix sees it
+  s.ncele[e] = (unsigned char)ncele;
   // dry element: every contribution is NCEle*DT x (...) = 0 (see k_rhs_elem)
   double tx[3] = {0.0, 0.0, 0.0}, ty[3] = {0.0, 0.0, 0.0};
   if (ncele != 0) {
@@ -1245,6 +1324,16 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
   s.qx2[i] = u2 * h2;
   s.qy2[i] = v2 * h2;
   const double e2 = s.eta2[i];
+  // run-time checks of the reference: ErrorElev over the wet nodes
+  // (src/timestep.F:1242-1251, 1381-1395) and a NaN velocity anywhere
+  // (src/timestep.F:1413-1471; an infinite one counts too).  The first step
+  // that raises either is kept.
+  const bool elev_bad = nci == 1 && fabs(e2) > q.errorelev;
+  const bool vel_bad = !(fabs(u2) <= 1.7976931348623157e308 && fabs(v2) <= 1.7976931348623157e308);
+  if (elev_bad || vel_bad) {
+    atomicOr(&q.flags[F_ERROR], (elev_bad ? ERR_ELEV : 0) | (vel_bad ? ERR_NONFINITE : 0));
+    atomicCAS(&q.flags[F_ERRSTEP], 0, q.it > 0 ? q.it : 1);
+  }
   // peak surge and its step (src/write_output.F:6357-6364; the host turns
   // the step into ETAMAX_Time = IT*DTDP + STATIM*86400)
   if (nci == 1 && e2 > s.etamax[i]) {

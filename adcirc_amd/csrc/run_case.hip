@@ -150,6 +150,7 @@ int main(int argc, char** argv) {
   p.nolibf = I("nolibf"); p.itmax = I("itmax"); p.nws = I("nws"); p.precision = I("precision");
   p.ilump = c.count("ilump") ? I("ilump") : 0;
   p.warm_start = c.count("warm_start") ? I("warm_start") : 0;
+  p.errorelev = c.count("errorelev") ? D("errorelev") : 0.0;
   p.tau0var = DA("tau0var"); p.evm = DA("evm"); p.fric = DA("fric"); p.corif = DA("corif");
 
   const std::vector<double>& table = need(c, "eta_bc_table", 0)->d;

@@ -32,6 +32,7 @@ module adcgpu_iface
      type(c_ptr) :: tau0var, evm, fric, corif
      integer(c_int32_t) :: ilump
      integer(c_int32_t) :: warm_start
+     real(c_double) :: errorelev
   end type
 
   !> the context of this process (one GPU per process)

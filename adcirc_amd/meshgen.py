@@ -55,6 +55,7 @@ class RunConfig:
     convcr: float = 1.0e-8
     ilump: int = 0                # GWCE LHS: 0 consistent (JCG), 1 lumped (explicit)
     warm_start: int = 0           # JCG start: 0 previous increment, 1 extrapolated increment
+    errorelev: float = 1000.0     # ErrorElev: |ETA2| above it at a wet node stops the run
     anginn: float = 110.0
     g: float = 9.80665
     # periodic elevation forcing on the open boundary

@@ -42,7 +42,8 @@ def c_layout(tables: Dict[str, np.ndarray], run, tau0var=None, evm=None, fric=No
           "nolifa": run.nolifa, "nolica": run.nolica, "nolicat": run.nolicat,
           "nolibf": run.nolibf, "itmax": run.itmax, "nws": run.nws,
           "ilump": int(getattr(run, "ilump", 0)),
-          "warm_start": int(getattr(run, "warm_start", 0))}
+          "warm_start": int(getattr(run, "warm_start", 0)),
+          "errorelev": float(getattr(run, "errorelev", 0.0))}
     i32 = lambda a: np.ascontiguousarray(a, np.int32)  # noqa: E731
     f64 = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
     ar = {
@@ -67,7 +68,8 @@ def c_layout(tables: Dict[str, np.ndarray], run, tau0var=None, evm=None, fric=No
 _MESH_I = ("np", "ne", "mnei", "neta", "nvel", "nvelme")
 _MESH_A = ("nm", "x", "y", "dp", "fdxe", "fdye", "areas", "nneigh", "neitab", "csr_position",
            "nbd", "nbv", "lbcodei", "csii", "siii", "me2gw")
-_PRM_D = ("dtdp", "g", "h0", "velmin", "a00", "b00", "c00", "convcr", "prbckgrnd_mh2o")
+_PRM_D = ("dtdp", "g", "h0", "velmin", "a00", "b00", "c00", "convcr", "prbckgrnd_mh2o",
+          "errorelev")
 _PRM_I = ("nolifa", "nolica", "nolicat", "nolibf", "itmax", "nws", "ilump", "warm_start")
 _PRM_A = ("tau0var", "evm", "fric", "corif")
 
@@ -257,9 +259,76 @@ class Model:
             hol = np.ascontiguousarray(list(holland) + [0.0] * (13 - len(holland)), np.float64)
         rc = self.lib.adcgpu_run(self.h, it0, nsteps, _p(tab, dp_),
                                  _p(hol, dp_) if hol is not None else None, int(use_graph))
+        self.last_run_rc = rc
         check(rc, "run")
         self.last_rebuilds = int(self.lib.adcgpu_run_rebuilds(self.h))
         return int(self.lib.adcgpu_run_iterations(self.h))
+
+    def run_stats(self):
+        """(JCG iterations, matrix rebuilds, solves that reached ITMAX, first
+        step that raised a run-time error or 0, solves that started from the
+        learned guess) of the last resident run."""
+        out = np.zeros(5, np.int64)
+        check(self.lib.adcgpu_run_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))),
+              "run_stats")
+        return tuple(int(v) for v in out)
+
+    # ------------------------------------------------------------- snapshot
+    def save_state(self) -> np.ndarray:
+        """The whole state between steps (adcgpu_state_save), as bytes."""
+        n = int(self.lib.adcgpu_state_bytes(self.h))
+        buf = np.empty(n, np.uint8)
+        check(self.lib.adcgpu_state_save(self.h, buf.ctypes.data, n), "state_save")
+        return buf
+
+    def load_state(self, buf: np.ndarray) -> None:
+        buf = np.ascontiguousarray(buf, np.uint8)
+        check(self.lib.adcgpu_state_load(self.h, buf.ctypes.data, buf.size), "state_load")
+
+    # ------------------------------------------------------------- learned start
+    def ml_params(self, new=None) -> np.ndarray:
+        """The network's FP32 parameters (after setting `new`, if given)."""
+        fp = C.POINTER(C.c_float)
+        out = np.zeros(_lib.ML_NPARAM, np.float32)
+        src = None if new is None else np.ascontiguousarray(new, np.float32)
+        check(self.lib.adcgpu_ml_params(self.h, None if src is None else _p(src, fp),
+                                        _p(out, fp)), "ml_params")
+        return out
+
+    def ml_collect(self, per_step: int) -> None:
+        check(self.lib.adcgpu_ml_collect(self.h, int(per_step)), "ml_collect")
+
+    def ml_samples(self) -> int:
+        return int(self.lib.adcgpu_ml_samples(self.h))
+
+    def ml_train(self, epochs: int, lr: float = 1e-3, seed: int = 1) -> float:
+        loss = C.c_double()
+        check(self.lib.adcgpu_ml_train(self.h, int(epochs), float(lr), int(seed),
+                                       C.byref(loss)), "ml_train")
+        return loss.value
+
+    def ml_grad(self, x, y, w):
+        fp = C.POINTER(C.c_float)
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        w = np.ascontiguousarray(w, np.float32)
+        g = np.zeros(_lib.ML_NPARAM, np.float32)
+        loss = C.c_double()
+        check(self.lib.adcgpu_ml_grad(self.h, x.shape[0], _p(x, fp), _p(y, fp), _p(w, fp),
+                                      _p(g, fp), C.byref(loss)), "ml_grad")
+        return loss.value, g
+
+    def ml_features(self):
+        fp = C.POINTER(C.c_float)
+        x = np.zeros((self.np_, 48), np.float32)
+        s = np.zeros(self.np_)
+        check(self.lib.adcgpu_ml_features(self.h, _p(x, fp), _p(s, dp_)), "ml_features")
+        return x, s
+
+    def ml_correction(self) -> np.ndarray:
+        out = np.zeros(self.np_)
+        check(self.lib.adcgpu_ml_correction(self.h, _p(out, dp_)), "ml_correction")
+        return out
 
     def holland(self, t: float, prm) -> None:
         """Device Holland forcing at time t into WSX2/WSY2/PR2."""
@@ -307,14 +376,22 @@ def cold_wetdry_state(tables: Dict[str, np.ndarray], run, eta2=None):
 
 
 def jcg(coef: np.ndarray, jcoef: np.ndarray, rhs: np.ndarray, u: np.ndarray, iparm: np.ndarray,
-        rparm: np.ndarray) -> int:
+        rparm: np.ndarray, raise_errors: bool = True) -> int:
     """Drop-in JCG on host arrays (Fortran layout (ndim, maxnz) passed as the
-    column-major flattening); arrays are updated in place like the reference."""
+    column-major flattening); arrays are updated in place like the reference.
+    Returns ITPACK's ier, or the negative status code when ``raise_errors``
+    is off."""
     lib = _lib.load()
     ndim = rhs.shape[0]
     maxnz = coef.size // ndim
-    rc = lib.adcgpu_jcg(ndim, ndim, maxnz, _p(jcoef, ip_), _p(coef, dp_), _p(rhs, dp_),
-                        _p(u, dp_), _p(iparm, ip_), _p(rparm, dp_))
-    if rc < 0:
+    rc = lib.adcgpu_jcg(ndim, ndim, maxnz, _p(jcoef, ip_), _p(coef, dp_), _p(rhs, dp_), _p(u, dp_),
+            _p(iparm, ip_), _p(rparm, dp_))
+    if rc < 0 and raise_errors:
         check(rc, "jcg")
     return rc
+
+
+def set_wait_limit(seconds: float) -> None:
+    """Bound of every wait inside the persistent JCG kernel (default 0.5 s;
+    0 makes every wait fail at once, which forces the error path)."""
+    check(_lib.load().adcgpu_set_wait_limit(float(seconds)), "set_wait_limit")

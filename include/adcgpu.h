@@ -42,7 +42,10 @@ extern "C" {
 #define ADCGPU_E_HIP (-3)         /* HIP runtime error (message via adcgpu_last_error) */
 #define ADCGPU_E_DIAG (-4)        /* GWCE diagonal <= 0 (src/gwce.F:678-707 terminates) */
 #define ADCGPU_E_CAPACITY (-5)    /* the mesh has more rows than the resident JCG grid holds */
-#define ADCGPU_E_NONFINITE (-6)   /* non-finite JCG residual */
+#define ADCGPU_E_NONFINITE (-6)   /* a non-finite velocity (UU2 or VV2) at some node: the
+                                     reference's NaN velocity check, src/timestep.F:1413-1471 */
+#define ADCGPU_E_ERRORELEV (-7)   /* |ETA2| above ErrorElev at a wet node: the reference
+                                     stops the run, src/timestep.F:1242-1251, 1381-1395 */
 
 typedef struct adcgpu_ctx adcgpu_ctx;
 
@@ -78,9 +81,13 @@ typedef struct {
                                diagonal LHS solved explicitly (src/gwce.F:141-155) */
   int32_t warm_start;       /* JCG initial guess: 0 the previous increment ETAS, as the
                                reference (src/gwce.F:2002-2004); 1 the increment
-                               extrapolated from the last two, 2*ETAS - ETAS_prev (same
-                               stopping test, results within the JCG tolerance, not
-                               bit-identical to the reference) */
+                               extrapolated from the last two, 2*ETAS - ETAS_prev; 2 the
+                               better of ETAS and the learned guess (adcgpu_ml_*).  1 and
+                               2 keep the stopping test: results within the JCG
+                               tolerance, not bit-identical to the reference */
+  double errorelev;         /* ErrorElev (fort.15 namelist WarnElevControl,
+                               src/read_input.F:794-812); 0 selects the reference's
+                               default, 1000 m */
 } adcgpu_params;
 
 #define ADCGPU_FP64 0
@@ -94,6 +101,8 @@ enum adcgpu_field {
   ADCGPU_WSY2, ADCGPU_PR1, ADCGPU_PR2, ADCGPU_GWCE_LV,
   ADCGPU_UMAX,      /* peak sqrt(UU2^2+VV2^2) at wet nodes, -99999 before any
                        (src/write_output.F:6367-6376) */
+  ADCGPU_ETAS_PREV, /* the increment before ETAS (kept when warm_start != 0 or
+                       the learned start is set up) */
   ADCGPU_NODECODE = 100, ADCGPU_NOFF, ADCGPU_MJU,
   ADCGPU_ETAMAX_IT, /* step IT of each node's last ETAMAX rise, 0 if none: the
                        reference's ETAMAX_Time is IT*DTDP + STATIM*86400
@@ -142,19 +151,84 @@ int adcgpu_momentum(adcgpu_ctx *ctx);
  * solve_gwce, wetdry (H2 only when NOLIFA != 2), momentum. */
 int adcgpu_timestep(adcgpu_ctx *ctx, int it, const double *eta_bc, const double *wsx2,
                     const double *wsy2, const double *pr2);
-/* Iterations of the last JCG solve (NUMITR, src/gwce.F:2005) and its ier. */
+/* Iterations of the last JCG solve (NUMITR, src/gwce.F:2005) and its ier.
+ * Also reports the sticky error conditions of the steps run so far
+ * (ADCGPU_E_DIAG, ADCGPU_E_ERRORELEV, ADCGPU_E_NONFINITE, ADCGPU_E_HIP for a
+ * JCG wait that timed out); adcgpu_cold_start clears them. */
 int adcgpu_last_numitr(adcgpu_ctx *ctx, int *numitr, int *ier);
 /* Waits for the context's stream. */
 int adcgpu_synchronize(adcgpu_ctx *ctx);
+
+/* Whole-state snapshot between steps, into / from a host buffer of
+ * adcgpu_state_bytes() bytes: every array a later step reads (time levels,
+ * flags, maxima, the GWCE matrix and its boundary columns, EP, ITPACK's saved
+ * state, the device flags) and the step reached.  Loading it into a context
+ * of the same mesh (the same or another warm_start / precision) continues the
+ * run exactly as the saved context would: the hot start of this library
+ * (the reference's fort.67/68 records, src/write_output.F:4429-5582 and
+ * src/hstart.F, hold less: its hot-started run rebuilds the matrix and starts
+ * ITPACK's saved state afresh). */
+int64_t adcgpu_state_bytes(adcgpu_ctx *ctx);
+int adcgpu_state_save(adcgpu_ctx *ctx, void *buf, int64_t bytes);
+int adcgpu_state_load(adcgpu_ctx *ctx, const void *buf, int64_t bytes);
 
 /* ---- resident multi-step run (no host round trip per step) ----
  * Runs `nsteps` steps from it0 with elevation-boundary values taken from
  * eta_bc_table[(it-it0)*NETA + I] (host pointer, copied once) and, when
  * holland != NULL, met forcing generated on the device by a Holland vortex
- * (13 parameters, see adcgpu_holland).  Each step is one captured hipGraph
- * replay when use_graph != 0. */
+ * (13 parameters, see adcgpu_holland).  use_graph must be 0 (the steps are
+ * issued as stream launches; ADCGPU_E_UNSUPPORTED otherwise).
+ * The reference's run-time checks are made on the device every step and read
+ * every 64 steps and at the end: ErrorElev (ADCGPU_E_ERRORELEV) and
+ * non-finite velocities (ADCGPU_E_NONFINITE); the run returns the error at
+ * the next read, and adcgpu_run_stats gives the first step that raised it. */
 int adcgpu_run(adcgpu_ctx *ctx, int it0, int nsteps, const double *eta_bc_table,
                const double *holland, int use_graph);
+/* Statistics of the last adcgpu_run: out[0] JCG iterations, out[1] GWCE
+ * matrix rebuilds, out[2] solves that reached ITMAX (ier = 13; the reference
+ * continues, src/gwce.F:2003-2005), out[3] the first step that raised a
+ * run-time error (0 if none), out[4] solves that started from the learned
+ * guess (warm_start = 2). */
+int adcgpu_run_stats(adcgpu_ctx *ctx, int64_t *out5);
+
+/* ---- learned JCG start (warm_start = 2) ----
+ * No counterpart in the reference: these replace only the initial guess of
+ * CALL JCG (ETAS, src/gwce.F:2002-2004); the stopping test
+ * (src/itpackv.F:809-890) and everything after it are unchanged.  A per-node
+ * network, 48 inputs -> 128 -> 128 -> 1 with GELU, BF16 weights and FP32
+ * accumulation, predicts the correction to ETAS from the node's stencil (the
+ * node and up to 12 NEITAB neighbours: last two increments, elevation
+ * differences), depth, wind stress, pressure change, friction and wet/dry
+ * flags.  Each solve starts from whichever of ETAS and ETAS + correction has
+ * the smaller initial pseudo-residual.  Training runs on the device, on
+ * samples of the model's own solves. */
+#define ADCGPU_ML_NPARAM 22913 /* W1[48][128], b1[128], W2[128][128], b2, w3[128], b3 (FP32) */
+/* Set (in != NULL; the Adam moments restart) and/or read (out != NULL) the
+ * FP32 parameters.  A new context starts from a fixed-seed Glorot draw with a
+ * zero output layer: correction 0, the reference's start. */
+int adcgpu_ml_params(adcgpu_ctx *ctx, const float *in, float *out);
+/* From the next step on, record per_step node snapshots of every solve as
+ * training samples (features before the solve, target (ETAS_new - ETAS) / s
+ * after it), in a ring of 2^20; 0 stops. */
+int adcgpu_ml_collect(adcgpu_ctx *ctx, int per_step);
+int64_t adcgpu_ml_samples(adcgpu_ctx *ctx);
+/* Adam (lr) over the held samples, minibatches of 4096 in a seeded random
+ * order, loss = sum w (y - t)^2 / 4096 with w = s^2 / mean(s^2); the mean
+ * loss of the last epoch in *loss. */
+int adcgpu_ml_train(adcgpu_ctx *ctx, int epochs, double lr, uint64_t seed, double *loss);
+/* Loss and gradient (ADCGPU_ML_NPARAM floats, the parameters' order) of one
+ * batch of n <= 4096 samples: x[n][48] in the kernels' input order, targets y,
+ * weights w, loss = sum w (y - t)^2 / n.  No update.  Overwrites the first n
+ * samples of the ring. */
+int adcgpu_ml_grad(adcgpu_ctx *ctx, int n, const float *x, const float *y, const float *w,
+                   float *grad, double *loss);
+/* The network's inputs for every node of the current state (x[NP][48], BF16
+ * values, the kernels' input order) and each node's scale s; overwrites the
+ * first NP samples of the ring. */
+int adcgpu_ml_features(adcgpu_ctx *ctx, float *x, double *s);
+/* The correction s * y the network gives every node of the current state
+ * (0 on dry and elevation-boundary nodes). */
+int adcgpu_ml_correction(adcgpu_ctx *ctx, double *corr);
 /* Sum of JCG iterations over the last adcgpu_run. */
 int64_t adcgpu_run_iterations(adcgpu_ctx *ctx);
 /* Steps of the last adcgpu_run that rebuilt the GWCE matrix (NCChange > 0,
@@ -193,6 +267,13 @@ int adcgpu_elevation_bc(int neta, int nbfr, const double *amig, const double *ff
  * reference's SAVE/COMMON data.  Host pointers; returns ier. */
 int adcgpu_jcg(int n, int ndim, int maxnz, int32_t *jcoef, double *coef, double *rhs,
                double *u, int32_t *iparm, double *rparm);
+
+/* Upper bound, in seconds of device wall clock, of every wait inside the
+ * persistent JCG kernel (grid barrier, neighbour flags); default 0.5.  A wait
+ * that runs out sets an error flag that every block sees, so the launch drains
+ * and the call returns ADCGPU_E_HIP.  No counterpart in the reference; for
+ * failure-handling tests (a tiny bound forces the error path from the host). */
+int adcgpu_set_wait_limit(double seconds);
 
 /* ---- synthetic forcing ---- */
 /* Holland (1980) vortex wind stress / rho0 and pressure head at time t on
