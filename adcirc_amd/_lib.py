@@ -29,13 +29,14 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_timestep", "adcgpu_last_numitr", "adcgpu_synchronize", "adcgpu_run",
            "adcgpu_run_iterations", "adcgpu_run_rebuilds", "adcgpu_elevation_bc", "adcgpu_jcg", "adcgpu_holland",
            "adcgpu_profile", "adcgpu_profile_read", "adcgpu_jcg_shape", "adcgpu_run_stats",
-           "adcgpu_set_wait_limit", "adcgpu_ml_params",
+           "adcgpu_set_wait_limit", "adcgpu_jcg_noalloc", "adcgpu_ml_params",
            "adcgpu_ml_collect", "adcgpu_ml_samples", "adcgpu_ml_train", "adcgpu_ml_grad",
            "adcgpu_ml_features", "adcgpu_ml_correction", "adcgpu_state_bytes",
            "adcgpu_state_save", "adcgpu_state_load")
 ML_NPARAM = 22913
 # error codes (ADCGPU_E_*)
 E_ARG, E_UNSUPPORTED, E_HIP, E_DIAG, E_CAPACITY, E_NONFINITE, E_ERRORELEV = range(-1, -8, -1)
+JCG_NOALLOC_MAX_N, JCG_NOALLOC_MAX_NZ = 32768, 16
 
 
 class Mesh(C.Structure):
@@ -99,6 +100,7 @@ def load() -> C.CDLL:
     lib.adcgpu_jcg_shape.argtypes = [vp, ip_]
     lib.adcgpu_run_stats.argtypes = [vp, C.POINTER(C.c_int64)]
     lib.adcgpu_set_wait_limit.argtypes = [C.c_double]
+    lib.adcgpu_jcg_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, ip_, dp_, dp_, dp_, ip_, dp_]
     lib.adcgpu_state_bytes.argtypes = [vp]
     lib.adcgpu_state_bytes.restype = C.c_int64
     lib.adcgpu_state_save.argtypes = [vp, vp, C.c_int64]

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libadcgpu.so")
 DRIVER = os.path.join(HERE, "adcgpu_run_case")
-SOURCES = ["kernels.hip", "jcg.hip", "mlp.hip", "api.hip"]
+SOURCES = ["kernels.hip", "jcg.hip", "mlp.hip", "api.hip", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",

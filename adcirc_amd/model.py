@@ -376,15 +376,17 @@ def cold_wetdry_state(tables: Dict[str, np.ndarray], run, eta2=None):
 
 
 def jcg(coef: np.ndarray, jcoef: np.ndarray, rhs: np.ndarray, u: np.ndarray, iparm: np.ndarray,
-        rparm: np.ndarray, raise_errors: bool = True) -> int:
+        rparm: np.ndarray, noalloc: bool = False, raise_errors: bool = True) -> int:
     """Drop-in JCG on host arrays (Fortran layout (ndim, maxnz) passed as the
     column-major flattening); arrays are updated in place like the reference.
-    Returns ITPACK's ier, or the negative status code when ``raise_errors``
-    is off."""
+    ``noalloc`` selects the entry point that allocates nothing
+    (adcgpu_jcg_noalloc).  Returns ITPACK's ier, or the negative status code
+    when ``raise_errors`` is off."""
     lib = _lib.load()
     ndim = rhs.shape[0]
     maxnz = coef.size // ndim
-    rc = lib.adcgpu_jcg(ndim, ndim, maxnz, _p(jcoef, ip_), _p(coef, dp_), _p(rhs, dp_), _p(u, dp_),
+    fn = lib.adcgpu_jcg_noalloc if noalloc else lib.adcgpu_jcg
+    rc = fn(ndim, ndim, maxnz, _p(jcoef, ip_), _p(coef, dp_), _p(rhs, dp_), _p(u, dp_),
             _p(iparm, ip_), _p(rparm, dp_))
     if rc < 0 and raise_errors:
         check(rc, "jcg")

@@ -268,6 +268,24 @@ int adcgpu_elevation_bc(int neta, int nbfr, const double *amig, const double *ff
 int adcgpu_jcg(int n, int ndim, int maxnz, int32_t *jcoef, double *coef, double *rhs,
                double *u, int32_t *iparm, double *rparm);
 
+/* ---- drop-in JCG that allocates nothing ----
+ * No counterpart in the reference: for callers that cannot allocate memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, semantics and return values as adcgpu_jcg, for
+ * systems of at most ADCGPU_JCG_NOALLOC_MAX_N rows and
+ * ADCGPU_JCG_NOALLOC_MAX_NZ columns (larger ones return ADCGPU_E_ARG before
+ * touching the device).  Every device buffer lives in one static device array
+ * of ADCGPU_JCG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends;
+ * the call makes no hipMalloc/hipFree and no host allocation.  The array keeps
+ * what the last call left, ITPACK's saved state included (it carries from call
+ * to call on a device, separately from adcgpu_jcg's).  Calls that overlap on
+ * one GPU race on the array.  Work runs on the null stream. */
+#define ADCGPU_JCG_NOALLOC_MAX_N 32768
+#define ADCGPU_JCG_NOALLOC_MAX_NZ 16
+#define ADCGPU_JCG_NOALLOC_SCRATCH_BYTES 17999616
+int adcgpu_jcg_noalloc(int n, int ndim, int maxnz, int32_t *jcoef, double *coef, double *rhs,
+                       double *u, int32_t *iparm, double *rparm);
+
 /* Upper bound, in seconds of device wall clock, of every wait inside the
  * persistent JCG kernel (grid barrier, neighbour flags); default 0.5.  A wait
  * that runs out sets an error flag that every block sees, so the launch drains
