@@ -103,6 +103,11 @@ struct adcgpu_ctx {
   double *lhs_td = nullptr, *lhs_ta = nullptr, *lhs_tb = nullptr;
   unsigned char* ncele = nullptr;
   bool ncele_fresh = false;
+  // mixed precision (ADCGPU_MIXED): FP32 element geometry, cached matrix
+  // terms and scaled off-diagonals; contrib, cx and cy then hold floats
+  bool mixed = false;
+  float *fdx32 = nullptr, *fdy32 = nullptr, *areas32 = nullptr;
+  float *lhs_td32 = nullptr, *lhs_ta32 = nullptr, *lhs_tb32 = nullptr, *cs32 = nullptr;
   // learned JCG start (set up on first use): FP32 master parameters and Adam
   // moments, the kernels' weight layouts, the per-step correction and second
   // start, training samples and the training work arrays
@@ -136,6 +141,7 @@ struct adcgpu_ctx {
     m.nneigh = nneigh; m.col = col; m.csrpos = csrpos; m.bnd_index = bnd_index; m.nbd = nbd;
     m.obc_q = obc_q; m.obc_s = obc_s; m.nobc = nobc; m.obcg_ptr = obcg_ptr; m.obcg = obcg; m.vbc_ptr = vbc_ptr;
     m.vbc = vbc; m.lbcodei = lbcodei; m.csii = csii; m.siii = siii;
+    m.fdx32 = fdx32; m.fdy32 = fdy32; m.areas32 = areas32;
     return m;
   }
   DevState state() const {
@@ -234,6 +240,7 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   // the learned second start, formed by the pass before (learned mode)
   a.u2 = u2;
   a.d2 = c->jd2;
+  a.cs32 = c->cs32;
   // a prescaling pass (k_rhs_node) has already cleared the barrier words
   if (!prescaled) {
     const int bar_words = jcg_barrier_words(c);
@@ -246,7 +253,8 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
       if (int rc = grow_events(c, c->jcg_events.size() + 256)) return rc;
     HIPCHK(hipEventRecord(c->jcg_events[c->ev_used].first, c->stream));
   }
-  const void* fn = u2 ? (const void*)k_jcg<true> : (const void*)k_jcg<false>;
+  const void* fn = c->mixed ? (u2 ? (const void*)k_jcg<true, float> : (const void*)k_jcg<false, float>)
+                            : (u2 ? (const void*)k_jcg<true, double> : (const void*)k_jcg<false, double>);
   HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->jcg_grid), dim3(JCG_BLOCK), args, 0, c->stream));
   if (c->profile) HIPCHK(hipEventRecord(c->jcg_events[c->ev_used++].second, c->stream));
   return 0;
@@ -403,7 +411,8 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   if (P->nolibf == 2) return fail(ADCGPU_E_UNSUPPORTED, "hybrid bottom friction (NOLIBF=2)");
   if (P->nolibf < 0 || P->nolibf > 3) return fail(ADCGPU_E_ARG, "NOLIBF");
   if (P->itmax <= 0 || P->itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
-  if (P->precision != ADCGPU_FP64) return fail(ADCGPU_E_UNSUPPORTED, "only the FP64 path");
+  if (P->precision != ADCGPU_FP64 && P->precision != ADCGPU_MIXED)
+    return fail(ADCGPU_E_ARG, "precision must be ADCGPU_FP64 or ADCGPU_MIXED");
   if (P->warm_start < 0 || P->warm_start > 2) return fail(ADCGPU_E_ARG, "warm_start must be 0, 1 or 2");
   if (P->warm_start == 2 && P->ilump != 0)
     return fail(ADCGPU_E_UNSUPPORTED, "the learned start needs the JCG solve (ILump = 0)");
@@ -596,6 +605,19 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   A(&c->contrib, 3 * (size_t)ne); A(&c->cx, 3 * (size_t)ne); A(&c->cy, 3 * (size_t)ne);
   A(&c->lhs_td, 3 * (size_t)ne); A(&c->lhs_ta, 3 * (size_t)ne); A(&c->lhs_tb, 3 * (size_t)ne);
   A(&c->ncele, ne);
+  c->mixed = P->precision == ADCGPU_MIXED;
+  if (c->mixed) {
+    std::vector<float> f32(3 * (size_t)ne);
+    for (size_t k = 0; k < f32.size(); ++k) f32[k] = (float)fdx[k];
+    U(&c->fdx32, f32);
+    for (size_t k = 0; k < f32.size(); ++k) f32[k] = (float)fdy[k];
+    U(&c->fdy32, f32);
+    f32.resize(ne);
+    for (int e = 0; e < ne; ++e) f32[e] = (float)var[e];
+    U(&c->areas32, f32);
+    A(&c->lhs_td32, 3 * (size_t)ne); A(&c->lhs_ta32, 3 * (size_t)ne);
+    A(&c->lhs_tb32, 3 * (size_t)ne); A(&c->cs32, (size_t)mnei * np);
+  }
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
   {
     const size_t nb = (size_t)(ne + 255) / 256;
@@ -617,13 +639,15 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   if (!rc) {
     // the LHS row kernel keeps MNEI x 512 doubles of LDS per block; the
     // wetting propagation stages candidates through LDS
-    hipError_t e = hipFuncSetAttribute((const void*)k_lhs_rows<false>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       mnei * RED_THREADS * (int)sizeof(double));
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)k_lhs_rows<true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              mnei * RED_THREADS * (int)sizeof(double));
+    hipError_t e = hipSuccess;
+    const void* rows_fns[] = {(const void*)k_lhs_rows<false, double>,
+                              (const void*)k_lhs_rows<true, double>,
+                              (const void*)k_lhs_rows<false, float>,
+                              (const void*)k_lhs_rows<true, float>};
+    for (const void* fn : rows_fns)
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                mnei * RED_THREADS * (int)sizeof(double));
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)k_wd_link, hipFuncAttributeMaxDynamicSharedMemorySize,
                               WD_LINK_LDS);
@@ -649,10 +673,12 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     // the GWCE matrix's unchanging terms, once
     if (P->ilump != 0)
       hipLaunchKernelGGL(k_lhs_terms<true>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
-                         c->mesh(), c->lhs_td, c->lhs_ta, c->lhs_tb);
+                         c->mesh(), c->lhs_td, c->lhs_ta, c->lhs_tb, c->lhs_td32, c->lhs_ta32,
+                         c->lhs_tb32);
     else
       hipLaunchKernelGGL(k_lhs_terms<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
-                         c->mesh(), c->lhs_td, c->lhs_ta, c->lhs_tb);
+                         c->mesh(), c->lhs_td, c->lhs_ta, c->lhs_tb, c->lhs_td32, c->lhs_ta32,
+                         c->lhs_tb32);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
@@ -865,19 +891,27 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
                        c->state());
     c->ncele_fresh = true;
   }
-  const LhsTerms lt{c->lhs_td, c->lhs_ta, c->lhs_tb};
+  const LhsTerms lt{c->lhs_td, c->lhs_ta, c->lhs_tb, c->lhs_td32, c->lhs_ta32, c->lhs_tb32};
+  const float* rcs32 = c->coef_scaled ? (const float*)c->cs32 : nullptr;
   // contrib, cx and cy are free until the GWCE right-hand side
-  if (lumped) {
-    hipLaunchKernelGGL(k_lhs_rows<true>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
-                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, lt,
-                       c->ep_part, rcs, rw);
-  } else {
-    // the element pass only runs while some ALPHAL differs from 1
+  if (!lumped)  // the element pass only runs while some ALPHAL differs from 1
     hipLaunchKernelGGL(k_lhs_elem<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
                        c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy);
-    hipLaunchKernelGGL(k_lhs_rows<false>, dim3(c->nch), dim3(RED_THREADS), rows_lds, c->stream,
-                       c->mesh(), c->state(), c->flags, c->contrib, c->cx, c->cy, lt,
-                       c->ep_part, rcs, rw);
+  const dim3 rg(c->nch), rb(RED_THREADS);
+  if (c->mixed) {
+    if (lumped)
+      hipLaunchKernelGGL((k_lhs_rows<true, float>), rg, rb, rows_lds, c->stream, c->mesh(),
+                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs32, rw);
+    else
+      hipLaunchKernelGGL((k_lhs_rows<false, float>), rg, rb, rows_lds, c->stream, c->mesh(),
+                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs32, rw);
+  } else {
+    if (lumped)
+      hipLaunchKernelGGL((k_lhs_rows<true, double>), rg, rb, rows_lds, c->stream, c->mesh(),
+                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs, rw);
+    else
+      hipLaunchKernelGGL((k_lhs_rows<false, double>), rg, rb, rows_lds, c->stream, c->mesh(),
+                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs, rw);
   }
   c->coef_scaled = false;
   RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, lumped ? 4.0 : 2.0, lumped ? 0.0 : 1.0,
@@ -885,8 +919,13 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   // its last block finishes the matrix (EP, elevation-boundary rows) and
   // clears NCChange
   static_assert(RED_XTHREADS == 256, "the matrix-finishing block is a 256-thread block");
-  hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256) + 1), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), q, c->flags, c->contrib, c->nch, c->ep_part, c->scal);
+  float* contrib32 = reinterpret_cast<float*>(c->contrib);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_rhs_elem32, dim3(nblk(ne, 256) + 1), dim3(256), 0, c->stream, c->mesh(),
+                       c->state(), q, c->flags, contrib32, c->nch, c->ep_part, c->scal);
+  else
+    hipLaunchKernelGGL(k_rhs_elem, dim3(nblk(ne, 256) + 1), dim3(256), 0, c->stream, c->mesh(),
+                       c->state(), q, c->flags, c->contrib, c->nch, c->ep_part, c->scal);
   // ETA1 <- ETA2, H1 <- H2 (src/gwce.F:1622-1631) by rotation
   std::swap(c->eta1, c->eta2);
   std::swap(c->h1, c->h2);
@@ -906,14 +945,24 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   }
   JcgPre pre{c->jw, c->jbar, jcg_barrier_words(c), c->flags, c->etas_prev, c->prm.warm_start,
              (c->prm.warm_start != 0 || c->ml_ready) ? 1 : 0, c->ml_corr, c->ml_u};
-  if (lumped) {
-    // the explicit solve of the lumped GWCE ends the node pass; no JCG
-    hipLaunchKernelGGL(k_rhs_node<true>, dim3(nblk(np, 256)), dim3(256), 0, c->stream,
-                       c->mesh(), c->state(), c->contrib, d_eta_bc, c->scal, pre);
-    return check_launch("gwce assembly and lumped solve");
+  const dim3 ng(nblk(np, 256)), nbk(256);
+  // with ILump = 1 the explicit solve of the lumped GWCE ends the node pass; no JCG
+  if (c->mixed) {
+    if (lumped)
+      hipLaunchKernelGGL((k_rhs_node<true, float>), ng, nbk, 0, c->stream, c->mesh(), c->state(),
+                         (const float*)contrib32, d_eta_bc, c->scal, pre);
+    else
+      hipLaunchKernelGGL((k_rhs_node<false, float>), ng, nbk, 0, c->stream, c->mesh(), c->state(),
+                         (const float*)contrib32, d_eta_bc, c->scal, pre);
+  } else {
+    if (lumped)
+      hipLaunchKernelGGL((k_rhs_node<true, double>), ng, nbk, 0, c->stream, c->mesh(), c->state(),
+                         (const double*)c->contrib, d_eta_bc, c->scal, pre);
+    else
+      hipLaunchKernelGGL((k_rhs_node<false, double>), ng, nbk, 0, c->stream, c->mesh(),
+                         c->state(), (const double*)c->contrib, d_eta_bc, c->scal, pre);
   }
-  hipLaunchKernelGGL(k_rhs_node<false>, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), c->contrib, d_eta_bc, c->scal, pre);
+  if (lumped) return check_launch("gwce assembly and lumped solve");
   int rc = check_launch("gwce assembly");
   if (rc) return rc;
   rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, np, c->prm.itmax,
@@ -933,8 +982,12 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
 // COEF with the off-diagonals of a deferred unscaling restored.
 static int restore_coef(adcgpu_ctx* c) {
   if (!c->coef_scaled) return 0;
-  hipLaunchKernelGGL(k_coef_restore, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), (const double*)c->cs, (const double*)c->jw);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_coef_restore<float>, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), (const float*)c->cs32, (const double*)c->jw);
+  else
+    hipLaunchKernelGGL(k_coef_restore<double>, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), (const double*)c->cs, (const double*)c->jw);
   c->coef_scaled = false;
   return check_launch("coef restore");
 }
@@ -995,14 +1048,28 @@ int adcgpu_wetdry(adcgpu_ctx* c, int it) {
 // Momentum; with `alpha` the element pass forms ALPHAL first.
 static int momentum_device(adcgpu_ctx* c, bool alpha) {
   MomPrm q{c->ifnlct, c->prm.nws, c->cur_it, c->errorelev, c->flags};
-  if (alpha)
-    hipLaunchKernelGGL(k_mom_elem<true>, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream,
-                       c->mesh(), c->state(), q, c->cx, c->cy, c->flags);
-  else
-    hipLaunchKernelGGL(k_mom_elem<false>, dim3(nblk(c->ne, 256)), dim3(256), 0, c->stream,
-                       c->mesh(), c->state(), q, c->cx, c->cy, c->flags);
-  hipLaunchKernelGGL(k_mom_node, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
-                     c->state(), q, c->cx, c->cy);
+  const dim3 eg(nblk(c->ne, 256)), ng(nblk(c->np, 256)), bk(256);
+  if (c->mixed) {
+    float* cx32 = reinterpret_cast<float*>(c->cx);
+    float* cy32 = reinterpret_cast<float*>(c->cy);
+    if (alpha)
+      hipLaunchKernelGGL(k_mom_elem32<true>, eg, bk, 0, c->stream, c->mesh(), c->state(), q, cx32,
+                         cy32, c->flags);
+    else
+      hipLaunchKernelGGL(k_mom_elem32<false>, eg, bk, 0, c->stream, c->mesh(), c->state(), q,
+                         cx32, cy32, c->flags);
+    hipLaunchKernelGGL(k_mom_node<float>, ng, bk, 0, c->stream, c->mesh(), c->state(), q,
+                       (const float*)cx32, (const float*)cy32);
+  } else {
+    if (alpha)
+      hipLaunchKernelGGL(k_mom_elem<true>, eg, bk, 0, c->stream, c->mesh(), c->state(), q, c->cx,
+                         c->cy, c->flags);
+    else
+      hipLaunchKernelGGL(k_mom_elem<false>, eg, bk, 0, c->stream, c->mesh(), c->state(), q,
+                         c->cx, c->cy, c->flags);
+    hipLaunchKernelGGL(k_mom_node<double>, ng, bk, 0, c->stream, c->mesh(), c->state(), q,
+                       (const double*)c->cx, (const double*)c->cy);
+  }
   c->ncele_fresh = true;
   return check_launch("momentum");
 }
@@ -1127,6 +1194,21 @@ int adcgpu_run_stats(adcgpu_ctx* c, int64_t* out5) {
   out5[2] = c->run_itmax;
   out5[3] = c->run_errstep;
   out5[4] = c->run_mlused;
+  return 0;
+}
+
+int adcgpu_last_solve(adcgpu_ctx* c, double* out4) {
+  if (!c || !out4) return fail(ADCGPU_E_ARG, "null argument");
+  double s[12];
+  HIPCHK(hipMemcpyAsync(s, c->jsaved, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  JcgArgs a;
+  jcg_tolerances(a, c->prm.itmax, c->prm.convcr);
+  out4[0] = s[5];
+  out4[1] = a.zeta;
+  const bool dual = c->prm.warm_start == 2 && c->prm.ilump == 0;
+  out4[2] = dual ? s[10] : 0.0;
+  out4[3] = dual ? s[11] : 0.0;
   return 0;
 }
 
@@ -1264,6 +1346,7 @@ int adcgpu_ml_correction(adcgpu_ctx* c, double* corr_out) {
 }
 
 // ---- whole-state snapshot (between steps)
+extern "C++" {
 namespace {
 struct StatePart {
   void* dev;
@@ -1290,6 +1373,7 @@ std::vector<StatePart> state_parts(adcgpu_ctx* c) {
   return v;
 }
 }  // namespace
+}  // extern "C++"
 
 int64_t adcgpu_state_bytes(adcgpu_ctx* c) {
   if (!c) return -1;

@@ -42,13 +42,17 @@ inline hipError_t jcg_resident_cap_of(const void* fn, int* per_cu_out) {
   return hipSuccess;
 }
 
-// Both variants of the kernel (one start, two starts) must fit.
+// Every variant of the kernel (one start or two; FP64 or FP32 matrix) must fit.
 inline hipError_t jcg_resident_cap(int device, int* cap) {
-  int pa = 0, pb = 0;
-  hipError_t e = jcg_resident_cap_of((const void*)k_jcg<false>, &pa);
-  if (e == hipSuccess) e = jcg_resident_cap_of((const void*)k_jcg<true>, &pb);
-  if (e != hipSuccess) return e;
-  const int per_cu = std::min(pa, pb);
+  const void* fns[] = {(const void*)k_jcg<false, double>, (const void*)k_jcg<true, double>,
+                       (const void*)k_jcg<false, float>, (const void*)k_jcg<true, float>};
+  int per_cu = 1 << 30;
+  hipError_t e = hipSuccess;
+  for (const void* fn : fns) {
+    int p = 0;
+    if ((e = jcg_resident_cap_of(fn, &p)) != hipSuccess) return e;
+    per_cu = std::min(per_cu, p);
+  }
   int cus = 0;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) !=
       hipSuccess)

@@ -85,6 +85,10 @@ struct DevMesh {
   const int* lbcodei;
   const double* csii;
   const double* siii;
+  // mixed precision: FP32 copies of the element geometry (null in FP64 runs)
+  const float* fdx32;
+  const float* fdy32;
+  const float* areas32;
 };
 
 struct DevState {
@@ -103,6 +107,21 @@ struct DevState {
 // a rebuild multiplies them by the element's NCEle.
 struct LhsTerms {
   const double *td, *ta, *tb;  // [3 ne] diagonal and the two off-diagonal terms
+  const float *td32, *ta32, *tb32;  // the same rounded to FP32 (mixed precision)
+};
+
+// Compensated (Kahan 1965) summation in FP32: the node sums of FP32
+// element contributions in the mixed-precision path.  The result carries the
+// compensation into FP64.
+struct Kahan32 {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
 struct RhsPrm {
@@ -183,6 +202,7 @@ struct JcgArgs {
   const int* nodecode;
   const double* u2;   // k_jcg<true>: the second (learned) start, scaled like u
   double* d2;         //              and its initial pseudo-residual
+  float* cs32;        // k_jcg<., float>: the scaled off-diagonals rounded to FP32
 };
 
 __global__ void k_init_h(int np, const double* dp, const double* eta2, double ifnlfa, double* h1,
@@ -197,20 +217,28 @@ __global__ void k_friction_holland(int np, FricPrm f, double* tk, const double* 
 template <bool LUMPED>
 __global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
                            double* sb);
-template <bool LUMPED>
+// CT: the type of the matrix terms summed per row and of the scaled
+// off-diagonals a deferred solve left (double; float in the mixed path, which
+// sums FP32 terms with compensation)
+template <bool LUMPED, typename CT>
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
                            const double* sa, const double* sb, LhsTerms lt, double* ep_part,
-                           const double* restore_cs, const double* restore_w);
+                           const CT* restore_cs, const double* restore_w);
 // the cached terms (once per context): k_lhs_elem's terms with NCEle = 1 and
-// ALPHAL = 1
+// ALPHAL = 1, and their FP32 roundings when td32 != null
 template <bool LUMPED>
-__global__ void k_lhs_terms(DevMesh m, double* td, double* ta, double* tb);
+__global__ void k_lhs_terms(DevMesh m, double* td, double* ta, double* tb, float* td32,
+                            float* ta32, float* tb32);
 // NCEle of every element from the current NODECODE and NOFF
 __global__ void k_ncele(DevMesh m, DevState s);
-__global__ void k_coef_restore(DevMesh m, DevState s, const double* cs, const double* w);
+template <typename CT>
+__global__ void k_coef_restore(DevMesh m, DevState s, const CT* cs, const double* w);
 // one block more than the elements need: the last finishes the GWCE matrix
 __global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags, double* contrib, int nch,
                            const double* ep_part, double* scal);
+// the same in FP32 (the mixed path): FP32 contributions
+__global__ void k_rhs_elem32(DevMesh m, DevState s, RhsPrm q, int* flags, float* contrib,
+                             int nch, const double* ep_part, double* scal);
 // Jacobi scaling of the GWCE rows ahead of k_jcg (its first scaling pass)
 struct JcgPre {
   double* w;          // 1/sqrt(diag)
@@ -224,11 +252,14 @@ struct JcgPre {
   const double* ml_corr;  // warm == 2: the network's correction per node
   double* ml_u;           // warm == 2: the learned start, scaled
 };
-template <bool LUMPED>
-__global__ void k_rhs_node(DevMesh m, DevState s, const double* contrib, const double* eta_bc,
+// CT: the type of the element contributions (float: compensated FP32 sums)
+template <bool LUMPED, typename CT>
+__global__ void k_rhs_node(DevMesh m, DevState s, const CT* contrib, const double* eta_bc,
                            const double* scal, JcgPre pre);
-// DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2
-template <bool DUAL>
+// DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2;
+// MT: the type of the scaled off-diagonals the iterations stream (float in the
+// mixed path, a.cs32; the vectors, dot products and norms stay FP64)
+template <bool DUAL, typename MT>
 __global__ void k_jcg(JcgArgs a);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);
@@ -243,7 +274,11 @@ __global__ void k_wd_alpha(DevMesh m, DevState s, int* flags, double slim);
 __global__ void k_h2(int np, const double* dp, const double* eta2, double ifnlfa, double* h2);
 template <bool ALPHA>
 __global__ void k_mom_elem(DevMesh m, DevState s, MomPrm q, double* cx, double* cy, int* flags);
-__global__ void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* cx, const double* cy);
+// the same in FP32 (the mixed path)
+template <bool ALPHA>
+__global__ void k_mom_elem32(DevMesh m, DevState s, MomPrm q, float* cx, float* cy, int* flags);
+template <typename CT>
+__global__ void k_mom_node(DevMesh m, DevState s, MomPrm q, const CT* cx, const CT* cy);
 __global__ void k_fill_u32(unsigned* p, unsigned v, int n);
 __global__ void k_fill_i32(int* p, int v, int n);
 __global__ void k_total_area(DevMesh m, DevState s);

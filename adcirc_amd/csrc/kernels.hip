@@ -186,7 +186,10 @@ template __global__ void k_lhs_elem<true>(DevMesh, DevState, const int*, double*
 template <bool LUMPED>
 __global__ __launch_bounds__(256) void k_lhs_terms(DevMesh m, double* __restrict__ td,
                                                    double* __restrict__ ta,
-                                                   double* __restrict__ tb) {
+                                                   double* __restrict__ tb,
+                                                   float* __restrict__ td32,
+                                                   float* __restrict__ ta32,
+                                                   float* __restrict__ tb32) {
   const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   double d[3], a[3], b[3];
@@ -197,10 +200,17 @@ __global__ __launch_bounds__(256) void k_lhs_terms(DevMesh m, double* __restrict
     td[p] = d[k];
     ta[p] = a[k];
     tb[p] = b[k];
+    if (td32) {
+      td32[p] = (float)d[k];
+      ta32[p] = (float)a[k];
+      tb32[p] = (float)b[k];
+    }
   }
 }
-template __global__ void k_lhs_terms<false>(DevMesh, double*, double*, double*);
-template __global__ void k_lhs_terms<true>(DevMesh, double*, double*, double*);
+template __global__ void k_lhs_terms<false>(DevMesh, double*, double*, double*, float*, float*,
+                                            float*);
+template __global__ void k_lhs_terms<true>(DevMesh, double*, double*, double*, float*, float*,
+                                           float*);
 
 __global__ __launch_bounds__(256) void k_ncele(DevMesh m, DevState s) {
   const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -214,7 +224,7 @@ __global__ __launch_bounds__(256) void k_ncele(DevMesh m, DevState s) {
 // reference adds into COEF(i,j) element by element, src/gwce.F:538-592);
 // also the EP chunk partials of sum COEF(i,1)^2 (src/gwce.F:646-650) in the
 // fixed blocked tree.  Dynamic LDS: MNEI x RED_THREADS doubles, slot-major.
-template <bool LUMPED>
+template <bool LUMPED, typename CT>
 __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
                                                           const int* __restrict__ flags,
                                                           const double* __restrict__ sd,
@@ -222,11 +232,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
                                                           const double* __restrict__ sb,
                                                           LhsTerms lt,
                                                           double* __restrict__ ep_part,
-                                                          const double* __restrict__ restore_cs,
+                                                          const CT* __restrict__ restore_cs,
                                                           const double* __restrict__ restore_w) {
+  constexpr bool MIXED = sizeof(CT) == 4;
   __shared__ double lds[8];
   extern __shared__ double racc_lds[];
   double(*racc)[RED_THREADS] = reinterpret_cast<double(*)[RED_THREADS]>(racc_lds);
+  // mixed path: the same LDS words hold a compensated FP32 pair per column
+  float2(*racc2)[RED_THREADS] = reinterpret_cast<float2(*)[RED_THREADS]>(racc_lds);
   const int np = m.np;
   const int t = threadIdx.x;
   const int chunk = xcd_tile(blockIdx.x, gridDim.x);
@@ -242,7 +255,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
         const double ri = 1.0 / restore_w[i];
         for (int j = 1; j < nn; ++j) {
           const size_t q = (size_t)j * np + i;
-          s.coef[q] = restore_cs[q] * ri * (1.0 / restore_w[m.col[q]]);
+          s.coef[q] = (double)restore_cs[q] * ri * (1.0 / restore_w[m.col[q]]);
         }
       }
     return;
@@ -259,10 +272,52 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
     if (LUMPED) {
       // COEFD only; the off-diagonal columns stay zero
       double diag = 0.0;
-      for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p)
-        diag = diag + lt.td[p] * (double)s.ncele[(unsigned)m.inc[p] / 3u];
+      if (MIXED) {
+        Kahan32 k;
+        for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p)
+          k.add(lt.td32[p] * (float)s.ncele[(unsigned)m.inc[p] / 3u]);
+        diag = k.value();
+      } else {
+        for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p)
+          diag = diag + lt.td[p] * (double)s.ncele[(unsigned)m.inc[p] / 3u];
+      }
       s.coef[i] = diag;
       for (int j = 1; j < nn; ++j) s.coef[(size_t)j * np + i] = 0.0;
+      acc = acc + diag * diag;
+      continue;
+    }
+    if (MIXED) {
+      // FP32 terms, compensated sums per column
+      for (int j = 1; j < nn; ++j) racc2[j][t] = make_float2(0.0f, 0.0f);
+      Kahan32 kd;
+      for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
+        const int jab = m.slot_cols[p];
+        const int ja = jab & 0xffff, jb = jab >> 16;
+        float vd, va, vb;
+        if (cached) {
+          const float nce = (float)s.ncele[(unsigned)m.inc[p] / 3u];
+          vd = lt.td32[p] * nce;
+          va = lt.ta32[p] * nce;
+          vb = lt.tb32[p] * nce;
+        } else {
+          vd = (float)sd[p];
+          va = (float)sa[p];
+          vb = (float)sb[p];
+        }
+        kd.add(vd);
+        Kahan32 ka{racc2[ja][t].x, racc2[ja][t].y};
+        ka.add(va);
+        racc2[ja][t] = make_float2(ka.s, ka.c);
+        Kahan32 kb{racc2[jb][t].x, racc2[jb][t].y};
+        kb.add(vb);
+        racc2[jb][t] = make_float2(kb.s, kb.c);
+      }
+      const double diag = kd.value();
+      s.coef[i] = diag;
+      for (int j = 1; j < nn; ++j) {
+        const Kahan32 kj{racc2[j][t].x, racc2[j][t].y};
+        s.coef[(size_t)j * np + i] = kj.value();
+      }
       acc = acc + diag * diag;
       continue;
     }
@@ -293,8 +348,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
   double tot = chunk_tree(acc, lds);
   if (t == 0) ep_part[chunk] = tot;
 }
-template __global__ void k_lhs_rows<false>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
-template __global__ void k_lhs_rows<true>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
+template __global__ void k_lhs_rows<false, double>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
+template __global__ void k_lhs_rows<true, double>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
+template __global__ void k_lhs_rows<false, float>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const float*, const double*);
+template __global__ void k_lhs_rows<true, float>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const float*, const double*);
 
 // EP = sqrt(sum of squared diagonals / NP) from the chunk partials, and the
 // elevation-boundary rows (src/gwce.F:700-743): a boundary row becomes EP on
@@ -339,8 +396,9 @@ __device__ void lhs_bc_block(const DevMesh& m, DevState s, int nch,
 
 // The off-diagonals of a solve whose unscaling was deferred, restored as
 // unscal does (src/itpackv.F:2460-2479).
+template <typename CT>
 __global__ __launch_bounds__(256) void k_coef_restore(DevMesh m, DevState s,
-                                                      const double* __restrict__ cs,
+                                                      const CT* __restrict__ cs,
                                                       const double* __restrict__ w) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
@@ -348,9 +406,11 @@ __global__ __launch_bounds__(256) void k_coef_restore(DevMesh m, DevState s,
   const double ri = 1.0 / w[i];
   for (int j = 1; j < nn; ++j) {
     const size_t q = (size_t)j * np + i;
-    s.coef[q] = cs[q] * ri * (1.0 / w[m.col[q]]);
+    s.coef[q] = (double)cs[q] * ri * (1.0 / w[m.col[q]]);
   }
 }
+template __global__ void k_coef_restore<double>(DevMesh, DevState, const double*, const double*);
+template __global__ void k_coef_restore<float>(DevMesh, DevState, const float*, const double*);
 
 // --------------------------------------------------------------- GWCE RHS
 // Element contributions Temp_LV_A1..3 * NCEle (src/gwce.F:1000-1615).  Terms
@@ -492,15 +552,157 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
   contrib[m.slot[3 * e + 2]] = R3 * nce;
 }
 
+// The gradient sum f1*D1 + f2*D2 + f3*D3 of FP64 nodal values in FP32: with
+// D1 + D2 + D3 = 0 (the shape-function derivatives of a triangle) it is
+// (f2 - f1) D2 + (f3 - f1) D3, and the differences are taken in FP64 before
+// rounding, so a field's mean (an elevation of metres, a pressure head of ten
+// metres) costs no FP32 digits of its variation.
+__device__ __forceinline__ float grad32(double f1, double f2, double f3, float d2, float d3) {
+  return (float)(f2 - f1) * d2 + (float)(f3 - f1) * d3;
+}
+
+// The GWCE right-hand side element pass in FP32 (the mixed-precision path):
+// the terms of k_rhs_elem with FP32 arithmetic, FP32 geometry and gradients
+// formed from FP64 differences (grad32); FP32 contributions, summed with
+// compensation by the node pass.
+__global__ __launch_bounds__(256) void k_rhs_elem32(DevMesh m, DevState s, RhsPrm q, int* flags,
+                                                    float* __restrict__ contrib, int nch,
+                                                    const double* __restrict__ ep_part,
+                                                    double* __restrict__ scal) {
+  const int nbe = (int)gridDim.x - 1;
+  if ((int)blockIdx.x == nbe) {
+    const int ncc = flags[F_NCCHANGE];
+    if (ncc > 0) {
+      if (q.lumped)
+        lhs_bc_block<true>(m, s, nch, ep_part, scal);
+      else
+        lhs_bc_block<false>(m, s, nch, ep_part, scal);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (ncc > 0) flags[F_NREBUILD] += 1;
+      flags[F_NCCHANGE] = 0;
+    }
+    return;
+  }
+  int e = xcd_tile(blockIdx.x, nbe) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
+  const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
+  float R1 = 0.0f, R2 = 0.0f, R3 = 0.0f;
+  if (ncele != 0) {
+    const float DT = (float)m.dt, G = (float)m.g, A00 = (float)m.a00, B00 = (float)m.b00,
+                C00 = (float)m.c00;
+    const float FDX1 = m.fdx32[3 * e], FDX2 = m.fdx32[3 * e + 1], FDX3 = m.fdx32[3 * e + 2];
+    const float FDY1 = m.fdy32[3 * e], FDY2 = m.fdy32[3 * e + 1], FDY3 = m.fdy32[3 * e + 2];
+    const float AreaIE2 = m.areas32[e];
+    const float AreaIE = AreaIE2 / 2.0f;
+    const float AreaIE4 = 2.0f * AreaIE2;
+    const double e0a = s.eta1[n1], e0b = s.eta1[n2], e0c = s.eta1[n3];
+    const double e1a = s.eta2[n1], e1b = s.eta2[n2], e1c = s.eta2[n3];
+    const float ESN1 = (float)s.etas[n1], ESN2 = (float)s.etas[n2], ESN3 = (float)s.etas[n3];
+    const double ua = s.uu1[n1], ub = s.uu1[n2], uc = s.uu1[n3];
+    const double va = s.vv1[n1], vb = s.vv1[n2], vc = s.vv1[n3];
+    const double qxa = s.qx1[n1], qxb = s.qx1[n2], qxc = s.qx1[n3];
+    const double qya = s.qy1[n1], qyb = s.qy1[n2], qyc = s.qy1[n3];
+    const float H1N1 = (float)s.h2[n1], H1N2 = (float)s.h2[n2], H1N3 = (float)s.h2[n3];
+    const double eva = m.evm[n1], evb = m.evm[n2], evc = m.evm[n3];
+    const double t0a = m.tau0[n1], t0b = m.tau0[n2], t0c = m.tau0[n3];
+    float PR1X = 0.0f, PR1Y = 0.0f, WSXAvg = 0.0f, WSYAvg = 0.0f;
+    if (q.nws) {
+      const double pa = s.pr1[n1], pb = s.pr1[n2], pc = s.pr1[n3];
+      PR1X = grad32(pa, pb, pc, FDX2, FDX3);
+      PR1Y = grad32(pa, pb, pc, FDY2, FDY3);
+      WSXAvg = ((float)s.wsx1[n1] + (float)s.wsx1[n2] + (float)s.wsx1[n3]) / 3.0f;
+      WSYAvg = ((float)s.wsy1[n1] + (float)s.wsy1[n2] + (float)s.wsy1[n3]) / 3.0f;
+    }
+    const float U1N1 = (float)ua, U1N2 = (float)ub, U1N3 = (float)uc;
+    const float V1N1 = (float)va, V1N2 = (float)vb, V1N3 = (float)vc;
+    const float QX1N1 = (float)qxa, QX1N2 = (float)qxb, QX1N3 = (float)qxc;
+    const float QY1N1 = (float)qya, QY1N2 = (float)qyb, QY1N3 = (float)qyc;
+    const float T0N1 = (float)t0a, T0N2 = (float)t0b, T0N3 = (float)t0c;
+    const float tk1 = (float)s.tk[n1], tk2 = (float)s.tk[n2], tk3 = (float)s.tk[n3];
+    const float BSXAvg = (tk1 * QX1N1 + tk2 * QX1N2 + tk3 * QX1N3) / 3.0f;
+    const float BSYAvg = (tk1 * QY1N1 + tk2 * QY1N2 + tk3 * QY1N3) / 3.0f;
+    const float E0XGrad2A = q.lumped ? 0.0f : grad32(e0a, e0b, e0c, FDX2, FDX3);
+    const float E0YGrad2A = q.lumped ? 0.0f : grad32(e0a, e0b, e0c, FDY2, FDY3);
+    const float E1XGrad2A = grad32(e1a, e1b, e1c, FDX2, FDX3);
+    const float E1YGrad2A = grad32(e1a, e1b, e1c, FDY2, FDY3);
+    const float E1SQX = grad32(e1a * e1a, e1b * e1b, e1c * e1c, FDX2, FDX3);
+    const float E1SQY = grad32(e1a * e1a, e1b * e1b, e1c * e1c, FDY2, FDY3);
+    const float Tau0XGrad2A = grad32(t0a, t0b, t0c, FDX2, FDX3);
+    const float Tau0YGrad2A = grad32(t0a, t0b, t0c, FDY2, FDY3);
+    const float EVMXGrad = grad32(eva, evb, evc, FDX2, FDX3) / AreaIE2;
+    const float EVMYGrad = grad32(eva, evb, evc, FDY2, FDY3) / AreaIE2;
+    const float EVMAvgODT = (((float)eva + (float)evb + (float)evc) / 3.0f) / DT;
+    const float ESX = ESN1 * FDX1 + ESN2 * FDX2 + ESN3 * FDX3;
+    const float ESY = ESN1 * FDY1 + ESN2 * FDY2 + ESN3 * FDY3;
+    const float MX = (EVMXGrad * grad32(qxa, qxb, qxc, FDX2, FDX3) +
+                      EVMYGrad * grad32(qya, qyb, qyc, FDX2, FDX3) - EVMAvgODT * ESX) / AreaIE2;
+    const float MY = (EVMXGrad * grad32(qxa, qxb, qxc, FDY2, FDY3) +
+                      EVMYGrad * grad32(qya, qyb, qyc, FDY2, FDY3) - EVMAvgODT * ESY) / AreaIE2;
+    const float CorifAvg = ((float)m.corif[n1] + (float)m.corif[n2] + (float)m.corif[n3]) / 3.0f;
+    const float Tau0Avg = (T0N1 + T0N2 + T0N3) / 3.0f;
+    const float Tau0QXAvg = (T0N1 * QX1N1 + T0N2 * QX1N2 + T0N3 * QX1N3) / 3.0f;
+    const float Tau0QYAvg = (T0N1 * QY1N1 + T0N2 * QY1N2 + T0N3 * QY1N3) / 3.0f;
+    const float U1Avg = (U1N1 + U1N2 + U1N3) / 3.0f;
+    const float V1Avg = (V1N1 + V1N2 + V1N3) / 3.0f;
+    const float QX1Avg = (QX1N1 + QX1N2 + QX1N3) / 3.0f;
+    const float QY1Avg = (QY1N1 + QY1N2 + QY1N3) / 3.0f;
+    const float ESAvg = (ESN1 + ESN2 + ESN3) / 3.0f;
+    const float DPAvg = (float)((m.dp[n1] + m.dp[n2] + m.dp[n3]) / 3.0);
+    const float al = (float)s.alphal[e];
+    const float GDP = al * G * DPAvg / AreaIE4;
+    const float HAvg = (H1N1 + H1N2 + H1N3) / 3.0f;
+    const float GHAvgOAreaIE2 = (al * G * HAvg) / AreaIE2;
+    const float MsFacR = AreaIE * (1.0f / DT - Tau0Avg / 2.0f) / DT / 12.0f;
+    const float GOAreaIE4 = al * G / AreaIE4;
+    const float Tau0SpaVar = (QX1Avg * Tau0XGrad2A + QY1Avg * Tau0YGrad2A) / 6.0f;
+    const float A00pB00 = A00 + B00;
+    const float fnlfa = (float)q.ifnlfa, fnlct = (float)q.ifnlct, fnlcat = (float)q.ifnlcat;
+    float JXAvg = CorifAvg * QY1Avg - fnlfa * GOAreaIE4 * E1SQX - GHAvgOAreaIE2 * PR1X + WSXAvg -
+                  BSXAvg + MX + Tau0QXAvg;
+    float JYAvg = -CorifAvg * QX1Avg - fnlfa * GOAreaIE4 * E1SQY - GHAvgOAreaIE2 * PR1Y + WSYAvg -
+                  BSYAvg + MY + Tau0QYAvg;
+    JXAvg = JXAvg -
+            fnlct * (QX1Avg * grad32(ua, ub, uc, FDX2, FDX3) +
+                     QY1Avg * grad32(ua, ub, uc, FDY2, FDY3)) / AreaIE2 +
+            fnlcat * U1Avg * ESAvg / DT;
+    JYAvg = JYAvg -
+            fnlct * (QX1Avg * grad32(va, vb, vc, FDX2, FDX3) +
+                     QY1Avg * grad32(va, vb, vc, FDY2, FDY3)) / AreaIE2 +
+            fnlcat * V1Avg * ESAvg / DT;
+    const float ond = (float)q.ondiag, offd = (float)q.offdiag;
+    const float FX[3] = {FDX1, FDX2, FDX3}, FY[3] = {FDY1, FDY2, FDY3};
+    const float ES[3] = {ESN1, ESN2, ESN3};
+    float Rk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float others = ES[(k + 1) % 3] + ES[(k + 2) % 3];
+      Rk[k] = (ond * ES[k] + offd * others) * MsFacR -
+              GDP * (C00 * (FX[k] * E0XGrad2A + FY[k] * E0YGrad2A)) -
+              GDP * (A00pB00 * (FX[k] * E1XGrad2A + FY[k] * E1YGrad2A)) +
+              (JXAvg * FX[k] + JYAvg * FY[k]) * 0.5f + Tau0SpaVar;
+    }
+    R1 = Rk[0];
+    R2 = Rk[1];
+    R3 = Rk[2];
+  }
+  const float nce = (float)ncele;
+  contrib[m.slot[3 * e]] = R1 * nce;
+  contrib[m.slot[3 * e + 1]] = R2 * nce;
+  contrib[m.slot[3 * e + 2]] = R3 * nce;
+}
+
 // Node gather of GWCE_LV, the time-level shift of ETA and the elevation
 // boundary conditions (src/gwce.F:1613-1631, 1638-1651, 1846-1855).  `eta1`
 // already points at the previous ETA2; `eta2` is rewritten.
-template <bool LUMPED>
-__global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const double* __restrict__ contrib,
+template <bool LUMPED, typename CT>
+__global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const CT* __restrict__ contrib,
                            const double* __restrict__ eta_bc, const double* __restrict__ scal,
                            JcgPre pre) {
   // the block's slot range staged in LDS, as in k_mom_node
-  __shared__ double sc[NODE_STAGE];
+  __shared__ CT sc[NODE_STAGE];
   const int r0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
   const int i = r0 + threadIdx.x;
   if (i < pre.barrier_words) pre.barrier[i] = 0u;
@@ -523,10 +725,19 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
     rhs = es * (double)s.nodecode[i] * (LUMPED ? s.coef[i] : scal[S_EP]);
   } else {
     double acc = 0.0;
-    if (staged)
+    if (sizeof(CT) == 4) {
+      // FP32 contributions: compensated sum
+      Kahan32 k;
+      if (staged)
+        for (int p = m.inc_ptr[i] - p0; p < m.inc_ptr[i + 1] - p0; ++p) k.add((float)sc[p]);
+      else
+        for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) k.add((float)contrib[p]);
+      acc = k.value();
+    } else if (staged) {
       for (int p = m.inc_ptr[i] - p0; p < m.inc_ptr[i + 1] - p0; ++p) acc = acc + sc[p];
-    else
+    } else {
       for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
+    }
     for (int p = m.obcg_ptr[i]; p < (LUMPED ? m.obcg_ptr[i] : m.obcg_ptr[i + 1]); ++p) {
       const int slot = m.obcg[p];
       const int I = slot / m.mnei;
@@ -576,8 +787,10 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const d
   pre.w[i] = w;
   s.gwce_lv[i] = rhs * w;
 }
-template __global__ void k_rhs_node<false>(DevMesh, DevState, const double*, const double*, const double*, JcgPre);
-template __global__ void k_rhs_node<true>(DevMesh, DevState, const double*, const double*, const double*, JcgPre);
+template __global__ void k_rhs_node<false, double>(DevMesh, DevState, const double*, const double*, const double*, JcgPre);
+template __global__ void k_rhs_node<true, double>(DevMesh, DevState, const double*, const double*, const double*, JcgPre);
+template __global__ void k_rhs_node<false, float>(DevMesh, DevState, const float*, const double*, const double*, JcgPre);
+template __global__ void k_rhs_node<true, float>(DevMesh, DevState, const float*, const double*, const double*, JcgPre);
 
 // ---------------------------------------------------------------- wet/dry
 // D1 (src/wetdry.F:234-285); also NNODECODE := NODECODE for the W1 writes.
@@ -1223,15 +1436,84 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
 template __global__ void k_mom_elem<false>(DevMesh, DevState, MomPrm, double*, double*, int*);
 template __global__ void k_mom_elem<true>(DevMesh, DevState, MomPrm, double*, double*, int*);
 
+// The momentum element pass in FP32 (the mixed-precision path): the terms of
+// k_mom_elem with FP32 arithmetic and geometry, gradients from FP64
+// differences (grad32), FP32 contributions.
+template <bool ALPHA>
+__global__ __launch_bounds__(256) void k_mom_elem32(DevMesh m, DevState s, MomPrm q,
+                                                    float* __restrict__ cx,
+                                                    float* __restrict__ cy, int* flags) {
+  int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const double al64 = ALPHA ? wd_alpha(m, s, e, flags, 1.0e9) : s.alphal[e];
+  const int n1 = m.nm[3 * e], n2 = m.nm[3 * e + 1], n3 = m.nm[3 * e + 2];
+  const int ncele = s.nodecode[n1] * s.nodecode[n2] * s.nodecode[n3] * s.noff[e];
+  s.ncele[e] = (unsigned char)ncele;
+  float tx[3] = {0.0f, 0.0f, 0.0f}, ty[3] = {0.0f, 0.0f, 0.0f};
+  if (ncele != 0) {
+    const float THIRD = 1.0f / 3.0f, DT = (float)m.dt, GO2 = (float)(m.g / 2.0);
+    const double ua = s.uu1[n1], ub = s.uu1[n2], uc = s.uu1[n3];
+    const double va = s.vv1[n1], vb = s.vv1[n2], vc = s.vv1[n3];
+    const float AreaIE2 = m.areas32[e];
+    const float AreaIE = 0.5f * AreaIE2;
+    const float FDX1 = m.fdx32[3 * e], FDX2 = m.fdx32[3 * e + 1], FDX3 = m.fdx32[3 * e + 2];
+    const float FDY1 = m.fdy32[3 * e], FDY2 = m.fdy32[3 * e + 1], FDY3 = m.fdy32[3 * e + 2];
+    const float U1Avg = THIRD * ((float)ua + (float)ub + (float)uc);
+    const float V1Avg = THIRD * ((float)va + (float)vb + (float)vc);
+    const float DU1DXA = grad32(ua, ub, uc, FDX2, FDX3) * 0.5f;
+    const float DU1DYA = grad32(ua, ub, uc, FDY2, FDY3) * 0.5f;
+    const float DV1DXA = grad32(va, vb, vc, FDX2, FDX3) * 0.5f;
+    const float DV1DYA = grad32(va, vb, vc, FDY2, FDY3) * 0.5f;
+    const float EVMEle = THIRD * ((float)m.evm[n1] + (float)m.evm[n2] + (float)m.evm[n3]);
+    double b1a = al64 * s.eta1[n1], b2a = al64 * s.eta2[n1];
+    double b1b = al64 * s.eta1[n2], b2b = al64 * s.eta2[n2];
+    double b1c = al64 * s.eta1[n3], b2c = al64 * s.eta2[n3];
+    if (q.nws) {
+      b1a = b1a + s.pr1[n1]; b2a = b2a + s.pr2[n1];
+      b1b = b1b + s.pr1[n2]; b2b = b2b + s.pr2[n2];
+      b1c = b1c + s.pr1[n3]; b2c = b2c + s.pr2[n3];
+    }
+    const float DBTPDXA = 0.5f * (grad32(b1a, b1b, b1c, FDX2, FDX3) +
+                                  grad32(b2a, b2b, b2c, FDX2, FDX3));
+    const float DBTPDYA = 0.5f * (grad32(b1a, b1b, b1c, FDY2, FDY3) +
+                                  grad32(b2a, b2b, b2c, FDY2, FDY3));
+    const float fnlct = (float)q.ifnlct;
+    const float advx = -fnlct * (U1Avg * DU1DXA + V1Avg * DU1DYA) - GO2 * DBTPDXA;
+    const float advy = -fnlct * (U1Avg * DV1DXA + V1Avg * DV1DYA) - GO2 * DBTPDYA;
+    const float DU1DX = DU1DXA / AreaIE, DU1DY = DU1DYA / AreaIE;
+    const float DV1DX = DV1DXA / AreaIE, DV1DY = DV1DYA / AreaIE;
+    const float NCDT = (float)ncele * DT;
+    const float Hs[3] = {(float)s.h1[n1], (float)s.h1[n2], (float)s.h1[n3]};
+    const float FX[3] = {FDX1, FDX2, FDX3}, FY[3] = {FDY1, FDY2, FDY3};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float eh = EVMEle * Hs[k];
+      const float lxx = eh * DU1DX, lxy = eh * DU1DY, lyx = eh * DV1DX, lyy = eh * DV1DY;
+      tx[k] = NCDT * (advx - (1.5f * (lxx * FX[k] + lxy * FY[k])) / Hs[k]);
+      ty[k] = NCDT * (advy - (1.5f * (lyx * FX[k] + lyy * FY[k])) / Hs[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int sl = m.slot[3 * e + k];
+    cx[sl] = tx[k];
+    cy[sl] = ty[k];
+  }
+}
+template __global__ void k_mom_elem32<false>(DevMesh, DevState, MomPrm, float*, float*, int*);
+template __global__ void k_mom_elem32<true>(DevMesh, DevState, MomPrm, float*, float*, int*);
+
 // Nodal part of Mom_Eqs_New_NC (src/momentum.F:713-1182): gather, /TotalArea,
 // wind, friction/Coriolis 2x2, land-boundary constraints, solve, fluxes;
-// then the peak-surge maximum (src/write_output.F:6357-6366).
-__global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm q, const double* __restrict__ cx,
-                           const double* __restrict__ cy) {
+// then the peak-surge maximum (src/write_output.F:6357-6366).  FP32
+// contributions (the mixed path) are summed with compensation.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm q, const CT* __restrict__ cx,
+                           const CT* __restrict__ cy) {
   // the block's 256 nodes own one contiguous range of incidence slots: it is
   // read once, coalesced, into LDS, and each node then sums its own slots
   // from there in order (the same sums as reading them directly)
-  __shared__ double sx[NODE_STAGE], sy[NODE_STAGE];
+  __shared__ CT sx[NODE_STAGE], sy[NODE_STAGE];
   const int r0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
   const int i = r0 + threadIdx.x;
   const int p0 = m.inc_ptr[r0];
@@ -1248,7 +1530,17 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
   const double DTO2 = m.dt / 2.0;
   const int nci = s.nodecode[i];
   double mx = 0.0, my = 0.0;
-  if (staged) {
+  if (sizeof(CT) == 4) {
+    Kahan32 kx, ky;
+    const CT* px = staged ? sx : cx + p0;
+    const CT* py = staged ? sy : cy + p0;
+    for (int p = m.inc_ptr[i] - p0; p < m.inc_ptr[i + 1] - p0; ++p) {
+      kx.add((float)px[p]);
+      ky.add((float)py[p]);
+    }
+    mx = kx.value();
+    my = ky.value();
+  } else if (staged) {
     for (int p = m.inc_ptr[i] - p0; p < m.inc_ptr[i + 1] - p0; ++p) {
       mx = mx + sx[p];
       my = my + sy[p];
@@ -1349,6 +1641,9 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
     }
   }
 }
+
+template __global__ void k_mom_node<double>(DevMesh, DevState, MomPrm, const double*, const double*);
+template __global__ void k_mom_node<float>(DevMesh, DevState, MomPrm, const float*, const float*);
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* p, unsigned v, int n) {
   int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;

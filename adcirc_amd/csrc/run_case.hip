@@ -5,10 +5,14 @@
 //   adcgpu_create -> adcgpu_cold_start -> adcgpu_holland (time 0)
 //   -> adcgpu_run (warm-up steps) -> adcgpu_run (timed steps)
 //
-//   adcgpu_run_case CASE [--warmup W] [--steps K] [--chunk C] [--copy-mib N]
+//   adcgpu_run_case CASE [--warmup W] [--steps K] [--chunk C] [--convcr TOL]
+//                   [--precision 0|1] [--copy-mib N]
 //   adcgpu_run_case --copy-mib N
 //
 // Prints one line of timing: steps, ms per step, JCG iterations per step.
+// --convcr replaces the case's JCG tolerance (a different iteration count per
+// launch, for fitting per-iteration and per-launch HBM traffic); --precision
+// selects ADCGPU_FP64 (0) or ADCGPU_MIXED (1).
 // --copy-mib runs a plain device-to-device copy kernel over N MiB ten times
 // (after the case, when one is given):
 // the reference reading of the HBM traffic counters (FETCH_SIZE, WRITE_SIZE)
@@ -112,17 +116,22 @@ int check(int rc, const char* what) {
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: %s CASE [--warmup W] [--steps K] [--chunk C] | --copy-mib N\n",
+    std::fprintf(stderr,
+                 "usage: %s CASE [--warmup W] [--steps K] [--chunk C] [--convcr TOL] "
+                 "[--precision 0|1] [--copy-mib N] | --copy-mib N\n",
                  argv[0]);
     return 2;
   }
   if (argc == 3 && !std::strcmp(argv[1], "--copy-mib")) return copy_probe(std::atoi(argv[2]));
-  int warmup = 10, steps = 40, chunk = 200, copy_mib = 0;
+  int warmup = 10, steps = 40, chunk = 200, copy_mib = 0, precision = -1;
+  double convcr = 0.0;  // > 0: replaces the case's JCG tolerance
   for (int a = 2; a + 1 < argc; a += 2) {
     if (!std::strcmp(argv[a], "--warmup")) warmup = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--steps")) steps = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--chunk")) chunk = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--copy-mib")) copy_mib = std::atoi(argv[a + 1]);
+    else if (!std::strcmp(argv[a], "--convcr")) convcr = std::atof(argv[a + 1]);
+    else if (!std::strcmp(argv[a], "--precision")) precision = std::atoi(argv[a + 1]);
   }
   std::map<std::string, Record> c;
   if (!read_case(argv[1], c)) {
@@ -152,6 +161,8 @@ int main(int argc, char** argv) {
   p.warm_start = c.count("warm_start") ? I("warm_start") : 0;
   p.errorelev = c.count("errorelev") ? D("errorelev") : 0.0;
   p.tau0var = DA("tau0var"); p.evm = DA("evm"); p.fric = DA("fric"); p.corif = DA("corif");
+  if (convcr > 0.0) p.convcr = convcr;
+  if (precision >= 0) p.precision = precision;
 
   const std::vector<double>& table = need(c, "eta_bc_table", 0)->d;
   const int neta = m.neta > 0 ? m.neta : 1;

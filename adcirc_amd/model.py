@@ -273,6 +273,14 @@ class Model:
               "run_stats")
         return tuple(int(v) for v in out)
 
+    def last_solve(self):
+        """(STPTST the last JCG solve ended with, its bound ZETA, squared
+        initial pseudo-residual norms of the reference's and of the learned
+        start, or 0 without the learned start)."""
+        out = np.zeros(4, np.float64)
+        check(self.lib.adcgpu_last_solve(self.h, out.ctypes.data_as(dp_)), "last_solve")
+        return tuple(float(v) for v in out)
+
     # ------------------------------------------------------------- snapshot
     def save_state(self) -> np.ndarray:
         """The whole state between steps (adcgpu_state_save), as bytes."""

@@ -190,6 +190,13 @@ int adcgpu_run(adcgpu_ctx *ctx, int it0, int nsteps, const double *eta_bc_table,
  * run-time error (0 if none), out[4] solves that started from the learned
  * guess (warm_start = 2). */
 int adcgpu_run_stats(adcgpu_ctx *ctx, int64_t *out5);
+/* The last JCG solve's end: out[0] the stopping-test value STPTST it ended
+ * with and out[1] the bound ZETA it was held to (pstop, src/itpackv.F:721-806:
+ * converged when out[0] < out[1]); out[2], out[3] the squared norms of the
+ * initial pseudo-residuals of the reference's start and of the learned start
+ * (warm_start = 2; the smaller one was used), 0 otherwise.  Synchronises the
+ * context's stream.  No counterpart in the reference: a reporting call. */
+int adcgpu_last_solve(adcgpu_ctx *ctx, double *out4);
 
 /* ---- learned JCG start (warm_start = 2) ----
  * No counterpart in the reference: these replace only the initial guess of

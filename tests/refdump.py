@@ -19,7 +19,8 @@ REPO = os.path.dirname(HERE)
 REF_EXE = os.path.join(REPO, "oracle", "_ref", "adchp_ref")
 
 
-def write_inputs(case, workdir: str, nsteps: int, dump_every: int = 0) -> None:
+def write_inputs(case, workdir: str, nsteps: int, dump_every: int = 0,
+                 hash_steps: bool = True) -> None:
     os.makedirs(workdir, exist_ok=True)
     write_fort14(os.path.join(workdir, "fort.14"), case.mesh)
     r = case.run
@@ -45,7 +46,7 @@ def write_inputs(case, workdir: str, nsteps: int, dump_every: int = 0) -> None:
         "nws": str(r.nws), "nbfr": str(r.nbfr), "anginn_in": fl(r.anginn),
         "amig_in": ",".join(fl(v) for v in r.amig), "ff_in": ",".join(fl(v) for v in r.ff),
         "face_in": ",".join(fl(v) for v in r.face), "hprm": ",".join(fl(v) for v in hol),
-        "ilump_in": str(r.ilump),
+        "ilump_in": str(r.ilump), "hash_steps": "1" if hash_steps else "0",
     }
     with open(os.path.join(workdir, "run.nml"), "w") as f:
         f.write("&run\n")

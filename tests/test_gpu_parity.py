@@ -128,23 +128,25 @@ def test_holland_forcing_vs_oracle():
             assert err <= 1e-13, (t, name)
 
 
-def _resident_vs_per_call(case, n):
+def _resident_vs_per_call(case, n, precision=None):
     """Run ``n`` steps through the resident run and the same steps one ABI
     call at a time; return the resident model's COEF rebuild count after
     checking every state array, COEF and ALPHAL identical."""
     import ctypes as C
     from gpu_helpers import case_setup, golden_tables
+    from adcirc_amd import _lib
     from adcirc_amd.model import Model
     d = case_setup(case)
     run = case.run
     t = golden_tables(d)
     nb = run.nbfr
+    prec = _lib.FP64 if precision is None else precision
 
     def model():
         g = Model(t, run, tau0var=d["tau0var"], evm=d["evm"], fric=d["fric"],
                   corif=np.full(t["np"], run.cori),
                   emo=np.asarray(d["emo"]).reshape(t["neta"], nb),
-                  efa=np.asarray(d["efa"]).reshape(t["neta"], nb))
+                  efa=np.asarray(d["efa"]).reshape(t["neta"], nb), precision=prec)
         g.cold_start(d["eta2_0"], d["nodecode0"], d["noff0"])
         g.holland(0.0, run.holland)
         return g
