@@ -103,11 +103,16 @@ struct adcgpu_ctx {
   double *lhs_td = nullptr, *lhs_ta = nullptr, *lhs_tb = nullptr;
   unsigned char* ncele = nullptr;
   bool ncele_fresh = false;
-  // mixed precision (ADCGPU_MIXED): FP32 element geometry, cached matrix
-  // terms and scaled off-diagonals; contrib, cx and cy then hold floats
+  // mixed precision (ADCGPU_MIXED): FP32 element geometry and cached matrix
+  // terms; contrib, cx and cy then hold floats
   bool mixed = false;
   float *fdx32 = nullptr, *fdy32 = nullptr, *areas32 = nullptr;
-  float *lhs_td32 = nullptr, *lhs_ta32 = nullptr, *lhs_tb32 = nullptr, *cs32 = nullptr;
+  float *lhs_td32 = nullptr, *lhs_ta32 = nullptr, *lhs_tb32 = nullptr;
+  // the cached terms per off-diagonal entry (k_lhs_pairs; ILump = 0)
+  bool lhs_pairs = false;
+  double *lp_t1 = nullptr, *lp_t2 = nullptr;
+  float *lp_t1f = nullptr, *lp_t2f = nullptr;
+  int *lp_e1 = nullptr, *lp_e2 = nullptr;
   // learned JCG start (set up on first use): FP32 master parameters and Adam
   // moments, the kernels' weight layouts, the per-step correction and second
   // start, training samples and the training work arrays
@@ -115,6 +120,8 @@ struct adcgpu_ctx {
   float *ml_master = nullptr, *ml_m1 = nullptr, *ml_m2 = nullptr;
   MlWeights mlw{};
   double *ml_corr = nullptr, *ml_u = nullptr, *jd2 = nullptr;
+  bf16* ml_fx = nullptr;      // the network's inputs, operand layout (k_ml_feat)
+  double* ml_sv = nullptr;    // the nodes' input scales
   MlSamples smp{};
   int64_t smp_total = 0;  // samples collected so far (the ring keeps the last cap)
   int ml_collect = 0;     // samples per step while collecting (0: off)
@@ -240,7 +247,6 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   // the learned second start, formed by the pass before (learned mode)
   a.u2 = u2;
   a.d2 = c->jd2;
-  a.cs32 = c->cs32;
   // a prescaling pass (k_rhs_node) has already cleared the barrier words
   if (!prescaled) {
     const int bar_words = jcg_barrier_words(c);
@@ -253,8 +259,7 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
       if (int rc = grow_events(c, c->jcg_events.size() + 256)) return rc;
     HIPCHK(hipEventRecord(c->jcg_events[c->ev_used].first, c->stream));
   }
-  const void* fn = c->mixed ? (u2 ? (const void*)k_jcg<true, float> : (const void*)k_jcg<false, float>)
-                            : (u2 ? (const void*)k_jcg<true, double> : (const void*)k_jcg<false, double>);
+  const void* fn = u2 ? (const void*)k_jcg<true> : (const void*)k_jcg<false>;
   HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->jcg_grid), dim3(JCG_BLOCK), args, 0, c->stream));
   if (c->profile) HIPCHK(hipEventRecord(c->jcg_events[c->ev_used++].second, c->stream));
   return 0;
@@ -314,6 +319,7 @@ int ml_setup(adcgpu_ctx* c) {
   A(&c->mlw.w2n, ML_W2); A(&c->mlw.b1p, 128); A(&c->mlw.b2p, 128); A(&c->mlw.w3p, 128);
   A(&c->mlw.b3, 1);
   A(&c->ml_corr, c->np); A(&c->ml_u, c->np); A(&c->jd2, c->np);
+  A(&c->ml_fx, (size_t)((c->np + 31) / 32) * 3 * 64 * 8); A(&c->ml_sv, c->np);
   c->smp.cap = ML_CAP;
   A(&c->smp.x, (size_t)ML_CAP * ML_F); A(&c->smp.s, ML_CAP); A(&c->smp.e0, ML_CAP);
   A(&c->smp.node, ML_CAP); A(&c->smp.y, ML_CAP);
@@ -353,6 +359,17 @@ MlIn ml_in(const adcgpu_ctx* c, const double* eta) {
   in.nodecode = c->nodecode;
   in.dt = c->prm.dtdp;
   return in;
+}
+
+// The network's correction for every node into ml_corr: the inputs pass, then
+// the network pass (grid-stride; its LDS and registers allow three blocks
+// per CU, so three per CU of the 256 are launched).
+void ml_infer(adcgpu_ctx* c, const MlIn& in) {
+  const int ntile = (c->np + 31) / 32;
+  bf16x8* fx = reinterpret_cast<bf16x8*>(c->ml_fx);
+  hipLaunchKernelGGL(k_ml_feat, dim3(nblk(ntile, 4)), dim3(256), 0, c->stream, in, fx, c->ml_sv);
+  hipLaunchKernelGGL(k_ml_mlp, dim3(std::min(nblk(ntile, 4), 3 * 256)), dim3(256), 0, c->stream,
+                     in, c->mlw, (const bf16x8*)fx, (const double*)c->ml_sv, c->ml_corr);
 }
 
 // One minibatch through the network and back; `perm` (device) names its
@@ -616,7 +633,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     for (int e = 0; e < ne; ++e) f32[e] = (float)var[e];
     U(&c->areas32, f32);
     A(&c->lhs_td32, 3 * (size_t)ne); A(&c->lhs_ta32, 3 * (size_t)ne);
-    A(&c->lhs_tb32, 3 * (size_t)ne); A(&c->cs32, (size_t)mnei * np);
+    A(&c->lhs_tb32, 3 * (size_t)ne);
   }
   A(&c->ep_part, c->nch); A(&c->scal, S_COUNT); A(&c->eta_bc, std::max(neta, 1));
   {
@@ -682,6 +699,50 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
+  }
+  if (!rc && P->ilump == 0) {
+    // the terms per off-diagonal entry: slot p's ta term goes to column ja,
+    // its tb term to jb, in ascending element order; more than two
+    // contributions to one entry (a non-manifold edge) keep the row pass
+    const size_t nq = (size_t)(mnei - 1) * np;
+    std::vector<int> pc1(nq, -1), pc2(nq, -1);
+    bool ok = true;
+    for (int i = 0; i < np && ok; ++i)
+      for (int p = cnt[i]; p < cnt[i + 1] && ok; ++p) {
+        const int jj[2] = {slot_cols[p] & 0xffff, slot_cols[p] >> 16};
+        for (int w = 0; w < 2; ++w) {
+          const size_t q = (size_t)(jj[w] - 1) * np + i;
+          const int code = 2 * p + w;
+          if (pc1[q] < 0) pc1[q] = code;
+          else if (pc2[q] < 0) pc2[q] = code;
+          else ok = false;
+        }
+      }
+    if (ok) {
+      // the codes are needed only here: uploaded outside the context's list
+      // of allocations and freed below
+      int *d1 = nullptr, *d2 = nullptr;
+      if (!rc) rc = upload(&d1, pc1);
+      if (!rc) rc = upload(&d2, pc2);
+      if (c->mixed) {
+        A(&c->lp_t1f, nq); A(&c->lp_t2f, nq);
+      } else {
+        A(&c->lp_t1, nq); A(&c->lp_t2, nq);
+      }
+      A(&c->lp_e1, nq); A(&c->lp_e2, nq);
+      if (!rc) {
+        const LhsTerms lt{c->lhs_td, c->lhs_ta, c->lhs_tb, c->lhs_td32, c->lhs_ta32, c->lhs_tb32, 0};
+        hipLaunchKernelGGL(k_lhs_pair_setup, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                           c->stream, c->mesh(), lt, (const int*)d1, (const int*)d2, c->lp_t1,
+                           c->lp_t2, c->lp_t1f, c->lp_t2f, c->lp_e1, c->lp_e2);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, hipGetErrorString(e));
+        c->lhs_pairs = rc == 0;
+      }
+      if (d1) (void)hipFree(d1);
+      if (d2) (void)hipFree(d2);
+    }
   }
   if (!rc && P->warm_start == 2) rc = ml_setup(c);
   if (rc) {
@@ -891,8 +952,8 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
                        c->state());
     c->ncele_fresh = true;
   }
-  const LhsTerms lt{c->lhs_td, c->lhs_ta, c->lhs_tb, c->lhs_td32, c->lhs_ta32, c->lhs_tb32};
-  const float* rcs32 = c->coef_scaled ? (const float*)c->cs32 : nullptr;
+  const LhsTerms lt{c->lhs_td,   c->lhs_ta,   c->lhs_tb,       c->lhs_td32,
+                    c->lhs_ta32, c->lhs_tb32, c->lhs_pairs ? 1 : 0};
   // contrib, cx and cy are free until the GWCE right-hand side
   if (!lumped)  // the element pass only runs while some ALPHAL differs from 1
     hipLaunchKernelGGL(k_lhs_elem<false>, dim3(nblk(ne, 256)), dim3(256), 0, c->stream,
@@ -901,10 +962,10 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   if (c->mixed) {
     if (lumped)
       hipLaunchKernelGGL((k_lhs_rows<true, float>), rg, rb, rows_lds, c->stream, c->mesh(),
-                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs32, rw);
+                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs, rw);
     else
       hipLaunchKernelGGL((k_lhs_rows<false, float>), rg, rb, rows_lds, c->stream, c->mesh(),
-                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs32, rw);
+                         c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs, rw);
   } else {
     if (lumped)
       hipLaunchKernelGGL((k_lhs_rows<true, double>), rg, rb, rows_lds, c->stream, c->mesh(),
@@ -912,6 +973,19 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
     else
       hipLaunchKernelGGL((k_lhs_rows<false, double>), rg, rb, rows_lds, c->stream, c->mesh(),
                          c->state(), c->flags, c->contrib, c->cx, c->cy, lt, c->ep_part, rcs, rw);
+  }
+  if (c->lhs_pairs) {
+    // the cached rebuild (k_lhs_rows above only restored or left at once)
+    const LhsPairs lp{c->lp_t1, c->lp_t2, c->lp_t1f, c->lp_t2f, c->lp_e1, c->lp_e2};
+    const dim3 pg(nblk(np, 256), c->mnei);
+    if (c->mixed)
+      hipLaunchKernelGGL(k_lhs_pairs<float>, pg, dim3(256), 0, c->stream, c->mesh(), c->state(),
+                         (const int*)c->flags, lt, lp);
+    else
+      hipLaunchKernelGGL(k_lhs_pairs<double>, pg, dim3(256), 0, c->stream, c->mesh(), c->state(),
+                         (const int*)c->flags, lt, lp);
+    hipLaunchKernelGGL(k_lhs_ep, rg, rb, 0, c->stream, c->mesh(), c->state(),
+                       (const int*)c->flags, c->ep_part);
   }
   c->coef_scaled = false;
   RhsPrm q{c->ifnlfa, c->ifnlct, c->ifnlcat, lumped ? 4.0 : 2.0, lumped ? 0.0 : 1.0,
@@ -936,9 +1010,7 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
   const int slot0 = collect ? (int)(c->smp_total % c->smp.cap) : 0;
   if (learned || collect) {
     const MlIn in = ml_in(c, c->eta1);
-    if (learned)
-      hipLaunchKernelGGL(k_ml_infer, dim3(std::min(nblk(np, 128), 4 * 256)), dim3(256), 0,
-                         c->stream, in, c->mlw, c->ml_corr);
+    if (learned) ml_infer(c, in);
     if (collect)
       hipLaunchKernelGGL(k_ml_collect, dim3(nblk(c->ml_collect, 128)), dim3(256), 0, c->stream,
                          in, c->smp, slot0, c->ml_collect, c->cur_it);
@@ -982,12 +1054,8 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
 // COEF with the off-diagonals of a deferred unscaling restored.
 static int restore_coef(adcgpu_ctx* c) {
   if (!c->coef_scaled) return 0;
-  if (c->mixed)
-    hipLaunchKernelGGL(k_coef_restore<float>, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream,
-                       c->mesh(), c->state(), (const float*)c->cs32, (const double*)c->jw);
-  else
-    hipLaunchKernelGGL(k_coef_restore<double>, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream,
-                       c->mesh(), c->state(), (const double*)c->cs, (const double*)c->jw);
+  hipLaunchKernelGGL(k_coef_restore, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->mesh(),
+                     c->state(), (const double*)c->cs, (const double*)c->jw);
   c->coef_scaled = false;
   return check_launch("coef restore");
 }
@@ -1337,8 +1405,7 @@ int adcgpu_ml_correction(adcgpu_ctx* c, double* corr_out) {
   if (!c || !corr_out) return fail(ADCGPU_E_ARG, "null argument");
   if (int rc = ml_setup(c)) return rc;
   const MlIn in = ml_in(c, c->eta2);
-  hipLaunchKernelGGL(k_ml_infer, dim3(std::min(nblk(c->np, 128), 4 * 256)), dim3(256), 0,
-                     c->stream, in, c->mlw, c->ml_corr);
+  ml_infer(c, in);
   HIPCHK(hipMemcpyAsync(corr_out, c->ml_corr, (size_t)c->np * 8, hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

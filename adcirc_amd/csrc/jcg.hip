@@ -310,12 +310,7 @@ __device__ void jcg_parcon(ItLocal& L, double dtnrm) {
 // one); the pseudo-residuals of both are formed in the scaling pass and the
 // iteration starts from the one with the smaller norm (the reference's on a
 // tie).  Everything after that choice is the same ITPACK iteration.
-// MT: the type the scaled off-diagonals are kept in between iterations;
-// float halves the matrix bytes each iteration reads (vectors, dot products
-// and the Chebyshev parameters stay FP64), and the initial pseudo-residual is
-// formed from the rounded values so the iteration solves one consistent
-// system.
-template <bool DUAL, typename MT>
+template <bool DUAL>
 __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   __shared__ ItLocal L;
   __shared__ double lds[12];
@@ -337,7 +332,6 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   const int base = c * RED_CHUNK + t % RED_THREADS;
   unsigned gen = 0;
   int* err = a.flags + F_ERROR;
-  MT* const csp = reinterpret_cast<MT*>(sizeof(MT) == 4 ? (void*)a.cs32 : (void*)a.cs);
 
   if (t == 0) {
     L.in = 0;
@@ -460,9 +454,8 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
 #pragma unroll
         for (int r = h; r < h + 4; ++r) {
           if (j < NN(r)) {
-            const MT csm = (MT)(kc[r - h] * wi[r] * at(a.w, cc[r - h]));
-            at(csp, q0 + RED_THREADS * r) = csm;
-            const double cs = (double)csm;
+            const double cs = kc[r - h] * wi[r] * at(a.w, cc[r - h]);
+            at(a.cs, q0 + RED_THREADS * r) = cs;
             dc[r] = dc[r] - cs * at(a.u, cc[r - h]);
             if (DUAL) d2c[r] = d2c[r] - cs * at(a.u2, cc[r - h]);
           }
@@ -568,7 +561,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
           jm = NN(r) > jm ? NN(r) : jm;
           const unsigned q = (unsigned)(np + base + RED_THREADS * r);
           cn[r - h] = 1 < NN(r) ? (unsigned)at(a.col, q) : 0u;
-          sn[r - h] = 1 < NN(r) ? (double)at(csp, q) : 0.0;
+          sn[r - h] = 1 < NN(r) ? at(a.cs, q) : 0.0;
         }
         for (int j = 1; j < jm; ++j) {
           unsigned cc[4];
@@ -584,7 +577,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
             for (int r = h; r < h + 4; ++r) {
               if (j + 1 < NN(r)) {
                 cn[r - h] = (unsigned)at(a.col, q0 + RED_THREADS * r);
-                sn[r - h] = (double)at(csp, q0 + RED_THREADS * r);
+                sn[r - h] = at(a.cs, q0 + RED_THREADS * r);
               }
             }
           }
@@ -718,7 +711,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       for (int r = 0; r < RED_PER_THREAD; ++r) {
         if (j < NN(r)) {
           const unsigned q = q0 + RED_THREADS * r;
-          at(a.coef, q) = (double)at(csp, q) * ri[r] * (1.0 / at(a.w, (unsigned)at(a.col, q)));
+          at(a.coef, q) = at(a.cs, q) * ri[r] * (1.0 / at(a.w, (unsigned)at(a.col, q)));
         }
       }
     }
@@ -739,9 +732,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   }
 }
 
-template __global__ void k_jcg<false, double>(JcgArgs);
-template __global__ void k_jcg<true, double>(JcgArgs);
-template __global__ void k_jcg<false, float>(JcgArgs);
-template __global__ void k_jcg<true, float>(JcgArgs);
+template __global__ void k_jcg<false>(JcgArgs);
+template __global__ void k_jcg<true>(JcgArgs);
 
 }  // namespace adcgpu

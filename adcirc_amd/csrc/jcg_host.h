@@ -42,10 +42,9 @@ inline hipError_t jcg_resident_cap_of(const void* fn, int* per_cu_out) {
   return hipSuccess;
 }
 
-// Every variant of the kernel (one start or two; FP64 or FP32 matrix) must fit.
+// Both variants of the kernel (one start, two starts) must fit.
 inline hipError_t jcg_resident_cap(int device, int* cap) {
-  const void* fns[] = {(const void*)k_jcg<false, double>, (const void*)k_jcg<true, double>,
-                       (const void*)k_jcg<false, float>, (const void*)k_jcg<true, float>};
+  const void* fns[] = {(const void*)k_jcg<false>, (const void*)k_jcg<true>};
   int per_cu = 1 << 30;
   hipError_t e = hipSuccess;
   for (const void* fn : fns) {

@@ -108,6 +108,19 @@ struct DevState {
 struct LhsTerms {
   const double *td, *ta, *tb;  // [3 ne] diagonal and the two off-diagonal terms
   const float *td32, *ta32, *tb32;  // the same rounded to FP32 (mixed precision)
+  int pairs;  // 1: a cached rebuild is k_lhs_pairs' and k_lhs_ep's (LhsPairs)
+};
+
+// The same terms gathered per off-diagonal entry: entry (i, j >= 1) of COEF
+// receives, in ascending element order, the terms of the (at most two)
+// elements that hold the edge i-j, so a rebuild is
+// COEF(i, j) = (0 + t1 NCEle(e1)) + t2 NCEle(e2), one thread per entry with
+// no accumulation in between.  Index (j - 1) NP + i; an absent second term is
+// a zero term of element 0.
+struct LhsPairs {
+  const double *t1, *t2;
+  const float *t1f, *t2f;  // mixed precision: the FP32 terms
+  const int *e1, *e2;
 };
 
 // Compensated (Kahan 1965) summation in FP32: the node sums of FP32
@@ -202,7 +215,6 @@ struct JcgArgs {
   const int* nodecode;
   const double* u2;   // k_jcg<true>: the second (learned) start, scaled like u
   double* d2;         //              and its initial pseudo-residual
-  float* cs32;        // k_jcg<., float>: the scaled off-diagonals rounded to FP32
 };
 
 __global__ void k_init_h(int np, const double* dp, const double* eta2, double ifnlfa, double* h1,
@@ -217,13 +229,12 @@ __global__ void k_friction_holland(int np, FricPrm f, double* tk, const double* 
 template <bool LUMPED>
 __global__ void k_lhs_elem(DevMesh m, DevState s, const int* flags, double* sd, double* sa,
                            double* sb);
-// CT: the type of the matrix terms summed per row and of the scaled
-// off-diagonals a deferred solve left (double; float in the mixed path, which
-// sums FP32 terms with compensation)
+// CT: the type of the matrix terms summed per row (double; float in the
+// mixed path, which sums FP32 terms with compensation)
 template <bool LUMPED, typename CT>
 __global__ void k_lhs_rows(DevMesh m, DevState s, const int* flags, const double* sd,
                            const double* sa, const double* sb, LhsTerms lt, double* ep_part,
-                           const CT* restore_cs, const double* restore_w);
+                           const double* restore_cs, const double* restore_w);
 // the cached terms (once per context): k_lhs_elem's terms with NCEle = 1 and
 // ALPHAL = 1, and their FP32 roundings when td32 != null
 template <bool LUMPED>
@@ -231,8 +242,19 @@ __global__ void k_lhs_terms(DevMesh m, double* td, double* ta, double* tb, float
                             float* ta32, float* tb32);
 // NCEle of every element from the current NODECODE and NOFF
 __global__ void k_ncele(DevMesh m, DevState s);
+// per-entry terms (once per context) from the cached terms and the host's
+// entry codes (2 p + 0/1 for slot p's ta/tb term, -1 for none)
+__global__ void k_lhs_pair_setup(DevMesh m, LhsTerms lt, const int* c1, const int* c2,
+                                 double* t1, double* t2, float* t1f, float* t2f, int* e1,
+                                 int* e2);
+// A rebuild of the consistent GWCE matrix from the cached terms (the common
+// case: NCChange set, every ALPHAL 1): grid (row blocks, MNEI), column j of
+// COEF by blockIdx.y; then the EP chunk partials of the diagonal in the
+// fixed blocked tree.  Both leave at once otherwise (k_lhs_rows does it).
 template <typename CT>
-__global__ void k_coef_restore(DevMesh m, DevState s, const CT* cs, const double* w);
+__global__ void k_lhs_pairs(DevMesh m, DevState s, const int* flags, LhsTerms lt, LhsPairs lp);
+__global__ void k_lhs_ep(DevMesh m, DevState s, const int* flags, double* ep_part);
+__global__ void k_coef_restore(DevMesh m, DevState s, const double* cs, const double* w);
 // one block more than the elements need: the last finishes the GWCE matrix
 __global__ void k_rhs_elem(DevMesh m, DevState s, RhsPrm q, int* flags, double* contrib, int nch,
                            const double* ep_part, double* scal);
@@ -256,10 +278,8 @@ struct JcgPre {
 template <bool LUMPED, typename CT>
 __global__ void k_rhs_node(DevMesh m, DevState s, const CT* contrib, const double* eta_bc,
                            const double* scal, JcgPre pre);
-// DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2;
-// MT: the type of the scaled off-diagonals the iterations stream (float in the
-// mixed path, a.cs32; the vectors, dot products and norms stay FP64)
-template <bool DUAL, typename MT>
+// DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2
+template <bool DUAL>
 __global__ void k_jcg(JcgArgs a);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
                                                           const double* __restrict__ sb,
                                                           LhsTerms lt,
                                                           double* __restrict__ ep_part,
-                                                          const CT* __restrict__ restore_cs,
+                                                          const double* __restrict__ restore_cs,
                                                           const double* __restrict__ restore_w) {
   constexpr bool MIXED = sizeof(CT) == 4;
   __shared__ double lds[8];
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
         const double ri = 1.0 / restore_w[i];
         for (int j = 1; j < nn; ++j) {
           const size_t q = (size_t)j * np + i;
-          s.coef[q] = (double)restore_cs[q] * ri * (1.0 / restore_w[m.col[q]]);
+          s.coef[q] = restore_cs[q] * ri * (1.0 / restore_w[m.col[q]]);
         }
       }
     return;
@@ -264,6 +264,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
   // the element pass has written the slot values); the lumped matrix has no
   // ALPHAL term
   const bool cached = LUMPED || flags[F_ALPHA] == 0;
+  if (!LUMPED && cached && lt.pairs) return;  // k_lhs_pairs and k_lhs_ep do it
   double acc = 0.0;
   for (int r = 0; r < RED_PER_THREAD; ++r) {
     const int i = base + RED_THREADS * r;
@@ -350,8 +351,87 @@ __global__ __launch_bounds__(RED_THREADS) void k_lhs_rows(DevMesh m, DevState s,
 }
 template __global__ void k_lhs_rows<false, double>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
 template __global__ void k_lhs_rows<true, double>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
-template __global__ void k_lhs_rows<false, float>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const float*, const double*);
-template __global__ void k_lhs_rows<true, float>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const float*, const double*);
+template __global__ void k_lhs_rows<false, float>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
+template __global__ void k_lhs_rows<true, float>(DevMesh, DevState, const int*, const double*, const double*, const double*, LhsTerms, double*, const double*, const double*);
+
+__global__ __launch_bounds__(256) void k_lhs_pair_setup(DevMesh m, LhsTerms lt,
+                                                        const int* __restrict__ c1,
+                                                        const int* __restrict__ c2,
+                                                        double* __restrict__ t1,
+                                                        double* __restrict__ t2,
+                                                        float* __restrict__ t1f,
+                                                        float* __restrict__ t2f,
+                                                        int* __restrict__ e1, int* __restrict__ e2) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)(m.mnei - 1) * m.np) return;
+  const int a = c1[q], b = c2[q];
+  // the term of code c: slot c/2's ta (c even) or tb (c odd) term
+  if (t1) {
+    t1[q] = a < 0 ? 0.0 : ((a & 1) ? lt.tb : lt.ta)[a >> 1];
+    t2[q] = b < 0 ? 0.0 : ((b & 1) ? lt.tb : lt.ta)[b >> 1];
+  }
+  if (t1f) {
+    t1f[q] = a < 0 ? 0.0f : ((a & 1) ? lt.tb32 : lt.ta32)[a >> 1];
+    t2f[q] = b < 0 ? 0.0f : ((b & 1) ? lt.tb32 : lt.ta32)[b >> 1];
+  }
+  e1[q] = a < 0 ? 0 : (int)((unsigned)m.inc[a >> 1] / 3u);
+  e2[q] = b < 0 ? 0 : (int)((unsigned)m.inc[b >> 1] / 3u);
+}
+
+// COEF(i, j) of a cached rebuild.  Column 0 (the diagonal) sums the row's
+// slots in ascending element order from +0, as k_lhs_rows does; an
+// off-diagonal is (+0 + t1 NCEle) + t2 NCEle, the same operations as
+// k_lhs_rows' accumulation of its one or two contributions (an absent second
+// term adds +0, which changes no sum that started from +0).  The mixed path
+// (CT = float) takes FP32 terms and sums in FP64.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_lhs_pairs(DevMesh m, DevState s,
+                                                   const int* __restrict__ flags, LhsTerms lt,
+                                                   LhsPairs lp) {
+  constexpr bool MIXED = sizeof(CT) == 4;
+  if (flags[F_NCCHANGE] == 0 || flags[F_ALPHA] != 0) return;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int np = m.np;
+  if (i >= np || j >= m.nneigh[i]) return;
+  if (j == 0) {
+    double diag = 0.0;
+    for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) {
+      const double nce = (double)s.ncele[(unsigned)m.inc[p] / 3u];
+      diag = diag + (MIXED ? (double)lt.td32[p] : lt.td[p]) * nce;
+    }
+    s.coef[i] = diag;
+    return;
+  }
+  const size_t q = (size_t)(j - 1) * np + i;
+  const double n1 = (double)s.ncele[lp.e1[q]], n2 = (double)s.ncele[lp.e2[q]];
+  const double a = MIXED ? (double)lp.t1f[q] : lp.t1[q];
+  const double b = MIXED ? (double)lp.t2f[q] : lp.t2[q];
+  s.coef[(size_t)j * np + i] = (0.0 + a * n1) + b * n2;
+}
+template __global__ void k_lhs_pairs<double>(DevMesh, DevState, const int*, LhsTerms, LhsPairs);
+template __global__ void k_lhs_pairs<float>(DevMesh, DevState, const int*, LhsTerms, LhsPairs);
+
+// The EP chunk partials of a cached rebuild: sum COEF(i,1)^2 in k_lhs_rows'
+// fixed blocked tree (thread t of chunk c adds rows c 2048 + t + 512 r,
+// r = 0..3, in order; then the chunk tree).
+__global__ __launch_bounds__(RED_THREADS) void k_lhs_ep(DevMesh m, DevState s,
+                                                        const int* __restrict__ flags,
+                                                        double* __restrict__ ep_part) {
+  __shared__ double lds[8];
+  if (flags[F_NCCHANGE] == 0 || flags[F_ALPHA] != 0) return;
+  const int chunk = xcd_tile(blockIdx.x, gridDim.x);
+  const int base = chunk * RED_CHUNK + threadIdx.x;
+  double acc = 0.0;
+  for (int r = 0; r < RED_PER_THREAD; ++r) {
+    const int i = base + RED_THREADS * r;
+    if (i >= m.np) break;
+    const double d = s.coef[i];
+    acc = acc + d * d;
+  }
+  const double tot = chunk_tree(acc, lds);
+  if (threadIdx.x == 0) ep_part[chunk] = tot;
+}
 
 // EP = sqrt(sum of squared diagonals / NP) from the chunk partials, and the
 // elevation-boundary rows (src/gwce.F:700-743): a boundary row becomes EP on
@@ -396,9 +476,8 @@ __device__ void lhs_bc_block(const DevMesh& m, DevState s, int nch,
 
 // The off-diagonals of a solve whose unscaling was deferred, restored as
 // unscal does (src/itpackv.F:2460-2479).
-template <typename CT>
 __global__ __launch_bounds__(256) void k_coef_restore(DevMesh m, DevState s,
-                                                      const CT* __restrict__ cs,
+                                                      const double* __restrict__ cs,
                                                       const double* __restrict__ w) {
   const int i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= m.np) return;
@@ -406,11 +485,9 @@ __global__ __launch_bounds__(256) void k_coef_restore(DevMesh m, DevState s,
   const double ri = 1.0 / w[i];
   for (int j = 1; j < nn; ++j) {
     const size_t q = (size_t)j * np + i;
-    s.coef[q] = (double)cs[q] * ri * (1.0 / w[m.col[q]]);
+    s.coef[q] = cs[q] * ri * (1.0 / w[m.col[q]]);
   }
 }
-template __global__ void k_coef_restore<double>(DevMesh, DevState, const double*, const double*);
-template __global__ void k_coef_restore<float>(DevMesh, DevState, const float*, const double*);
 
 // --------------------------------------------------------------- GWCE RHS
 // Element contributions Temp_LV_A1..3 * NCEle (src/gwce.F:1000-1615).  Terms

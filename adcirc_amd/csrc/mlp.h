@@ -24,8 +24,8 @@ constexpr int ML_OFF_W1 = 0, ML_OFF_B1 = ML_W1, ML_OFF_W2 = ML_OFF_B1 + ML_H,
 typedef __bf16 bf16;
 
 // Weights in the layouts the kernels read (written by k_ml_pack):
-//  - w1p, w2p: the matrix-core operand fragments of the fused inference
-//    kernel, lane by lane (see k_ml_infer);
+//  - w1p, w2p: the matrix-core operand fragments of the inference kernel,
+//    lane by lane (see k_ml_mlp);
 //  - b1p, b2p, w3p: biases and output weights in the same lane order;
 //  - w1t [128][48], w2t [128 out][128 in], w2n [128 in][128 out]: row-major
 //    copies for the training GEMMs (K contiguous).
@@ -67,7 +67,11 @@ struct MlSamples {
 };
 
 __global__ void k_ml_pack(const float* master, MlWeights w);
-__global__ void k_ml_infer(MlIn in, MlWeights w, double* corr);
+// inference: the inputs of every node (fx: 3 x 64 fragments of 8 bf16 per
+// tile of 32 nodes; sv: the nodes' scales), then the network over them
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__global__ void k_ml_feat(MlIn in, bf16x8* fx, double* sv);
+__global__ void k_ml_mlp(MlIn in, MlWeights w, const bf16x8* fx, const double* sv, double* corr);
 __global__ void k_ml_collect(MlIn in, MlSamples smp, int slot0, int count, int it);
 __global__ void k_ml_target(MlSamples smp, int slot0, int count, const double* etas_new);
 // training (minibatches of ML_BATCH samples; see mlp.hip)

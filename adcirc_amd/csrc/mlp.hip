@@ -25,7 +25,6 @@
 
 namespace adcgpu {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr float GELU_C0 = 0.7978845608028654f;  // sqrt(2/pi)
@@ -38,9 +37,14 @@ __device__ __forceinline__ float ml_tanh(float v) {
   const float e = __builtin_amdgcn_exp2f(2.0f * LOG2E * v);
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
 }
+// The same function as x / (1 + exp(-2 c0 (x + c1 x^3))), since
+// 0.5 (1 + tanh(z)) = 1 / (1 + exp(-2z)): six VALU operations and two
+// transcendentals per value, the inference kernel's VALU floor.
 __device__ __forceinline__ float ml_gelu(float x) {
-  const float t = ml_tanh(GELU_C0 * (x + GELU_C1 * x * x * x));
-  return 0.5f * x * (1.0f + t);
+  const float x2 = x * x;
+  const float p = __builtin_fmaf(GELU_C1 * x2, x, x);
+  const float e = __builtin_amdgcn_exp2f((-2.0f * GELU_C0 * LOG2E) * p);
+  return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 __device__ __forceinline__ float ml_gelu_grad(float x) {
   const float t = ml_tanh(GELU_C0 * (x + GELU_C1 * x * x * x));
@@ -158,10 +162,44 @@ __global__ __launch_bounds__(256) void k_ml_pack(const float* __restrict__ maste
 }
 
 // ---------------------------------------------------------------- inference
-// Four waves per block, each taking 32 nodes at a time.  The packed W2 sits
-// in LDS (32 KB); W1 stays in registers.  Output: the correction s * y for
-// wet nodes off the elevation boundary, zero elsewhere.
-__global__ __launch_bounds__(256) void k_ml_infer(MlIn in, MlWeights w, double* __restrict__ corr) {
+// Two passes.  The stencil gathers of the inputs are latency-bound and the
+// network is matrix-core and VALU work with a large register footprint, so
+// they run as separate kernels: k_ml_feat at full occupancy, one wave per
+// tile of 32 nodes, stores every node's inputs in the operand layout of the
+// first layer (fx[(tile * 3 + s3) * 64 + lane] = the 8 inputs K = 16 s3 +
+// 8 (lane / 32) .. +7 of node 32 tile + lane % 32) and its scale s; nodes that
+// take no correction (dry, or on the elevation boundary) get zero inputs
+// without gathers.  k_ml_mlp reads them back coalesced (96 bytes per node).
+__device__ __forceinline__ bool ml_use(const MlIn& in, int i) {
+  return in.nodecode[i] == 1 && in.bnd_index[i] < 0;
+}
+
+__global__ __launch_bounds__(256) void k_ml_feat(MlIn in, bf16x8* __restrict__ fx,
+                                                 double* __restrict__ sv) {
+  const int lane = threadIdx.x & 63, h = lane >> 5;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (tile >= (in.np + 31) >> 5) return;  // the whole wave
+  const int i = tile * 32 + (lane & 31);
+  const bool valid = i < in.np;
+  float f[24];
+  double s;
+  ml_features(in, i, h, valid && ml_use(in, i), f, s);
+  bf16x8 x[3];
+#pragma unroll
+  for (int q = 0; q < 24; ++q) x[q >> 3][q & 7] = (bf16)f[q];
+#pragma unroll
+  for (int s3 = 0; s3 < 3; ++s3) fx[((size_t)tile * 3 + s3) * 64 + lane] = x[s3];
+  if (valid && h == 0) sv[i] = s;
+}
+
+// The network over the stored inputs.  Four waves per block, each taking 32
+// nodes at a time; a tile with no node that takes a correction is skipped.
+// The packed weights sit in LDS.  Output: the correction s * y for wet nodes
+// off the elevation boundary, zero elsewhere.
+__global__ __launch_bounds__(256) void k_ml_mlp(MlIn in, MlWeights w,
+                                                const bf16x8* __restrict__ fx,
+                                                const double* __restrict__ sv,
+                                                double* __restrict__ corr) {
   __shared__ bf16x8 w2s[4 * 8 * 64];
   __shared__ bf16x8 w1s[4 * 3 * 64];
   __shared__ float bs[3][128];
@@ -183,12 +221,14 @@ __global__ __launch_bounds__(256) void k_ml_infer(MlIn in, MlWeights w, double* 
   for (int tile = wave; tile < ntile; tile += waves) {
     const int i = tile * 32 + (lane & 31);
     const bool valid = i < in.np;
-    float f[24];
-    double s;
-    ml_features(in, i, h, valid, f, s);
+    const bool use = valid && ml_use(in, i);
+    if (__ballot(use) == 0) {
+      if (valid && h == 0) corr[i] = 0.0;
+      continue;
+    }
     bf16x8 x[3];
 #pragma unroll
-    for (int q = 0; q < 24; ++q) x[q >> 3][q & 7] = (bf16)f[q];
+    for (int s3 = 0; s3 < 3; ++s3) x[s3] = fx[((size_t)tile * 3 + s3) * 64 + lane];
     // layer 1: 4 hidden tiles x 3 K steps, each tile activated at once
     bf16x8 a1[8];
 #pragma unroll
@@ -218,17 +258,14 @@ __global__ __launch_bounds__(256) void k_ml_infer(MlIn in, MlWeights w, double* 
       }
     }
     const float y = part + __shfl_xor(part, 32, 64) + b3;
-    if (valid && h == 0) {
-      const bool use = in.nodecode[i] == 1 && in.bnd_index[i] < 0;
-      corr[i] = use ? s * (double)y : 0.0;
-    }
+    if (valid && h == 0) corr[i] = use ? sv[i] * (double)y : 0.0;
   }
 }
 
 // ---------------------------------------------------------------- training data
 // Sample q of this step is node (it * 7919 + q * stride) mod NP: a spread,
 // deterministic set.  Nodes that are dry or on the elevation boundary are
-// kept with node = -1 (no weight).  Same lane layout as k_ml_infer.
+// kept with node = -1 (no weight).  Same lane layout as k_ml_feat.
 __global__ __launch_bounds__(256) void k_ml_collect(MlIn in, MlSamples smp, int slot0, int count,
                                                     int it) {
   const int lane = threadIdx.x & 63, h = lane >> 5;

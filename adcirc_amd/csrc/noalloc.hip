@@ -244,9 +244,8 @@ int adcgpu_jcg_noalloc(int n, int ndim, int maxnz, int32_t* jcoef, double* coef,
   a.nodecode = nullptr;
   a.u2 = nullptr;
   a.d2 = nullptr;
-  a.cs32 = nullptr;
   void* args[] = {&a};
-  NACHK(hipLaunchCooperativeKernel((const void*)k_jcg<false, double>, dim3(grid), dim3(JCG_BLOCK), args,
+  NACHK(hipLaunchCooperativeKernel((const void*)k_jcg<false>, dim3(grid), dim3(JCG_BLOCK), args,
                                    0, st));
   int f[F_COUNT];
   double saved[8];

@@ -12,7 +12,11 @@
 // Prints one line of timing: steps, ms per step, JCG iterations per step.
 // --convcr replaces the case's JCG tolerance (a different iteration count per
 // launch, for fitting per-iteration and per-launch HBM traffic); --precision
-// selects ADCGPU_FP64 (0) or ADCGPU_MIXED (1).
+// selects ADCGPU_FP64 (0) or ADCGPU_MIXED (1), --warm-start the JCG start.
+// --state FILE --it0 S starts from a snapshot (adcgpu_state_save) taken
+// after step S instead of the cold start, and --params FILE loads the
+// learned start's ADCGPU_ML_NPARAM float32 parameters (adcgpu_ml_params);
+// bench.py --dump-legs writes all three for the legs it times.
 // --copy-mib runs a plain device-to-device copy kernel over N MiB ten times
 // (after the case, when one is given):
 // the reference reading of the HBM traffic counters (FETCH_SIZE, WRITE_SIZE)
@@ -118,13 +122,15 @@ int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr,
                  "usage: %s CASE [--warmup W] [--steps K] [--chunk C] [--convcr TOL] "
-                 "[--precision 0|1] [--copy-mib N] | --copy-mib N\n",
+                 "[--precision 0|1] [--warm-start 0|1|2] [--state FILE --it0 S] "
+                 "[--params FILE] [--copy-mib N] | --copy-mib N\n",
                  argv[0]);
     return 2;
   }
   if (argc == 3 && !std::strcmp(argv[1], "--copy-mib")) return copy_probe(std::atoi(argv[2]));
-  int warmup = 10, steps = 40, chunk = 200, copy_mib = 0, precision = -1;
+  int warmup = 10, steps = 40, chunk = 200, copy_mib = 0, precision = -1, warm = -1, it0 = 0;
   double convcr = 0.0;  // > 0: replaces the case's JCG tolerance
+  const char *state_path = nullptr, *params_path = nullptr;
   for (int a = 2; a + 1 < argc; a += 2) {
     if (!std::strcmp(argv[a], "--warmup")) warmup = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--steps")) steps = std::atoi(argv[a + 1]);
@@ -132,6 +138,14 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[a], "--copy-mib")) copy_mib = std::atoi(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--convcr")) convcr = std::atof(argv[a + 1]);
     else if (!std::strcmp(argv[a], "--precision")) precision = std::atoi(argv[a + 1]);
+    else if (!std::strcmp(argv[a], "--warm-start")) warm = std::atoi(argv[a + 1]);
+    else if (!std::strcmp(argv[a], "--state")) state_path = argv[a + 1];
+    else if (!std::strcmp(argv[a], "--it0")) it0 = std::atoi(argv[a + 1]);
+    else if (!std::strcmp(argv[a], "--params")) params_path = argv[a + 1];
+  }
+  if (it0 < 0 || (it0 > 0) != (state_path != nullptr)) {
+    std::fprintf(stderr, "--state and --it0 go together\n");
+    return 2;
   }
   std::map<std::string, Record> c;
   if (!read_case(argv[1], c)) {
@@ -163,12 +177,13 @@ int main(int argc, char** argv) {
   p.tau0var = DA("tau0var"); p.evm = DA("evm"); p.fric = DA("fric"); p.corif = DA("corif");
   if (convcr > 0.0) p.convcr = convcr;
   if (precision >= 0) p.precision = precision;
+  if (warm >= 0) p.warm_start = warm;
 
   const std::vector<double>& table = need(c, "eta_bc_table", 0)->d;
   const int neta = m.neta > 0 ? m.neta : 1;
-  if ((int64_t)table.size() < (int64_t)(warmup + steps) * neta) {
+  if ((int64_t)table.size() < (int64_t)(it0 + warmup + steps) * neta) {
     std::fprintf(stderr, "eta_bc_table holds %zu values, %d steps need %lld\n", table.size(),
-                 warmup + steps, (long long)(warmup + steps) * neta);
+                 it0 + warmup + steps, (long long)(it0 + warmup + steps) * neta);
     return 2;
   }
   const double* hol = c.count("holland") ? DA("holland") : nullptr;
@@ -177,18 +192,39 @@ int main(int argc, char** argv) {
   check(adcgpu_create(&m, &p, 0, &ctx), "adcgpu_create");
   check(adcgpu_cold_start(ctx, DA("eta2_0"), IA("nodecode_0"), IA("noff_0")), "cold_start");
   if (hol) check(adcgpu_holland(ctx, 0.0, hol), "holland");
-  std::fprintf(stderr, "[run_case] %d nodes, %d elements: warm-up %d steps\n", m.np, m.ne,
-               warmup);
-  if (warmup > 0) check(adcgpu_run(ctx, 1, warmup, table.data(), hol, 0), "run");
+  if (state_path) {
+    // a snapshot (adcgpu_state_save) of the same mesh taken after step it0
+    std::vector<char> buf((size_t)adcgpu_state_bytes(ctx));
+    FILE* f = std::fopen(state_path, "rb");
+    if (!f || std::fread(buf.data(), 1, buf.size(), f) != buf.size()) {
+      std::fprintf(stderr, "cannot read %zu snapshot bytes from %s\n", buf.size(), state_path);
+      return 2;
+    }
+    std::fclose(f);
+    check(adcgpu_state_load(ctx, buf.data(), (int64_t)buf.size()), "state_load");
+  }
+  if (params_path) {
+    std::vector<float> prm(ADCGPU_ML_NPARAM);
+    FILE* f = std::fopen(params_path, "rb");
+    if (!f || std::fread(prm.data(), 4, prm.size(), f) != prm.size()) {
+      std::fprintf(stderr, "cannot read %d parameters from %s\n", ADCGPU_ML_NPARAM, params_path);
+      return 2;
+    }
+    std::fclose(f);
+    check(adcgpu_ml_params(ctx, prm.data(), nullptr), "ml_params");
+  }
+  std::fprintf(stderr, "[run_case] %d nodes, %d elements: warm-up %d steps from step %d\n", m.np,
+               m.ne, warmup, it0 + 1);
+  if (warmup > 0)
+    check(adcgpu_run(ctx, it0 + 1, warmup, table.data() + (size_t)it0 * neta, hol, 0), "run");
   check(adcgpu_synchronize(ctx), "synchronize");
   std::fprintf(stderr, "[run_case] timed %d steps\n", steps);
   int64_t iters = 0;
   const auto t0 = std::chrono::steady_clock::now();
   for (int done = 0; done < steps;) {
     const int k = std::min(chunk, steps - done);
-    check(adcgpu_run(ctx, warmup + done + 1, k, table.data() + (size_t)(warmup + done) * neta,
-                     hol, 0),
-          "run");
+    const int s0 = it0 + warmup + done;
+    check(adcgpu_run(ctx, s0 + 1, k, table.data() + (size_t)s0 * neta, hol, 0), "run");
     iters += adcgpu_run_iterations(ctx);
     done += k;
     std::fprintf(stderr, "[run_case]   %d/%d steps\n", done, steps);
@@ -198,6 +234,7 @@ int main(int argc, char** argv) {
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   std::printf("steps %d ms_per_step %.4f jcg_iterations_per_step %.3f\n", steps, ms / steps,
               steps ? (double)iters / steps : 0.0);
+  std::fflush(stdout);
   adcgpu_destroy(ctx);
   return copy_mib > 0 ? copy_probe(copy_mib) : 0;
 }

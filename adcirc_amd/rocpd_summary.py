@@ -31,6 +31,11 @@ def _name(full: str) -> str:
     return n[5:] if n.startswith("void ") else n
 
 
+def _base(name: str) -> str:
+    """The kernel's name without its template arguments."""
+    return name.split("<")[0]
+
+
 def _groups(path: str):
     rows = sqlite3.connect(path).execute(
         "select name, start, end from kernels order by start").fetchall()
@@ -39,14 +44,14 @@ def _groups(path: str):
     groups, cur = [], None
     for name, t0, t1 in rows:
         name = _name(name)
-        if name in ("k_friction", "k_friction_holland"):
+        if _base(name) in ("k_friction", "k_friction_holland"):
             cur = []
             groups.append(cur)
         if cur is not None:
             cur.append((name, t1 - t0))
     # a last step cut off by the window's end (no k_jcg, or not all of the
     # step's kernels) is dropped: the step's last kernel is k_mom_node
-    while groups and groups[-1][-1][0] != "k_mom_node":
+    while groups and _base(groups[-1][-1][0]) != "k_mom_node":
         groups.pop()
     return groups
 
@@ -59,7 +64,7 @@ def counters(path: str, kernel: str, skip: int) -> int:
     seen = defaultdict(set)
     for name, did, cname, val in rows:
         k = _name(name)
-        if k == kernel:
+        if _base(k) == kernel:
             seen[cname].add(did)
             if len(seen[cname]) <= skip:
                 continue
@@ -112,7 +117,7 @@ def main(argv=None) -> int:
     print()
     print(f"Kernel time {tot / 1000:.2f} ms over {len(used)} steps = "
           f"{tot / 1000 / len(used):.4f} ms per step.")
-    j = agg.get("k_jcg")
+    j = [d for k, v in agg.items() if _base(k) == "k_jcg" for d in v]
     if j:
         print(f"k_jcg: {len(j)} launches, average {sum(j) / len(j):.1f} us.")
     return 0

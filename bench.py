@@ -26,7 +26,8 @@ Procedure:
        learned        FP64, JCG started from the better of the reference's
                       start and the network's (smaller initial residual);
        mixed          FP32 element passes and JCG matrix, FP64 solve vectors;
-       learned_mixed  both: the combination the speed target names.
+       learned_mixed  both: the combination the speed target names;
+       extrapolated_mixed  the extrapolated start in mixed precision.
      Every solve runs to the same stopping test.  Each leg reports its time,
      JCG iterations (and those saved against plain), the last solve's
      stopping-test value against its bound, and its distance to the plain
@@ -194,7 +195,8 @@ def main() -> int:
                     help="untimed fast-forward from cold start to this step")
     ap.add_argument("--nodes", type=int, default=1_000_000)
     ap.add_argument("--train-epochs", type=int, default=8)
-    ap.add_argument("--legs", default="plain,extrapolated,learned,mixed,learned_mixed")
+    ap.add_argument("--legs",
+                    default="plain,extrapolated,learned,mixed,learned_mixed,extrapolated_mixed")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -205,6 +207,12 @@ def main() -> int:
                     help="write the case for the native driver adcirc_amd/adcgpu_run_case "
                          "(warmup + steps steps of forcing from cold start) and exit "
                          "without using the GPU")
+    ap.add_argument("--dump-legs", metavar="DIR",
+                    help="after the fast-forward and the training, write DIR/case (forcing "
+                         "for start-step + warmup + steps steps), DIR/state.bin (the snapshot "
+                         "at the start step) and DIR/params.f32 (the trained network) for "
+                         "adcgpu_run_case --state DIR/state.bin --it0 START --params ..., "
+                         "and exit without timing the legs")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -267,11 +275,19 @@ def main() -> int:
     params = f.ml_params()
     snap = f.save_state()
     f.close()
+    if a.dump_legs:
+        os.makedirs(a.dump_legs, exist_ok=True)
+        write_case(os.path.join(a.dump_legs, "case"), tables, run, eta2, nodecode, noff,
+                   elevation_table(run, case.emo, efa, 1, S + W + K), run.holland)
+        snap.tofile(os.path.join(a.dump_legs, "state.bin"))
+        np.asarray(params, np.float32).tofile(os.path.join(a.dump_legs, "params.f32"))
+        log(f"legs written to {a.dump_legs} (snapshot after step {S})")
+        return 0
 
     # ---- 3. legs
     spec = {"plain": (0, _lib.FP64), "extrapolated": (1, _lib.FP64),
             "learned": (2, _lib.FP64), "mixed": (0, _lib.MIXED),
-            "learned_mixed": (2, _lib.MIXED)}
+            "learned_mixed": (2, _lib.MIXED), "extrapolated_mixed": (1, _lib.MIXED)}
     res, final = {}, {}
     for name in legs:
         warm, prec = spec[name]

@@ -191,7 +191,7 @@ int adcgpu_run(adcgpu_ctx *ctx, int it0, int nsteps, const double *eta_bc_table,
  * guess (warm_start = 2). */
 int adcgpu_run_stats(adcgpu_ctx *ctx, int64_t *out5);
 /* The last JCG solve's end: out[0] the stopping-test value STPTST it ended
- * with and out[1] the bound ZETA it was held to (pstop, src/itpackv.F:721-806:
+ * with and out[1] the bound ZETA it was held to (pstop_nrms, src/itpackv.F:809-890:
  * converged when out[0] < out[1]); out[2], out[3] the squared norms of the
  * initial pseudo-residuals of the reference's start and of the learned start
  * (warm_start = 2; the smaller one was used), 0 otherwise.  Synchronises the
