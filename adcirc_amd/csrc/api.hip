@@ -95,6 +95,7 @@ struct adcgpu_ctx {
   int clock_khz = 0;            // device wall-clock rate: the JCG wait bound in ticks
   int jcg_cap = 0;              // resident k_jcg blocks on this device
   int jcg_halo = -1;            // row tiles each side a tile's rows reference (-1: any)
+  int jcg_hrows = -1;           // rows each side staged in LDS with a tile's (-1: none)
   int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // the GWCE matrix's unchanging terms per incidence slot, and each element's
@@ -218,11 +219,12 @@ int grow_events(adcgpu_ctx* c, size_t n) {
 
 int jcg_barrier_words(const adcgpu_ctx* c) { return jcg_barrier_words_for(c->jcg_grid); }
 
-int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
-               double* u, int n, int itmax, double zeta, double* eta2, const double* eta1,
-               const int* nodecode, bool prescaled, bool defer_unscal,
-               const double* u2 = nullptr) {
+JcgArgs make_jcg_args(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh,
+                      double* rhs, double* u, int n, int itmax, double zeta, double* eta2,
+                      const double* eta1, const int* nodecode, bool prescaled,
+                      bool defer_unscal, const double* u2) {
   JcgArgs a;
+  std::memset(&a, 0, sizeof(a));  // padding too: batches compare whole structs
   a.n = n;
   jcg_tolerances(a, itmax, zeta);
   a.coef = coef;
@@ -239,6 +241,7 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   a.defer_unscal = defer_unscal;
   a.barrier = c->jbar;
   a.halo = c->jcg_halo;
+  a.hrows = c->jcg_hrows;
   a.barrier_ticks = (long long)((double)c->clock_khz * 1000.0 * g_wait_seconds);
   a.flags = c->flags;
   a.eta2 = eta2;
@@ -247,6 +250,18 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
   // the learned second start, formed by the pass before (learned mode)
   a.u2 = u2;
   a.d2 = c->jd2;
+  // the two-start kernel runs unstaged: staging spills its registers further
+  // (measured 559 against 516 us per launch mid-storm at 1M nodes)
+  if (u2) a.hrows = -1;
+  return a;
+}
+
+int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, double* rhs,
+               double* u, int n, int itmax, double zeta, double* eta2, const double* eta1,
+               const int* nodecode, bool prescaled, bool defer_unscal,
+               const double* u2 = nullptr) {
+  JcgArgs a = make_jcg_args(c, coef, col, nneigh, rhs, u, n, itmax, zeta, eta2, eta1, nodecode,
+                            prescaled, defer_unscal, u2);
   // a prescaling pass (k_rhs_node) has already cleared the barrier words
   if (!prescaled) {
     const int bar_words = jcg_barrier_words(c);
@@ -259,8 +274,11 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
       if (int rc = grow_events(c, c->jcg_events.size() + 256)) return rc;
     HIPCHK(hipEventRecord(c->jcg_events[c->ev_used].first, c->stream));
   }
-  const void* fn = u2 ? (const void*)k_jcg<true> : (const void*)k_jcg<false>;
-  HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->jcg_grid), dim3(JCG_BLOCK), args, 0, c->stream));
+  const void* fn =
+      a.hrows >= 0 ? (const void*)k_jcg<false, true>
+                   : (u2 ? (const void*)k_jcg<true, false> : (const void*)k_jcg<false, false>);
+  HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->jcg_grid), dim3(JCG_BLOCK), args,
+                                    (unsigned)jcg_stage_bytes(a.hrows), c->stream));
   if (c->profile) HIPCHK(hipEventRecord(c->jcg_events[c->ev_used++].second, c->stream));
   return 0;
 }
@@ -572,6 +590,9 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   c->np = np; c->ne = ne; c->mnei = mnei; c->neta = neta;
   c->nch = (np + RED_CHUNK - 1) / RED_CHUNK;
   c->jcg_halo = halo_blocks <= JCG_HALO_MAX ? halo_blocks : -1;
+  // staging needs the point-to-point halo wait (every staged row belongs to a
+  // tile within jcg_halo)
+  c->jcg_hrows = c->jcg_halo >= 0 ? jcg_stage_rows(bw) : -1;
   c->prm = *P;
   c->ifnlfa = P->nolifa == 0 ? 0.0 : 1.0;
   c->ifnlct = P->nolica == 0 ? 0.0 : 1.0;
@@ -939,7 +960,18 @@ int adcgpu_set_forcing(adcgpu_ctx* c, const double* wsx2, const double* wsy2, co
 // One GWCE assembly and solve.  With `deferred` the solve leaves the
 // off-diagonals of COEF scaled and the next assembly (or adcgpu_run's end)
 // restores them; the resident run uses it, since most steps rebuild COEF.
-static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
+// What the solve and the pass after it need from the assembly.
+struct GwceStep {
+  bool solve = false;    // a JCG solve follows (ILump = 0)
+  bool learned = false;  // with the learned second start
+  bool collect = false;  // training samples were recorded at slot0
+  int slot0 = 0;
+};
+
+// The GWCE assembly up to the solve: matrix, right-hand side, time-level
+// rotation, the learned start's inputs, and the node pass that scales the
+// rows for JCG (or, with ILump = 1, solves).
+static int gwce_assemble(adcgpu_ctx* c, const double* d_eta_bc, GwceStep* g) {
   const int np = c->np, ne = c->ne;
   const bool lumped = c->prm.ilump != 0;
   const size_t rows_lds = (size_t)c->mnei * RED_THREADS * sizeof(double);
@@ -1034,20 +1066,40 @@ static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
       hipLaunchKernelGGL((k_rhs_node<false, double>), ng, nbk, 0, c->stream, c->mesh(),
                          c->state(), (const double*)c->contrib, d_eta_bc, c->scal, pre);
   }
-  if (lumped) return check_launch("gwce assembly and lumped solve");
-  int rc = check_launch("gwce assembly");
-  if (rc) return rc;
-  rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, np, c->prm.itmax,
-                  c->prm.convcr, c->eta2, c->eta1, c->nodecode, true, deferred,
-                  learned ? c->ml_u : nullptr);
-  if (!rc && deferred) c->coef_scaled = true;
-  if (!rc && collect) {
-    // targets from this step's solution
-    hipLaunchKernelGGL(k_ml_target, dim3(nblk(c->ml_collect, 256)), dim3(256), 0, c->stream,
-                       c->smp, slot0, c->ml_collect, (const double*)c->etas);
-    c->smp_total += c->ml_collect;
-    rc = check_launch("training samples");
-  }
+  g->solve = !lumped;
+  g->learned = learned;
+  g->collect = collect;
+  g->slot0 = slot0;
+  return check_launch(lumped ? "gwce assembly and lumped solve" : "gwce assembly");
+}
+
+// The arguments of the step's JCG launch (prescaled rows, ETA2 written at the end).
+static JcgArgs gwce_jcg_args(adcgpu_ctx* c, const GwceStep& g, bool deferred) {
+  return make_jcg_args(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, c->np, c->prm.itmax,
+                       c->prm.convcr, c->eta2, c->eta1, c->nodecode, true, deferred,
+                       g.learned ? c->ml_u : nullptr);
+}
+
+// After the solve: the off-diagonals left scaled (deferred) and the training
+// targets from this step's solution.
+static int gwce_finish(adcgpu_ctx* c, const GwceStep& g, bool deferred) {
+  if (!g.solve) return 0;
+  if (deferred) c->coef_scaled = true;
+  if (!g.collect) return 0;
+  hipLaunchKernelGGL(k_ml_target, dim3(nblk(c->ml_collect, 256)), dim3(256), 0, c->stream,
+                     c->smp, g.slot0, c->ml_collect, (const double*)c->etas);
+  c->smp_total += c->ml_collect;
+  return check_launch("training samples");
+}
+
+static int gwce_device(adcgpu_ctx* c, const double* d_eta_bc, bool deferred) {
+  GwceStep g;
+  int rc = gwce_assemble(c, d_eta_bc, &g);
+  if (!rc && g.solve)
+    rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, c->np, c->prm.itmax,
+                    c->prm.convcr, c->eta2, c->eta1, c->nodecode, true, deferred,
+                    g.learned ? c->ml_u : nullptr);
+  if (!rc) rc = gwce_finish(c, g, deferred);
   return rc;
 }
 
@@ -1184,13 +1236,11 @@ int adcgpu_holland(adcgpu_ctx* c, double t, const double* prm13) {
   return check_launch("holland");
 }
 
-int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
-               const double* holland, int use_graph) {
-  if (!c || nsteps < 0) return fail(ADCGPU_E_ARG, "invalid argument");
-  if (use_graph != 0)
-    return fail(ADCGPU_E_UNSUPPORTED, "graph capture of the step is not implemented");
+// A resident run's set-up: the elevation-boundary table on the device and
+// the run's counters cleared.
+static int run_prepare(adcgpu_ctx* c, int nsteps, const double* eta_bc_table,
+                       const double* holland) {
   if (c->prm.nws != 0 && !holland) return fail(ADCGPU_E_ARG, "met forcing required");
-  HIPCHK(hipSetDevice(c->device));
   if (c->neta > 0) {
     if (!eta_bc_table) return fail(ADCGPU_E_ARG, "eta_bc_table required");
     if (c->eta_bc_table_steps < nsteps) {
@@ -1206,44 +1256,43 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_ITSUM, 0, 2);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_NITMAX, 0, 1);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->flags + F_MLUSED, 0, 1);
-  for (int s = 0; s < nsteps; ++s) {
-    const int it = it0 + s;
-    int rc = adcgpu_begin_step(c, it);
-    if (!rc) {
-      if (holland) {
-        // friction and this step's forcing in one pass
-        HollandPrm hp;
-        for (int k = 0; k < 13; ++k) hp.v[k] = holland[k];
-        hp.t = it * c->prm.dtdp;
-        hipLaunchKernelGGL(k_friction_holland, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream,
-                           c->np, fric_prm(c), c->tk, c->x, c->y, hp, c->wsx2, c->wsy2, c->pr2);
-        rc = check_launch("friction and forcing");
-      } else {
-        rc = adcgpu_friction(c);
-      }
-    }
-    if (!rc)
-      rc = gwce_device(c, c->neta > 0 ? c->eta_bc_table + (size_t)s * c->neta : c->eta_bc, true);
-    // with wet/dry the ALPHAL pass rides on the momentum element pass
-    const bool wd = c->prm.nolifa == 2;
-    if (!rc) rc = wetdry_device(c, !wd);
-    if (!rc) rc = momentum_device(c, wd);
-    if (rc) return rc;
-    if ((s & 63) == 63) {
-      int f[F_COUNT];
-      HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-      if (f[F_ERROR]) {
-        c->run_iters = f[F_ITSUM];
-        c->run_rebuilds = f[F_NREBUILD];
-        c->run_itmax = f[F_NITMAX];
-        c->run_errstep = f[F_ERRSTEP];
-        c->run_mlused = f[F_MLUSED];
-        return flag_errors(f);
-      }
+  return check_launch("run set-up");
+}
+
+// A resident step up to the GWCE solve: begin, friction (with this step's
+// Holland forcing in the same pass), the assembly.
+static int run_step_assemble(adcgpu_ctx* c, int it, int s, const double* holland, GwceStep* g) {
+  int rc = adcgpu_begin_step(c, it);
+  if (!rc) {
+    if (holland) {
+      // friction and this step's forcing in one pass
+      HollandPrm hp;
+      for (int k = 0; k < 13; ++k) hp.v[k] = holland[k];
+      hp.t = it * c->prm.dtdp;
+      hipLaunchKernelGGL(k_friction_holland, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream,
+                         c->np, fric_prm(c), c->tk, c->x, c->y, hp, c->wsx2, c->wsy2, c->pr2);
+      rc = check_launch("friction and forcing");
+    } else {
+      rc = adcgpu_friction(c);
     }
   }
-  if (int rc = restore_coef(c)) return rc;
+  if (!rc)
+    rc = gwce_assemble(c, c->neta > 0 ? c->eta_bc_table + (size_t)s * c->neta : c->eta_bc, g);
+  return rc;
+}
+
+// A resident step after the solve: wet/dry and momentum (with wet/dry the
+// ALPHAL pass rides on the momentum element pass).
+static int run_step_finish(adcgpu_ctx* c, const GwceStep& g) {
+  int rc = gwce_finish(c, g, true);
+  const bool wd = c->prm.nolifa == 2;
+  if (!rc) rc = wetdry_device(c, !wd);
+  if (!rc) rc = momentum_device(c, wd);
+  return rc;
+}
+
+// The run's counters from the device flags; the sticky error, if any.
+static int run_read_flags(adcgpu_ctx* c, bool* error) {
   int f[F_COUNT];
   HIPCHK(hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1252,7 +1301,214 @@ int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
   c->run_itmax = f[F_NITMAX];
   c->run_errstep = f[F_ERRSTEP];
   c->run_mlused = f[F_MLUSED];
+  *error = f[F_ERROR] != 0;
   return flag_errors(f);
+}
+
+int adcgpu_run(adcgpu_ctx* c, int it0, int nsteps, const double* eta_bc_table,
+               const double* holland, int use_graph) {
+  if (!c || nsteps < 0) return fail(ADCGPU_E_ARG, "invalid argument");
+  if (use_graph != 0)
+    return fail(ADCGPU_E_UNSUPPORTED, "graph capture of the step is not implemented");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = run_prepare(c, nsteps, eta_bc_table, holland)) return rc;
+  for (int s = 0; s < nsteps; ++s) {
+    GwceStep g;
+    int rc = run_step_assemble(c, it0 + s, s, holland, &g);
+    if (!rc && g.solve)
+      rc = launch_jcg(c, c->coef, c->col, c->nneigh, c->gwce_lv, c->etas, c->np, c->prm.itmax,
+                      c->prm.convcr, c->eta2, c->eta1, c->nodecode, true, true,
+                      g.learned ? c->ml_u : nullptr);
+    if (!rc) rc = run_step_finish(c, g);
+    if (rc) return rc;
+    if ((s & 63) == 63) {
+      bool error = false;
+      const int frc = run_read_flags(c, &error);
+      if (error) return frc;
+    }
+  }
+  if (int rc = restore_coef(c)) return rc;
+  bool error = false;
+  return run_read_flags(c, &error);
+}
+
+}  // extern "C"
+
+// ---- batches of contexts (ensembles, uneven meshes)
+struct adcgpu_batch {
+  std::vector<adcgpu_ctx*> m;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // the members' JCG solves, grouped into cooperative launches whose blocks
+  // are all resident: group g is members[gm[g] .. gm[g+1]-1] of the solving
+  // members `sm`, launched with first-block offsets dfirst[gf[g] ..]
+  std::vector<int> sm, gm, gf;
+  std::vector<bool> gdual;
+  int* dfirst = nullptr;
+  JcgArgs* dargs = nullptr;  // [2][n]: the launch arguments by step parity
+  JcgArgs* hargs = nullptr;  // pinned host copy of what dargs holds
+  bool hvalid[2] = {false, false};
+};
+
+namespace {
+void batch_free(adcgpu_batch* b) {
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->dfirst) (void)hipFree(b->dfirst);
+  if (b->dargs) (void)hipFree(b->dargs);
+  if (b->hargs) (void)hipHostFree(b->hargs);
+  delete b;
+}
+}  // namespace
+
+extern "C" {
+
+int adcgpu_batch_create(adcgpu_ctx* const* members, int n, adcgpu_batch** out) {
+  if (!members || n <= 0 || !out) return fail(ADCGPU_E_ARG, "invalid argument");
+  *out = nullptr;
+  for (int k = 0; k < n; ++k) {
+    if (!members[k]) return fail(ADCGPU_E_ARG, "null member");
+    if (members[k]->device != members[0]->device)
+      return fail(ADCGPU_E_ARG, "batch members must share one device");
+    for (int l = 0; l < k; ++l)
+      if (members[l] == members[k]) return fail(ADCGPU_E_ARG, "a context appears twice");
+  }
+  adcgpu_batch* b = new adcgpu_batch();
+  b->m.assign(members, members + n);
+  b->device = members[0]->device;
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) {
+    batch_free(b);
+    return fail(ADCGPU_E_HIP, "stream creation");
+  }
+  // greedy grouping in member order: a group holds members with the same
+  // start (one or two) whose JCG grids fit on the device together
+  const int cap = members[0]->jcg_cap;
+  for (int k = 0; k < n; ++k)
+    if (members[k]->prm.ilump == 0) b->sm.push_back(k);
+  int sum = 0;
+  bool dual = false;
+  std::vector<int> first;
+  for (size_t q = 0; q < b->sm.size(); ++q) {
+    adcgpu_ctx* c = members[b->sm[q]];
+    const bool d = c->prm.warm_start == 2;
+    if (q == 0 || sum + c->jcg_grid > cap || d != dual) {
+      b->gm.push_back((int)q);
+      b->gf.push_back((int)first.size());
+      b->gdual.push_back(d);
+      first.push_back(0);
+      sum = 0;
+      dual = d;
+    }
+    sum += c->jcg_grid;
+    first.push_back(sum);
+  }
+  b->gm.push_back((int)b->sm.size());
+  int rc = 0;
+  if (!first.empty()) rc = upload(&b->dfirst, first);
+  if (!rc && !b->sm.empty()) rc = dmalloc(&b->dargs, 2 * b->sm.size());
+  if (!rc && !b->sm.empty() &&
+      hipHostMalloc((void**)&b->hargs, 2 * b->sm.size() * sizeof(JcgArgs), 0) != hipSuccess)
+    rc = fail(ADCGPU_E_HIP, "pinned host buffer");
+  if (rc) {
+    batch_free(b);
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+void adcgpu_batch_destroy(adcgpu_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+}
+
+int adcgpu_batch_run(adcgpu_batch* b, int it0, int nsteps, const double* const* eta_bc_tables,
+                     const double* const* holland) {
+  if (!b || nsteps < 0) return fail(ADCGPU_E_ARG, "invalid argument");
+  HIPCHK(hipSetDevice(b->device));
+  const int n = (int)b->m.size();
+  auto table = [&](int k) { return eta_bc_tables ? eta_bc_tables[k] : nullptr; };
+  auto hol = [&](int k) { return holland ? holland[k] : nullptr; };
+  // every member's work goes to the batch's stream for the run, so that the
+  // shared JCG launches are ordered after and before each member's passes
+  std::vector<hipStream_t> own(n);
+  for (int k = 0; k < n; ++k) {
+    own[k] = b->m[k]->stream;
+    HIPCHK(hipStreamSynchronize(own[k]));
+    b->m[k]->stream = b->stream;
+  }
+  auto restore = [&]() {
+    (void)hipStreamSynchronize(b->stream);
+    for (int k = 0; k < n; ++k) b->m[k]->stream = own[k];
+  };
+  int rc = 0;
+  for (int k = 0; k < n && !rc; ++k) rc = run_prepare(b->m[k], nsteps, table(k), hol(k));
+  std::vector<GwceStep> g(n);
+  const size_t ns = b->sm.size();
+  for (int s = 0; s < nsteps && !rc; ++s) {
+    for (int k = 0; k < n && !rc; ++k) rc = run_step_assemble(b->m[k], it0 + s, s, hol(k), &g[k]);
+    // the solves: this step's arguments (they alternate with the time-level
+    // rotation, so each parity's are uploaded once), then one launch per group
+    const int par = s & 1;
+    if (!rc && ns) {
+      JcgArgs* h = b->hargs + par * ns;
+      std::vector<JcgArgs> cur(ns);
+      for (size_t q = 0; q < ns; ++q) {
+        adcgpu_ctx* c = b->m[b->sm[q]];
+        cur[q] = gwce_jcg_args(c, g[b->sm[q]], true);
+      }
+      for (size_t gi = 0; gi + 1 < b->gm.size(); ++gi) {
+        // a group runs staged only if all its members can
+        bool st = !b->gdual[gi];
+        for (int q = b->gm[gi]; q < b->gm[gi + 1]; ++q) st = st && cur[q].hrows >= 0;
+        if (!st)
+          for (int q = b->gm[gi]; q < b->gm[gi + 1]; ++q) cur[q].hrows = -1;
+      }
+      if (!b->hvalid[par] || std::memcmp(h, cur.data(), ns * sizeof(JcgArgs)) != 0) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        std::memcpy(h, cur.data(), ns * sizeof(JcgArgs));
+        HIPCHK(hipMemcpyAsync(b->dargs + par * ns, h, ns * sizeof(JcgArgs),
+                              hipMemcpyHostToDevice, b->stream));
+        b->hvalid[par] = true;
+      }
+      for (size_t gi = 0; gi + 1 < b->gm.size() && !rc; ++gi) {
+        const int q0 = b->gm[gi], q1 = b->gm[gi + 1];
+        int grid = 0, hmax = -1;
+        for (int q = q0; q < q1; ++q) {
+          grid += b->m[b->sm[q]]->jcg_grid;
+          hmax = std::max(hmax, cur[q].hrows);
+        }
+        const bool st = cur[q0].hrows >= 0;
+        const void* fn = b->gdual[gi] ? (const void*)k_jcg_batch<true, false>
+                                      : (st ? (const void*)k_jcg_batch<false, true>
+                                            : (const void*)k_jcg_batch<false, false>);
+        const JcgArgs* pa = b->dargs + par * ns + q0;
+        const int* pf = b->dfirst + b->gf[gi];
+        int nsys = q1 - q0;
+        void* args[] = {(void*)&pa, (void*)&pf, (void*)&nsys};
+        const hipError_t e = hipLaunchCooperativeKernel(
+            fn, dim3(grid), dim3(JCG_BLOCK), args, (unsigned)jcg_stage_bytes(st ? hmax : -1),
+            b->stream);
+        if (e != hipSuccess) rc = fail(ADCGPU_E_HIP, std::string("batched JCG: ") +
+                                                         hipGetErrorString(e));
+      }
+    }
+    for (int k = 0; k < n && !rc; ++k) rc = run_step_finish(b->m[k], g[k]);
+    if (!rc && (s & 63) == 63)
+      for (int k = 0; k < n && !rc; ++k) {
+        bool error = false;
+        const int frc = run_read_flags(b->m[k], &error);
+        if (error) rc = frc;
+      }
+  }
+  for (int k = 0; k < n && !rc; ++k) rc = restore_coef(b->m[k]);
+  for (int k = 0; k < n && !rc; ++k) {
+    bool error = false;
+    rc = run_read_flags(b->m[k], &error);
+  }
+  restore();
+  return rc;
 }
 
 int adcgpu_run_stats(adcgpu_ctx* c, int64_t* out5) {

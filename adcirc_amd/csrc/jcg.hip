@@ -39,17 +39,18 @@ namespace adcgpu {
 // The wait is bounded in wall-clock time and a block that sees the error
 // flag leaves at once, so a launch that went wrong drains quickly.  Returns
 // false on time-out or error (a logic error; the caller leaves the kernel).
+// `bid` and `nb` are the block's index and the number of blocks of the
+// system it works on (the whole grid, or one system of a batched launch).
 __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& gen, int* err,
-                                          long long ticks) {
+                                          long long ticks, unsigned bid, unsigned nb) {
   __shared__ int failed;
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     failed = 0;
     ++gen;
-    const unsigned nb = gridDim.x;
     const unsigned ng = (nb + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP;
-    const unsigned g = blockIdx.x / JCG_BAR_GROUP;
+    const unsigned g = bid / JCG_BAR_GROUP;
     const unsigned gcount = (g == ng - 1) ? nb - g * JCG_BAR_GROUP : JCG_BAR_GROUP;
     unsigned* genw = bar + JCG_BAR_STRIDE * (1 + ng);
     const unsigned o1 = __hip_atomic_fetch_add(bar + JCG_BAR_STRIDE * (1 + g), 1u,
@@ -310,28 +311,38 @@ __device__ void jcg_parcon(ItLocal& L, double dtnrm) {
 // one); the pseudo-residuals of both are formed in the scaling pass and the
 // iteration starts from the one with the smaller norm (the reference's on a
 // tie).  Everything after that choice is the same ITPACK iteration.
-template <bool DUAL>
-__global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
+// STAGED: the products read the pseudo-residual from LDS (a.hrows >= 0).
+// `bid` and `nblocks`: this block's index among the blocks that solve this
+// system, and their number (jcg_grid_for(a.n)).
+template <bool DUAL, bool STAGED>
+__device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks) {
   __shared__ ItLocal L;
   __shared__ double lds[12];
   __shared__ double tri1[JCG_MAX_ITMAX], tri2[JCG_MAX_ITMAX];
+  extern __shared__ double dsh[];  // staged pseudo-residual, rows slo .. shi-1
   const int np = a.n;
   const int t = threadIdx.x;
   const int nch = (np + RED_CHUNK - 1) / RED_CHUNK;
   const int nrb = (nch + JCG_CPB - 1) / JCG_CPB;  // blocks with rows
-  const bool eig_block = (int)blockIdx.x == nrb;  // the last block: no rows
+  const bool eig_block = bid == nrb;  // the last block: no rows
   // each group of RED_THREADS threads owns one chunk; rows past NP (including
   // a whole missing chunk in the last block) are skipped like padding
-  const int tile = eig_block ? nrb : xcd_tile(blockIdx.x, nrb);
+  const int tile = eig_block ? nrb : xcd_tile(bid, nrb);
   const int c = tile * JCG_CPB + t / RED_THREADS;
   // halo flags follow the grid barrier's counters
   unsigned* hflags =
-      a.barrier + ((gridDim.x + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE;
+      a.barrier + ((nblocks + JCG_BAR_GROUP - 1) / JCG_BAR_GROUP + 2) * JCG_BAR_STRIDE;
   const int hlo = a.halo >= 0 ? (tile - a.halo > 0 ? tile - a.halo : 0) : 0;
   const int hhi = a.halo >= 0 ? (tile + a.halo < nrb - 1 ? tile + a.halo : nrb - 1) : 0;
   const int base = c * RED_CHUNK + t % RED_THREADS;
   unsigned gen = 0;
   int* err = a.flags + F_ERROR;
+  // the staged rows: the tile's own rows t0 .. t1-1 and hrows each side
+  const bool staged = STAGED && !eig_block;
+  const int t0 = tile * JCG_CPB * RED_CHUNK;
+  const int t1 = t0 + JCG_CPB * RED_CHUNK < np ? t0 + JCG_CPB * RED_CHUNK : np;
+  const int slo = staged ? (t0 - a.hrows > 0 ? t0 - a.hrows : 0) : 0;
+  const int shi = staged ? (t1 + a.hrows < np ? t1 + a.hrows : np) : 0;
 
   if (t == 0) {
     L.in = 0;
@@ -393,7 +404,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
     }
   }
   if (!a.prescaled)
-    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
 
   // ---- scal, part 2: scale off-diagonals; initial pseudo-residual
   // d0 = rhs - N u - u (src/itpackv.F:2394-2400, 346-353)
@@ -500,7 +511,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
       }
     }
   }
-  if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+  if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
   // the start with the smaller initial pseudo-residual
   double* dv = a.d;
   if (DUAL) {
@@ -515,7 +526,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
     __shared__ int use2;
     if (t == 0) {
       use2 = r1 < r0 ? 1 : 0;
-      if (blockIdx.x == 0) {
+      if (bid == 0) {
         a.saved[10] = r0;
         a.saved[11] = r1;
         if (use2) a.flags[F_MLUSED] += 1;
@@ -532,6 +543,19 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         }
     }
   }
+
+  // the initial pseudo-residual into LDS: the block's rows from registers, the
+  // rows each side from memory (complete since the last grid barrier)
+  if (staged) {
+#pragma unroll
+    for (int r = 0; r < RED_PER_THREAD; ++r) {
+      const int i = base + RED_THREADS * r;
+      if (i < np) dsh[i - slo] = dc[r];
+    }
+    for (int k = slo + t; k < t0; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
+    for (int k = t1 + t; k < shi; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
+  }
+  __syncthreads();
 
   // ---- iterations
   for (int in = 0; in <= a.itmax; ++in) {
@@ -583,7 +607,9 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
           }
 #pragma unroll
           for (int r = h; r < h + 4; ++r)
-            if (j < NN(r)) nd[r] = nd[r] - sc[r - h] * at(dv, cc[r - h]);
+            if (j < NN(r))
+              nd[r] = nd[r] - sc[r - h] * (STAGED ? dsh[cc[r - h] - (unsigned)slo]
+                                                  : at(dv, cc[r - h]));
         }
       }
       double add = 0.0, adt = 0.0, auu = 0.0;
@@ -612,7 +638,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
             ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
       }
     }
-    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
 
     // the eigenvalue block's estimate for this iteration, loaded alongside
     // the partials
@@ -668,15 +694,25 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
         dc[r] = dn;
         // written through to memory: the next grid barrier's release then
         // has no pseudo-residual lines to write back from this XCD's L2
-        if (i < np) __hip_atomic_store(&at(dv, i), dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (i < np) {
+          __hip_atomic_store(&at(dv, i), dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (staged) dsh[i - slo] = dn;
+        }
       }
     }
     if (a.halo < 0) {
-      if (!grid_sync(a.barrier, gen, err, a.barrier_ticks)) return;
+      if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
     } else if (!eig_block) {
       // the partials and the eigenvalue pair are double-buffered by
       // iteration parity, so only the neighbours' d has to be waited for
       if (!halo_sync(hflags, tile, hlo, hhi, (unsigned)(in + 1), err, a.barrier_ticks)) return;
+    }
+    // the neighbouring tiles' new d into LDS (each barrier above ends with
+    // the block's own rows written and the neighbours' visible)
+    if (staged) {
+      for (int k = slo + t; k < t0; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
+      for (int k = t1 + t; k < shi; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
+      __syncthreads();
     }
   }
   // the result is the iterate of the last iteration index: u at the halt,
@@ -717,7 +753,7 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
     }
   }
 #undef NN
-  if (blockIdx.x == 0 && t == 0) {
+  if (bid == 0 && t == 0) {
     const int in_final = halted ? L.in : a.itmax;
     a.flags[F_NUMITR] = in_final;
     a.flags[F_IER] = halted ? 0 : 13;
@@ -732,7 +768,29 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
   }
 }
 
-template __global__ void k_jcg<false>(JcgArgs);
-template __global__ void k_jcg<true>(JcgArgs);
+template <bool DUAL, bool STAGED>
+__global__ __launch_bounds__(JCG_BLOCK) void k_jcg(JcgArgs a) {
+  jcg_solve<DUAL, STAGED>(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// Several systems in one cooperative launch (an ensemble's members): blocks
+// first[m] .. first[m+1]-1 solve system m with args[m].  A system's blocks
+// synchronise only among themselves, through its own barrier words, and
+// leave when it has converged; its arithmetic is that of its own launch.
+template <bool DUAL, bool STAGED>
+__global__ __launch_bounds__(JCG_BLOCK) void k_jcg_batch(const JcgArgs* __restrict__ args,
+                                                         const int* __restrict__ first, int nsys) {
+  const int b = blockIdx.x;
+  int m = 0;
+  while (m + 1 < nsys && first[m + 1] <= b) ++m;
+  jcg_solve<DUAL, STAGED>(args[m], b - first[m], first[m + 1] - first[m]);
+}
+
+template __global__ void k_jcg<false, false>(JcgArgs);
+template __global__ void k_jcg<true, false>(JcgArgs);
+template __global__ void k_jcg<false, true>(JcgArgs);
+template __global__ void k_jcg_batch<false, false>(const JcgArgs*, const int*, int);
+template __global__ void k_jcg_batch<true, false>(const JcgArgs*, const int*, int);
+template __global__ void k_jcg_batch<false, true>(const JcgArgs*, const int*, int);
 
 }  // namespace adcgpu

@@ -13,6 +13,17 @@ namespace adcgpu {
 // Records the message adcgpu_last_error returns (a fixed per-thread buffer).
 void set_last_error(const char* msg);
 
+// Rows each side of a block's rows whose pseudo-residual is staged in LDS for
+// a matrix of bandwidth bw (the widest |column - row|), or -1 when they do
+// not fit; and the dynamic LDS bytes of a launch.
+inline int jcg_stage_rows(int bw) {
+  const size_t bytes = 8 * ((size_t)JCG_CPB * RED_CHUNK + 2 * (size_t)bw);
+  return bytes <= (size_t)JCG_STAGE_MAX_BYTES ? bw : -1;
+}
+inline size_t jcg_stage_bytes(int hrows) {
+  return hrows < 0 ? 0 : 8 * ((size_t)JCG_CPB * RED_CHUNK + 2 * (size_t)hrows);
+}
+
 // Barrier words of a grid: the grid barrier's counters, then one halo flag
 // per block.
 inline int jcg_barrier_words_for(int grid) {
@@ -31,7 +42,11 @@ inline int jcg_barrier_words_for(int grid) {
 // the barrier words hold.
 inline hipError_t jcg_resident_cap_of(const void* fn, int* per_cu_out) {
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, JCG_BLOCK, 0);
+  // the largest staged pseudo-residual a launch may ask for
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     JCG_STAGE_MAX_BYTES);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, JCG_BLOCK, JCG_STAGE_MAX_BYTES);
   if (e != hipSuccess) return e;
   hipFuncAttributes fa;
   if ((e = hipFuncGetAttributes(&fa, fn)) != hipSuccess) return e;
@@ -42,9 +57,12 @@ inline hipError_t jcg_resident_cap_of(const void* fn, int* per_cu_out) {
   return hipSuccess;
 }
 
-// Both variants of the kernel (one start, two starts) must fit.
+// Every variant of the kernel that is launched must fit: one start or two,
+// and one start with the staged pseudo-residual; single and batched.
 inline hipError_t jcg_resident_cap(int device, int* cap) {
-  const void* fns[] = {(const void*)k_jcg<false>, (const void*)k_jcg<true>};
+  const void* fns[] = {(const void*)k_jcg<false, false>,       (const void*)k_jcg<true, false>,
+                       (const void*)k_jcg<false, true>,        (const void*)k_jcg_batch<false, false>,
+                       (const void*)k_jcg_batch<true, false>,  (const void*)k_jcg_batch<false, true>};
   int per_cu = 1 << 30;
   hipError_t e = hipSuccess;
   for (const void* fn : fns) {

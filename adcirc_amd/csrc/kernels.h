@@ -22,6 +22,9 @@ constexpr int JCG_BAR_WORDS = 10240;     // barrier words: (groups + 2) * JCG_BA
 constexpr int JCG_HALO_STRIDE = 16;      // words between per-tile halo flags (64 B)
 constexpr int JCG_HALO_MAX = 2;          // widest halo (tiles each side) waited on by flags
 constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
+// dynamic LDS for the staged pseudo-residual of a k_jcg block (JcgArgs.hrows),
+// at most: the block's rows and 5,632 rows each side
+constexpr int JCG_STAGE_MAX_BYTES = 120 * 1024;
 constexpr int NODE_STAGE = 2048;   // incidence slots a 256-node block stages in LDS
 constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
 // ordered wetting updates: candidates are closed in blocks of WD_BLOCK, each
@@ -206,6 +209,11 @@ struct JcgArgs {
   int prescaled;      // rows already scaled (w, u, rhs) by k_rhs_node
   int defer_unscal;   // leave the off-diagonals scaled (k_lhs_rows restores them)
   unsigned* barrier;  // [JCG_BAR_WORDS] grid barrier counters, then per-tile halo flags
+  int hrows;          // >= 0: the pseudo-residual of the tile's rows and of hrows
+                      // rows each side (at least the matrix bandwidth) is kept
+                      // in LDS for the products (dynamic shared memory of
+                      // 8 (JCG_CPB RED_CHUNK + 2 hrows) bytes); -1: gathered
+                      // from memory
   int halo;           // row tiles each side whose pseudo-residual a tile's rows read;
                       // -1: wait for all tiles (a grid barrier) after each update
   long long barrier_ticks;  // longest grid-barrier wait, in wall-clock ticks
@@ -278,9 +286,14 @@ struct JcgPre {
 template <bool LUMPED, typename CT>
 __global__ void k_rhs_node(DevMesh m, DevState s, const CT* contrib, const double* eta_bc,
                            const double* scal, JcgPre pre);
-// DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2
-template <bool DUAL>
+// DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2;
+// STAGED: the products read the pseudo-residual from LDS (a.hrows >= 0).
+// Instantiated: <false, false>, <true, false>, <false, true>.
+template <bool DUAL, bool STAGED>
 __global__ void k_jcg(JcgArgs a);
+// nsys systems in one launch: blocks first[m] .. first[m+1]-1 solve args[m]
+template <bool DUAL, bool STAGED>
+__global__ void k_jcg_batch(const JcgArgs* args, const int* first, int nsys);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);
 __global__ void k_wd_index(const int* bcnt, int nb, int* bofs, int* flags);

@@ -237,6 +237,7 @@ int adcgpu_jcg_noalloc(int n, int ndim, int maxnz, int32_t* jcoef, double* coef,
   a.defer_unscal = 0;
   a.barrier = dbar;
   a.halo = -1;  // the caller's matrix: a grid barrier after each update
+  a.hrows = -1;
   a.barrier_ticks = (long long)g_na_khz[dev] * 500;  // 0.5 s
   a.flags = dflags;
   a.eta2 = nullptr;
@@ -245,7 +246,7 @@ int adcgpu_jcg_noalloc(int n, int ndim, int maxnz, int32_t* jcoef, double* coef,
   a.u2 = nullptr;
   a.d2 = nullptr;
   void* args[] = {&a};
-  NACHK(hipLaunchCooperativeKernel((const void*)k_jcg<false>, dim3(grid), dim3(JCG_BLOCK), args,
+  NACHK(hipLaunchCooperativeKernel((const void*)k_jcg<false, false>, dim3(grid), dim3(JCG_BLOCK), args,
                                    0, st));
   int f[F_COUNT];
   double saved[8];

@@ -288,4 +288,29 @@ def coastal(target_nodes: int = 1_000_000, seed: int = 7, band: float = 10.0e3,
     return Case(mesh, run, emo, efa)
 
 
+def storm_ensemble(case: Case, n: int, seed: int = 11) -> List[Tuple[float, ...]]:
+    """n Holland vortices around the case's storm, drawn from a seeded
+    generator: central pressure U(920, 980) hPa, radius of maximum winds
+    U(20, 60) km, landfall point shifted along the coast by U(-50, 50) km and
+    heading turned by U(-20, 20) degrees at the case's forward speed."""
+    base = list(case.run.holland)
+    rng = np.random.default_rng(seed)
+    speed = math.hypot(base[6], base[7])
+    heading0 = math.atan2(base[7], base[6])
+    out = []
+    for _ in range(n):
+        h = list(base)
+        h[0] = float(rng.uniform(920.0, 980.0))
+        h[2] = float(rng.uniform(20.0e3, 60.0e3))
+        shift = float(rng.uniform(-50.0e3, 50.0e3))
+        turn = math.radians(float(rng.uniform(-20.0, 20.0)))
+        # the track still crosses the coast (x = COAST_X) at y0 + shift
+        vx, vy = speed * math.cos(heading0 + turn), speed * math.sin(heading0 + turn)
+        t_land = (COAST_X - base[4]) / vx
+        h[5] = base[5] + shift - vy * t_land
+        h[6], h[7] = vx, vy
+        out.append(tuple(h))
+    return out
+
+
 CASES = {"basin": basin, "beach": beach, "coastal": coastal}

@@ -405,3 +405,49 @@ def set_wait_limit(seconds: float) -> None:
     """Bound of every wait inside the persistent JCG kernel (default 0.5 s;
     0 makes every wait fail at once, which forces the error path)."""
     check(_lib.load().adcgpu_set_wait_limit(float(seconds)), "set_wait_limit")
+
+
+class Batch:
+    """Several models stepped together (adcgpu_batch): their JCG solves run
+    as one cooperative launch per step (more when their blocks do not all
+    fit at once); each member's results equal its own resident run's."""
+
+    def __init__(self, models: Sequence["Model"]):
+        self.models = list(models)
+        self.lib = self.models[0].lib
+        arr = (C.c_void_p * len(self.models))(*[m.h for m in self.models])
+        h = C.c_void_p()
+        check(self.lib.adcgpu_batch_create(arr, len(self.models), C.byref(h)), "batch_create")
+        self.h = h
+
+    def run(self, it0: int, nsteps: int, holland=None, tables=None):
+        """Advance every member by ``nsteps`` steps from ``it0``.  ``holland``
+        and ``tables`` are per member (None: no met forcing, and each
+        member's elevation-boundary table).  Returns each member's JCG
+        iterations."""
+        n = len(self.models)
+        tabs = [np.ascontiguousarray(m.eta_bc_table(it0, nsteps) if tables is None
+                                     else tables[k], np.float64)
+                for k, m in enumerate(self.models)]
+        tp = (dp_ * n)(*[_p(t, dp_) for t in tabs])
+        hp = None
+        hols = []
+        if holland is not None:
+            for k in range(n):
+                hk = holland[k]
+                hols.append(None if hk is None else
+                            np.ascontiguousarray(list(hk) + [0.0] * (13 - len(hk)), np.float64))
+            hp = (dp_ * n)(*[_p(h, dp_) if h is not None else None for h in hols])
+        check(self.lib.adcgpu_batch_run(self.h, it0, nsteps, tp, hp), "batch_run")
+        return [m.run_stats()[0] for m in self.models]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.adcgpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

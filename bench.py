@@ -32,7 +32,9 @@ Procedure:
      JCG iterations (and those saved against plain), the last solve's
      stopping-test value against its bound, and its distance to the plain
      leg's state at the end of the window.
-  4. C1 (basin, 1 simulated day, launch-latency bound) on its own.
+  4. C1 (basin, 1 simulated day, launch-latency bound) on its own, and C4:
+     storm ensembles on meshes of 50k, 250k and the 1M nodes (eight storms
+     each) stepped as one batch, against the same members one at a time.
   5. CPU baseline: the reference build (oracle/_ref/adchp_ref, the
      reference's own Fortran compiled from its sources) timed on one host
      core on the first steps of the same 1M-node run; the oracle's C
@@ -186,6 +188,68 @@ def basin_day(steps_warm: int = 100):
             "ms_per_step": wall * 1e3 / k, "jcg_iterations_per_step": iters / k}
 
 
+def batch_leg(case1m, tables1m, steps: int, warmup: int, per_mesh: int, sizes):
+    """C4: storm ensembles on coastal meshes of the given sizes (the 1M-node
+    mesh of the main workload last), ``per_mesh`` storms each, stepped from
+    cold start as one batch (adcgpu_batch: one JCG launch for all members
+    whose blocks fit together), then the same members one by one for as many
+    steps (adcgpu_run per member)."""
+    from adcirc_amd import meshgen, prep
+    from adcirc_amd.model import Batch, Model, cold_wetdry_state
+
+    meshes = []
+    for n in sizes:
+        if n == int(tables1m["np"]) or n >= 1_000_000:
+            meshes.append((case1m, tables1m))
+        else:
+            c = meshgen.coastal(n)
+            meshes.append((c, prep.build_tables(c.mesh, c.run.nolifa, c.run.h0, c.run.anginn)))
+    models, hols, tabs, desc = [], [], [], []
+    for c, t in meshes:
+        eta2, nc, nf = cold_wetdry_state(t, c.run)
+        efa = np.asarray(c.efa, np.float64) * (math.pi / 180.0)
+        tab = None
+        for h in meshgen.storm_ensemble(c, per_mesh):
+            m = Model(t, c.run, emo=c.emo, efa=efa)
+            m.cold_start(eta2, nc, nf)
+            m.holland(0.0, h)
+            if tab is None:
+                tab = m.eta_bc_table(1, warmup + 2 * steps)
+            models.append(m)
+            hols.append(h)
+            tabs.append(tab)
+        desc.append(f"{per_mesh} x {int(t['np'])} nodes")
+    log(f"batch of {len(models)} members ({', '.join(desc)}): {warmup} warm-up steps")
+    b = Batch(models)
+    b.run(1, warmup, holland=hols, tables=[t[:warmup] for t in tabs])
+    log(f"  batched: {steps} timed steps")
+    t0 = time.perf_counter()
+    its = b.run(warmup + 1, steps, holland=hols, tables=[t[warmup:warmup + steps] for t in tabs])
+    wall_b = time.perf_counter() - t0
+    b.close()
+    log(f"  one by one: {steps} timed steps per member")
+    t0 = time.perf_counter()
+    its1 = 0
+    s0 = warmup + steps
+    for m, h, t in zip(models, hols, tabs):
+        its1 += m.run_resident(s0 + 1, steps, holland=h, table=t[s0:s0 + steps])
+    wall_1 = time.perf_counter() - t0
+    for m in models:
+        m.close()
+    ms = len(models) * steps
+    return {"workload": f"{', '.join(desc)} coastal meshes, storms drawn by "
+                        f"meshgen.storm_ensemble (Pc 920-980 hPa, Rmax 20-60 km, landfall "
+                        f"+-50 km, heading +-20 deg), from cold start: {warmup} warm-up steps, "
+                        f"steps {warmup + 1}-{warmup + steps} batched, then "
+                        f"{s0 + 1}-{s0 + steps} one member at a time",
+            "members": len(models), "member_steps_per_s": ms / wall_b,
+            "ms_per_step": wall_b * 1e3 / steps,
+            "one_by_one_member_steps_per_s": ms / wall_1,
+            "speedup_vs_one_by_one": wall_1 / wall_b,
+            "jcg_iterations_per_member_step": sum(its) / ms,
+            "one_by_one_jcg_iterations_per_member_step": its1 / ms}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +265,10 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-basin", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the C4 batch leg")
+    ap.add_argument("--batch-members", type=int, default=8, help="storms per mesh in C4")
+    ap.add_argument("--batch-sizes", default="50000,250000,1000000",
+                    help="node counts of the C4 meshes (1000000: the main workload's mesh)")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no HIP events around the JCG launches (no roofline entry)")
     ap.add_argument("--dump", metavar="PATH",
@@ -417,6 +485,10 @@ def main() -> int:
     if not a.no_basin:
         log("C1: basin, one simulated day")
         out["basin_day"] = basin_day()
+    if not a.no_batch:
+        log("C4: storm ensembles as one batch")
+        sizes = [int(v) for v in a.batch_sizes.split(",") if v]
+        out["batch"] = batch_leg(case, tables, K, W, a.batch_members, sizes)
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(case, tables, eta2, nodecode, noff, a.cpu_steps,

@@ -198,6 +198,26 @@ int adcgpu_run_stats(adcgpu_ctx *ctx, int64_t *out5);
  * context's stream.  No counterpart in the reference: a reporting call. */
 int adcgpu_last_solve(adcgpu_ctx *ctx, double *out4);
 
+/* ---- batches: several contexts stepped together (ensembles, uneven meshes)
+ * No counterpart in the reference (an ensemble there is separate runs of
+ * the TIMESTEP loop, src/timestep.F).  The members are contexts on one
+ * device, of the same mesh or of different meshes; each keeps its own
+ * state, matrices and wet/dry flags.  adcgpu_batch_run advances all of them
+ * by nsteps steps: every member's passes are issued in turn on the batch's
+ * stream, and the members' JCG solves run together as one cooperative launch
+ * (more than one when their blocks would not all be resident at once), each
+ * system's blocks synchronising only among themselves and leaving at its own
+ * convergence.  Each member's results are bit-identical to adcgpu_run on that
+ * member alone.  eta_bc_tables[k] and holland[k] are what adcgpu_run takes for
+ * member k (either array may be NULL when no member needs it);
+ * adcgpu_run_stats then reports each member's run.  Members must not be used
+ * by another thread during a batch run. */
+typedef struct adcgpu_batch adcgpu_batch;
+int adcgpu_batch_create(adcgpu_ctx *const *members, int n, adcgpu_batch **out);
+int adcgpu_batch_run(adcgpu_batch *batch, int it0, int nsteps,
+                     const double *const *eta_bc_tables, const double *const *holland);
+void adcgpu_batch_destroy(adcgpu_batch *batch);
+
 /* ---- learned JCG start (warm_start = 2) ----
  * No counterpart in the reference: these replace only the initial guess of
  * CALL JCG (ETAS, src/gwce.F:2002-2004); the stopping test
