@@ -83,6 +83,18 @@ def test_coastal_large_mesh_bitexact():
     _run_bitexact(d, case.run, 12, check_every=4)
 
 
+def test_coastal_bench_size_bitexact():
+    """The 1M-node coastal mesh the benchmark times: 245 JCG blocks, near the
+    256 that are resident at once, with the pseudo-residual staged in LDS
+    (bandwidth 3,873 rows): bit-exact vs oracle over the first steps."""
+    from gpu_helpers import case_setup
+    case = meshgen.coastal(1_000_000)
+    d = case_setup(case)
+    log(f"case coastal {int(d['np'])} nodes: 8 steps bit-exact vs oracle")
+    g, _ = _run_bitexact(d, case.run, 8, check_every=4)
+    assert g.jcg_shape()[0] == 245
+
+
 def test_wetting_many_candidates_shuffled_elements():
     """A step with tens of thousands of wetting candidates whose friction
     updates chain through shared nodes, on a mesh with shuffled element
