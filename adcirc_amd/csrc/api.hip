@@ -96,6 +96,7 @@ struct adcgpu_ctx {
   int jcg_cap = 0;              // resident k_jcg blocks on this device
   int jcg_halo = -1;            // row tiles each side a tile's rows reference (-1: any)
   int jcg_hrows = -1;           // rows each side staged in LDS with a tile's (-1: none)
+  int* jhrange = nullptr;       // per row tile: first and one past the last row it references
   int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // the GWCE matrix's unchanging terms per incidence slot, and each element's
@@ -242,6 +243,7 @@ JcgArgs make_jcg_args(adcgpu_ctx* c, double* coef, const int* col, const int* nn
   a.barrier = c->jbar;
   a.halo = c->jcg_halo;
   a.hrows = c->jcg_hrows;
+  a.hrange = c->jhrange;
   a.barrier_ticks = (long long)((double)c->clock_khz * 1000.0 * g_wait_seconds);
   a.flags = c->flags;
   a.eta2 = eta2;
@@ -494,6 +496,21 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   }
   // a JCG block's rows reference rows at most this many blocks away
   const int halo_blocks = (bw + JCG_CPB * RED_CHUNK - 1) / (JCG_CPB * RED_CHUNK);
+  // per row tile of a JCG block, the rows its rows reference: [lo, hi)
+  const int ntile = (np + JCG_CPB * RED_CHUNK - 1) / (JCG_CPB * RED_CHUNK);
+  std::vector<int> hrange(2 * (size_t)ntile);
+  for (int tl = 0; tl < ntile; ++tl) {
+    hrange[2 * tl] = tl * JCG_CPB * RED_CHUNK;
+    hrange[2 * tl + 1] = std::min(np, (tl + 1) * JCG_CPB * RED_CHUNK);
+  }
+  for (int i = 0; i < np; ++i) {
+    const int tl = i / (JCG_CPB * RED_CHUNK);
+    for (int j = 1; j < nneigh[i]; ++j) {
+      const int v = col[(size_t)j * np + i];
+      hrange[2 * tl] = std::min(hrange[2 * tl], v);
+      hrange[2 * tl + 1] = std::max(hrange[2 * tl + 1], v + 1);
+    }
+  }
   std::vector<int> csrpos(6 * (size_t)ne);
   for (int e = 0; e < ne; ++e)
     for (int k = 0; k < 3; ++k)
@@ -624,6 +641,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   std::vector<double> vcor(P->corif, P->corif + np), vfric(P->fric, P->fric + np);
   std::vector<double> var(M->areas, M->areas + ne);
   U(&c->nm, nm0); U(&c->inc_ptr, cnt); U(&c->inc, inc); U(&c->slot, slot); U(&c->slot_cols, slot_cols); U(&c->nneigh, nneigh); U(&c->col, col);
+  U(&c->jhrange, hrange);
   U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_q, obc_q);
   U(&c->obc_s, obc_s); c->nobc = nobc;
   U(&c->obcg_ptr, obcg_ptr); U(&c->obcg, obcg); U(&c->vbc_ptr, vbc_ptr); U(&c->vbc, vbc);

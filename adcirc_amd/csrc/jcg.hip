@@ -41,8 +41,16 @@ namespace adcgpu {
 // false on time-out or error (a logic error; the caller leaves the kernel).
 // `bid` and `nb` are the block's index and the number of blocks of the
 // system it works on (the whole grid, or one system of a batched launch).
+// `scoped`: the block reads other blocks' data after the barrier only with
+// agent-scope atomic loads, which are coherent at agent scope by themselves
+// (they miss the CU cache and the XCD's non-coherent L2 lines; the barrier's
+// own polls rely on the same) and are issued after the poll that saw the new
+// generation returned.  The L2 invalidate of the acquire fence (about a
+// microsecond per block on MI355X) is then left out; only the compiler is
+// kept from moving loads above the wait.
 __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& gen, int* err,
-                                          long long ticks, unsigned bid, unsigned nb) {
+                                          long long ticks, unsigned bid, unsigned nb,
+                                          bool scoped = false) {
   __shared__ int failed;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -89,7 +97,10 @@ __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& gen, int* err
     }
     if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)
       failed = 1;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (scoped)
+      __asm__ volatile("" ::: "memory");
+    else
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
   return failed == 0;
@@ -100,9 +111,10 @@ __device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned& gen, int* err
 // neighbours, the mesh bandwidth wide) have published theirs.  Only the
 // neighbours' d is read before the next grid barrier, so this replaces a
 // second grid-wide barrier per iteration.  Same release / acquire pattern
-// and the same bounded wait as grid_sync.
+// (`scoped` as there) and the same bounded wait as grid_sync.
 __device__ __forceinline__ bool halo_sync(unsigned* hflags, int tile, int lo, int hi,
-                                          unsigned gen, int* err, long long ticks) {
+                                          unsigned gen, int* err, long long ticks,
+                                          bool scoped = false) {
   __shared__ int failed;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -134,7 +146,10 @@ __device__ __forceinline__ bool halo_sync(unsigned* hflags, int tile, int lo, in
         }
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (scoped)
+      __asm__ volatile("" ::: "memory");
+    else
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
   return failed == 0;
@@ -151,6 +166,33 @@ __device__ __forceinline__ T& at(T* base, unsigned i) {
 template <class T>
 __device__ __forceinline__ const T& at(const T* base, unsigned i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+
+// Agent-scope load of a value another block wrote (coherent across XCDs).
+__device__ __forceinline__ double ld_agent(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The neighbouring tiles' rows lo .. t0-1 and t1 .. hi-1 of v into the staged
+// rows (dsh holds rows slo ..); a thread's loads are all issued before their
+// LDS stores.
+__device__ __forceinline__ void stage_halo(double* dsh, const double* v, int slo, int lo, int t0,
+                                           int t1, int hi) {
+  constexpr int K = 4;
+  const int nl = t0 - lo, nh = nl + (hi - t1);
+  for (int k0 = (int)threadIdx.x; k0 < nh; k0 += K * JCG_BLOCK) {
+    double x[K];
+    int row[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const int k = k0 + m * JCG_BLOCK;
+      row[m] = k < nl ? lo + k : t1 + (k - nl);
+      x[m] = k < nh ? ld_agent(&at(v, (unsigned)row[m])) : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < K; ++m)
+      if (k0 + m * JCG_BLOCK < nh) dsh[row[m] - slo] = x[m];
+  }
 }
 
 struct ItLocal {
@@ -319,7 +361,7 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
   __shared__ ItLocal L;
   __shared__ double lds[12];
   __shared__ double tri1[JCG_MAX_ITMAX], tri2[JCG_MAX_ITMAX];
-  extern __shared__ double dsh[];  // staged pseudo-residual, rows slo .. shi-1
+  extern __shared__ double dsh[];  // staged pseudo-residual, rows slo .. t1+hrows-1
   const int np = a.n;
   const int t = threadIdx.x;
   const int nch = (np + RED_CHUNK - 1) / RED_CHUNK;
@@ -342,7 +384,9 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
   const int t0 = tile * JCG_CPB * RED_CHUNK;
   const int t1 = t0 + JCG_CPB * RED_CHUNK < np ? t0 + JCG_CPB * RED_CHUNK : np;
   const int slo = staged ? (t0 - a.hrows > 0 ? t0 - a.hrows : 0) : 0;
-  const int shi = staged ? (t1 + a.hrows < np ? t1 + a.hrows : np) : 0;
+  // the rows the tile's rows reference, within slo .. t1 + hrows - 1
+  const int hlo_row = staged ? a.hrange[2 * tile] : 0;
+  const int hhi_row = staged ? a.hrange[2 * tile + 1] : 0;
 
   if (t == 0) {
     L.in = 0;
@@ -552,8 +596,7 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
       const int i = base + RED_THREADS * r;
       if (i < np) dsh[i - slo] = dc[r];
     }
-    for (int k = slo + t; k < t0; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
-    for (int k = t1 + t; k < shi; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
+    stage_halo(dsh, dv, slo, hlo_row, t0, t1, hhi_row);
   }
   __syncthreads();
 
@@ -638,14 +681,14 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
             ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
       }
     }
-    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks, STAGED)) return;
 
     // the eigenvalue block's estimate for this iteration, loaded alongside
     // the partials
     double cme_in = 0.0, cmold_in = 0.0;
     if (t == 0) {
-      cme_in = a.saved[6 + 2 * (in & 1)];
-      cmold_in = a.saved[7 + 2 * (in & 1)];
+      cme_in = ld_agent(a.saved + 6 + 2 * (in & 1));
+      cmold_in = ld_agent(a.saved + 7 + 2 * (in & 1));
     }
     // reduce the chunk partials (every block, identical tree); the loads of
     // two chunks are issued before their sums
@@ -656,17 +699,17 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
       for (; cc + RED_XTHREADS < nch; cc += 2 * RED_XTHREADS) {
         const double* pa = part + (size_t)cc * 4;
         const double* pb = part + (size_t)(cc + RED_XTHREADS) * 4;
-        const double a0 = pa[0], a1 = pa[1], a2 = pa[2];
-        const double b0 = pb[0], b1 = pb[1], b2 = pb[2];
+        const double a0 = ld_agent(pa), a1 = ld_agent(pa + 1), a2 = ld_agent(pa + 2);
+        const double b0 = ld_agent(pb), b1 = ld_agent(pb + 1), b2 = ld_agent(pb + 2);
         r0 = (r0 + a0) + b0;
         r1 = (r1 + a1) + b1;
         r2 = (r2 + a2) + b2;
       }
       if (cc < nch) {
         const double* pa = part + (size_t)cc * 4;
-        r0 = r0 + pa[0];
-        r1 = r1 + pa[1];
-        r2 = r2 + pa[2];
+        r0 = r0 + ld_agent(pa);
+        r1 = r1 + ld_agent(pa + 1);
+        r2 = r2 + ld_agent(pa + 2);
       }
     }
     block_tree3(r0, r1, r2, lds);
@@ -705,13 +748,13 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
     } else if (!eig_block) {
       // the partials and the eigenvalue pair are double-buffered by
       // iteration parity, so only the neighbours' d has to be waited for
-      if (!halo_sync(hflags, tile, hlo, hhi, (unsigned)(in + 1), err, a.barrier_ticks)) return;
+      if (!halo_sync(hflags, tile, hlo, hhi, (unsigned)(in + 1), err, a.barrier_ticks, STAGED))
+        return;
     }
     // the neighbouring tiles' new d into LDS (each barrier above ends with
     // the block's own rows written and the neighbours' visible)
     if (staged) {
-      for (int k = slo + t; k < t0; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
-      for (int k = t1 + t; k < shi; k += JCG_BLOCK) dsh[k - slo] = at(dv, (unsigned)k);
+      stage_halo(dsh, dv, slo, hlo_row, t0, t1, hhi_row);
       __syncthreads();
     }
   }

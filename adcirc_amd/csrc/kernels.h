@@ -214,6 +214,8 @@ struct JcgArgs {
                       // in LDS for the products (dynamic shared memory of
                       // 8 (JCG_CPB RED_CHUNK + 2 hrows) bytes); -1: gathered
                       // from memory
+  const int* hrange;  // staged: per row tile, the first and one past the last row
+                      // its rows reference (within hrows of the tile)
   int halo;           // row tiles each side whose pseudo-residual a tile's rows read;
                       // -1: wait for all tiles (a grid barrier) after each update
   long long barrier_ticks;  // longest grid-barrier wait, in wall-clock ticks
