@@ -97,6 +97,7 @@ struct adcgpu_ctx {
   int jcg_halo = -1;            // row tiles each side a tile's rows reference (-1: any)
   int jcg_hrows = -1;           // rows each side staged in LDS with a tile's (-1: none)
   int* jhrange = nullptr;       // per row tile: first and one past the last row it references
+  unsigned* jlcol = nullptr;    // staged: the columns as 16-bit LDS positions, two slots a word
   int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // the GWCE matrix's unchanging terms per incidence slot, and each element's
@@ -244,6 +245,7 @@ JcgArgs make_jcg_args(adcgpu_ctx* c, double* coef, const int* col, const int* nn
   a.halo = c->jcg_halo;
   a.hrows = c->jcg_hrows;
   a.hrange = c->jhrange;
+  a.lcol = c->jlcol;
   a.barrier_ticks = (long long)((double)c->clock_khz * 1000.0 * g_wait_seconds);
   a.flags = c->flags;
   a.eta2 = eta2;
@@ -610,6 +612,20 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   // staging needs the point-to-point halo wait (every staged row belongs to a
   // tile within jcg_halo)
   c->jcg_hrows = c->jcg_halo >= 0 ? jcg_stage_rows(bw) : -1;
+  // staged: each off-diagonal column as its position in the tile's staged
+  // rows (fewer than 2^16 by the LDS bound), two slots per word
+  std::vector<uint32_t> lcol;
+  if (c->jcg_hrows >= 0) {
+    lcol.assign((size_t)std::max(mnei / 2, 1) * np, 0u);
+    for (int i = 0; i < np; ++i) {
+      const int t0 = i / (JCG_CPB * RED_CHUNK) * (JCG_CPB * RED_CHUNK);
+      const int slo = std::max(0, t0 - c->jcg_hrows);
+      for (int j = 1; j < nneigh[i]; ++j) {
+        const uint32_t v = (uint32_t)(col[(size_t)j * np + i] - slo);
+        lcol[(size_t)((j - 1) / 2) * np + i] |= v << (16 * ((j - 1) & 1));
+      }
+    }
+  }
   c->prm = *P;
   c->ifnlfa = P->nolifa == 0 ? 0.0 : 1.0;
   c->ifnlct = P->nolica == 0 ? 0.0 : 1.0;
@@ -642,6 +658,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   std::vector<double> var(M->areas, M->areas + ne);
   U(&c->nm, nm0); U(&c->inc_ptr, cnt); U(&c->inc, inc); U(&c->slot, slot); U(&c->slot_cols, slot_cols); U(&c->nneigh, nneigh); U(&c->col, col);
   U(&c->jhrange, hrange);
+  if (!lcol.empty()) U(&c->jlcol, lcol);
   U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_q, obc_q);
   U(&c->obc_s, obc_s); c->nobc = nobc;
   U(&c->obcg_ptr, obcg_ptr); U(&c->obcg, obcg); U(&c->vbc_ptr, vbc_ptr); U(&c->vbc, vbc);

@@ -617,17 +617,21 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
       // the neighbours are taken in slot order.  The next slot's column
       // index and coefficient are loaded while this slot's d is gathered, so
       // each slot costs one memory round trip instead of two.
+      // Staged, the column indices come as LDS positions, 16 bits per slot,
+      // two slots per word (a.lcol): one index load per two slots.
 #pragma unroll
       for (int h = 0; h < RED_PER_THREAD; h += 4) {
         int jm = 1;
-        unsigned cn[4];
+        unsigned cn[4];  // unstaged: the next slot's column; staged: its pair word
         double sn[4];
 #pragma unroll
         for (int r = h; r < h + 4; ++r) {
           nd[r] = 0.0;
           jm = NN(r) > jm ? NN(r) : jm;
           const unsigned q = (unsigned)(np + base + RED_THREADS * r);
-          cn[r - h] = 1 < NN(r) ? (unsigned)at(a.col, q) : 0u;
+          cn[r - h] = 1 < NN(r) ? (STAGED ? at(a.lcol, (unsigned)(base + RED_THREADS * r))
+                                          : (unsigned)at(a.col, q))
+                                : 0u;
           sn[r - h] = 1 < NN(r) ? at(a.cs, q) : 0.0;
         }
         for (int j = 1; j < jm; ++j) {
@@ -635,15 +639,20 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
           double sc[4];
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            cc[k] = cn[k];
+            // slot j is the low half of its pair word when j is odd
+            cc[k] = STAGED ? ((j & 1) ? (cn[k] & 0xffffu) : (cn[k] >> 16)) : cn[k];
             sc[k] = sn[k];
           }
           if (j + 1 < jm) {
             const unsigned q0 = (unsigned)((j + 1) * np + base);
+            const unsigned p0 = (unsigned)((j >> 1) * np + base);  // pair of slot j+1
 #pragma unroll
             for (int r = h; r < h + 4; ++r) {
               if (j + 1 < NN(r)) {
-                cn[r - h] = (unsigned)at(a.col, q0 + RED_THREADS * r);
+                if (!STAGED)
+                  cn[r - h] = (unsigned)at(a.col, q0 + RED_THREADS * r);
+                else if (!(j & 1))
+                  cn[r - h] = at(a.lcol, p0 + RED_THREADS * r);
                 sn[r - h] = at(a.cs, q0 + RED_THREADS * r);
               }
             }
@@ -651,8 +660,7 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
 #pragma unroll
           for (int r = h; r < h + 4; ++r)
             if (j < NN(r))
-              nd[r] = nd[r] - sc[r - h] * (STAGED ? dsh[cc[r - h] - (unsigned)slo]
-                                                  : at(dv, cc[r - h]));
+              nd[r] = nd[r] - sc[r - h] * (STAGED ? dsh[cc[r - h]] : at(dv, cc[r - h]));
         }
       }
       double add = 0.0, adt = 0.0, auu = 0.0;

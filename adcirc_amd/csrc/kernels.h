@@ -25,6 +25,7 @@ constexpr int JCG_MAX_ITMAX = 256;       // tridiagonal history length (itmax)
 // dynamic LDS for the staged pseudo-residual of a k_jcg block (JcgArgs.hrows),
 // at most: the block's rows and 5,632 rows each side
 constexpr int JCG_STAGE_MAX_BYTES = 120 * 1024;
+static_assert(JCG_STAGE_MAX_BYTES / 8 <= 65536, "staged positions are 16-bit");
 constexpr int NODE_STAGE = 2048;   // incidence slots a 256-node block stages in LDS
 constexpr int LHS_MAXW = 32;         // widest COEF row (MNEI): LDS of the row kernel
 // ordered wetting updates: candidates are closed in blocks of WD_BLOCK, each
@@ -216,6 +217,9 @@ struct JcgArgs {
                       // from memory
   const int* hrange;  // staged: per row tile, the first and one past the last row
                       // its rows reference (within hrows of the tile)
+  const unsigned* lcol;  // staged: [(mnei/2)][n] per row, off-diagonal slots 2p+1
+                         // and 2p+2 as positions in the tile's staged rows, 16
+                         // bits each (low half first)
   int halo;           // row tiles each side whose pseudo-residual a tile's rows read;
                       // -1: wait for all tiles (a grid barrier) after each update
   long long barrier_ticks;  // longest grid-barrier wait, in wall-clock ticks
