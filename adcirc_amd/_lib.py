@@ -33,7 +33,7 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_ml_collect", "adcgpu_ml_samples", "adcgpu_ml_train", "adcgpu_ml_grad",
            "adcgpu_ml_features", "adcgpu_ml_correction", "adcgpu_state_bytes",
            "adcgpu_state_save", "adcgpu_state_load", "adcgpu_last_solve",
-           "adcgpu_batch_create", "adcgpu_batch_run", "adcgpu_batch_destroy")
+           "adcgpu_set_jcg_blocks", "adcgpu_batch_create", "adcgpu_batch_run", "adcgpu_batch_destroy")
 ML_NPARAM = 22913
 # error codes (ADCGPU_E_*)
 E_ARG, E_UNSUPPORTED, E_HIP, E_DIAG, E_CAPACITY, E_NONFINITE, E_ERRORELEV = range(-1, -8, -1)
@@ -101,6 +101,7 @@ def load() -> C.CDLL:
     lib.adcgpu_jcg_shape.argtypes = [vp, ip_]
     lib.adcgpu_run_stats.argtypes = [vp, C.POINTER(C.c_int64)]
     lib.adcgpu_last_solve.argtypes = [vp, dp_]
+    lib.adcgpu_set_jcg_blocks.argtypes = [vp, C.c_int]
     lib.adcgpu_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     lib.adcgpu_batch_run.argtypes = [vp, C.c_int, C.c_int, C.POINTER(dp_), C.POINTER(dp_)]
     lib.adcgpu_batch_destroy.argtypes = [vp]

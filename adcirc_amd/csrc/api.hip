@@ -98,6 +98,10 @@ struct adcgpu_ctx {
   int jcg_hrows = -1;           // rows each side staged in LDS with a tile's (-1: none)
   int* jhrange = nullptr;       // per row tile: first and one past the last row it references
   unsigned* jlcol = nullptr;    // staged: the columns as 16-bit LDS positions, two slots a word
+  int jcg_limit = 0;            // caller's limit on JCG blocks per launch (0: the device's)
+  bool jcg_mem = false;         // more row tiles than blocks: k_jcg_mem
+  double *juo = nullptr, *jdpv = nullptr, *jndv = nullptr;  // k_jcg_mem's iterates
+  int jmem_n = 0;               // rows those hold
   int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // the GWCE matrix's unchanging terms per incidence slot, and each element's
@@ -186,14 +190,39 @@ int flag_errors(const int* f) {
   return 0;
 }
 
-// JCG_CPB chunks of 2048 rows per block plus the eigenvalue block, all resident.
+// JCG_CPB chunks of 2048 rows per block plus the eigenvalue block, all
+// resident; a system with more row tiles than the blocks allowed (the
+// resident ones, or the caller's limit) runs memory-resident (k_jcg_mem) on
+// all of them.
 int set_jcg_grid(adcgpu_ctx* c) {
-  const int grid = jcg_grid_for(c->np);
-  if (grid > c->jcg_cap)
-    return fail(ADCGPU_E_CAPACITY, "system too large for the register-resident JCG (" +
-                                       std::to_string(c->nch) + " chunks of 2048 rows, " +
-                                       std::to_string(c->jcg_cap) + " resident blocks)");
-  c->jcg_grid = grid;
+  const int need = jcg_grid_for(c->np);
+  const int lim = c->jcg_limit > 0 ? std::min(c->jcg_limit, c->jcg_cap) : c->jcg_cap;
+  if (need <= lim) {
+    c->jcg_grid = need;
+    c->jcg_mem = false;
+    return 0;
+  }
+  if (lim < 2)
+    return fail(ADCGPU_E_CAPACITY, "JCG needs at least two resident blocks");
+  if (c->prm.warm_start == 2)
+    return fail(ADCGPU_E_UNSUPPORTED,
+                "the learned start needs a system the register-resident JCG holds (" +
+                    std::to_string((lim - 1) * JCG_CPB * RED_CHUNK) + " rows in " +
+                    std::to_string(lim) + " blocks)");
+  if (c->jmem_n < c->np) {
+    for (double** p : {&c->juo, &c->jdpv, &c->jndv}) {
+      if (*p) {
+        c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)*p));
+        (void)hipFree(*p);
+        *p = nullptr;
+      }
+      if (int rc = dmalloc(p, (size_t)c->np)) return rc;
+      c->allocs.push_back((void*)*p);
+    }
+    c->jmem_n = c->np;
+  }
+  c->jcg_grid = lim;
+  c->jcg_mem = true;
   return 0;
 }
 
@@ -246,6 +275,13 @@ JcgArgs make_jcg_args(adcgpu_ctx* c, double* coef, const int* col, const int* nn
   a.hrows = c->jcg_hrows;
   a.hrange = c->jhrange;
   a.lcol = c->jlcol;
+  a.uo = c->juo;
+  a.dpv = c->jdpv;
+  a.ndv = c->jndv;
+  if (c->jcg_mem) {  // gathered products, grid barriers
+    a.hrows = -1;
+    a.halo = -1;
+  }
   a.barrier_ticks = (long long)((double)c->clock_khz * 1000.0 * g_wait_seconds);
   a.flags = c->flags;
   a.eta2 = eta2;
@@ -279,8 +315,9 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
     HIPCHK(hipEventRecord(c->jcg_events[c->ev_used].first, c->stream));
   }
   const void* fn =
-      a.hrows >= 0 ? (const void*)k_jcg<false, true>
-                   : (u2 ? (const void*)k_jcg<true, false> : (const void*)k_jcg<false, false>);
+      c->jcg_mem ? (const void*)k_jcg_mem
+      : a.hrows >= 0 ? (const void*)k_jcg<false, true>
+                     : (u2 ? (const void*)k_jcg<true, false> : (const void*)k_jcg<false, false>);
   HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->jcg_grid), dim3(JCG_BLOCK), args,
                                     (unsigned)jcg_stage_bytes(a.hrows), c->stream));
   if (c->profile) HIPCHK(hipEventRecord(c->jcg_events[c->ev_used++].second, c->stream));
@@ -445,6 +482,11 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   const int np = M->np, ne = M->ne, mnei = M->mnei, neta = M->neta;
   if (np <= 0 || ne <= 0 || mnei < 2 || neta < 0 || M->nvel < 0 || M->nvelme < 0)
     return fail(ADCGPU_E_ARG, "invalid sizes");
+  // the kernels index with 32-bit offsets: the widest arrays (MNEI doubles
+  // per node, three slots per element) stay within them
+  if ((size_t)mnei * (size_t)np * sizeof(double) >= ((size_t)1 << 32) ||
+      3 * (size_t)ne >= ((size_t)1 << 31))
+    return fail(ADCGPU_E_CAPACITY, "mesh too large for 32-bit offsets");
   if (mnei > LHS_MAXW)
     return fail(ADCGPU_E_UNSUPPORTED, "MNEI above " + std::to_string(LHS_MAXW));
   if (P->nolibf == 2) return fail(ADCGPU_E_UNSUPPORTED, "hybrid bottom friction (NOLIBF=2)");
@@ -1404,6 +1446,8 @@ int adcgpu_batch_create(adcgpu_ctx* const* members, int n, adcgpu_batch** out) {
     if (!members[k]) return fail(ADCGPU_E_ARG, "null member");
     if (members[k]->device != members[0]->device)
       return fail(ADCGPU_E_ARG, "batch members must share one device");
+    if (members[k]->jcg_mem)
+      return fail(ADCGPU_E_UNSUPPORTED, "batch members must fit the register-resident JCG");
     for (int l = 0; l < k; ++l)
       if (members[l] == members[k]) return fail(ADCGPU_E_ARG, "a context appears twice");
   }
@@ -1568,6 +1612,19 @@ int adcgpu_last_solve(adcgpu_ctx* c, double* out4) {
   const bool dual = c->prm.warm_start == 2 && c->prm.ilump == 0;
   out4[2] = dual ? s[10] : 0.0;
   out4[3] = dual ? s[11] : 0.0;
+  return 0;
+}
+
+int adcgpu_set_jcg_blocks(adcgpu_ctx* c, int blocks) {
+  if (!c || blocks < 0 || blocks == 1) return fail(ADCGPU_E_ARG, "invalid argument");
+  HIPCHK(hipSetDevice(c->device));
+  const int old = c->jcg_limit;
+  c->jcg_limit = blocks;
+  if (int rc = set_jcg_grid(c)) {
+    c->jcg_limit = old;
+    (void)set_jcg_grid(c);
+    return rc;
+  }
   return 0;
 }
 
@@ -1855,6 +1912,8 @@ int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double*
     return fail(ADCGPU_E_UNSUPPORTED, "only adaptive JCG without red-black or row removal");
   const int itmax = iparm[0];
   if (itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
+  if ((size_t)maxnz * (size_t)n * sizeof(double) >= ((size_t)1 << 32))
+    return fail(ADCGPU_E_CAPACITY, "system too large for 32-bit offsets");
   // compact the ELLPACK rows: diagonal first (scal moves it there,
   // src/itpackv.F:2331-2356), padding (jcoef <= 0) dropped
   int width = 1;

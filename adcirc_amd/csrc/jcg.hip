@@ -349,6 +349,127 @@ __device__ void jcg_parcon(ItLocal& L, double dtnrm) {
 }
 
 
+// The solve's scalar state at the start of a launch: ITPACK's SAVEd values
+// from the last solve, the rest cleared (thread 0).
+__device__ __forceinline__ void jcg_init_state(ItLocal& L, const JcgArgs& a) {
+  L.in = 0;
+  L.adapt = 1;
+  L.halt = 0;
+  L.q1 = a.saved[0] != 0.0;
+  L.gamold = a.saved[1];
+  L.rhoold = a.saved[2];
+  L.cmold = a.saved[3];
+  L.itmax = a.itmax;
+  L.zeta = a.zeta;
+  L.cme = 0.0;
+  L.gamma = 0.0;
+  L.rho = 0.0;
+  L.delnnm = 0.0;
+  L.delsnm = 0.0;
+  L.udnm = 0.0;
+  L.stptst = 0.0;
+  L.maxfn = a.maxfn;
+  L.tz = a.tz;
+  L.stop = 0;
+}
+
+// After iteration in's grid barrier: the chunk partials reduced by every
+// block with the same fixed tree (the loads of two chunks issued before
+// their sums), with the eigenvalue block's estimate for the iteration; then
+// pstop and parcon (thread 0), so that every block takes the same decision.
+__device__ __forceinline__ void jcg_scalars(ItLocal& L, const JcgArgs& a, double* lds, int in,
+                                            int nch, bool need_u) {
+  const int t = threadIdx.x;
+  double cme_in = 0.0, cmold_in = 0.0;
+  if (t == 0) {
+    cme_in = ld_agent(a.saved + 6 + 2 * (in & 1));
+    cmold_in = ld_agent(a.saved + 7 + 2 * (in & 1));
+  }
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+  const double* part = a.part + (size_t)(in & 1) * nch * 4;
+  if (t < RED_XTHREADS) {
+    int cc = t;
+    for (; cc + RED_XTHREADS < nch; cc += 2 * RED_XTHREADS) {
+      const double* pa = part + (size_t)cc * 4;
+      const double* pb = part + (size_t)(cc + RED_XTHREADS) * 4;
+      const double a0 = ld_agent(pa), a1 = ld_agent(pa + 1), a2 = ld_agent(pa + 2);
+      const double b0 = ld_agent(pb), b1 = ld_agent(pb + 1), b2 = ld_agent(pb + 2);
+      r0 = (r0 + a0) + b0;
+      r1 = (r1 + a1) + b1;
+      r2 = (r2 + a2) + b2;
+    }
+    if (cc < nch) {
+      const double* pa = part + (size_t)cc * 4;
+      r0 = r0 + ld_agent(pa);
+      r1 = r1 + ld_agent(pa + 1);
+      r2 = r2 + ld_agent(pa + 2);
+    }
+  }
+  block_tree3(r0, r1, r2, lds);
+  const double delnnm = r0, dtnrm = r1, unorm_s = r2;
+  if (t == 0) {
+    L.in = in;
+    L.cme = cme_in;
+    L.cmold = cmold_in;
+    L.delnnm = delnnm;
+    jcg_pstop(L, need_u ? unorm_s : 1.0, delnnm);
+    if (!L.halt) jcg_parcon(L, dtnrm);
+  }
+  __syncthreads();
+}
+
+// The eigenvalue block's part of iteration in: the adaptive estimate,
+// published by iteration parity (a block may still read the last
+// iteration's pair).
+__device__ __forceinline__ void jcg_eig_step(ItLocal& L, const JcgArgs& a, double* tri1,
+                                             double* tri2, int in) {
+  if (threadIdx.x == 0) {
+    L.in = in;
+    if (L.adapt) jcg_chgcon(L, tri1, tri2);
+    a.saved[6 + 2 * (in & 1)] = L.cme;
+    a.saved[7 + 2 * (in & 1)] = L.cmold;
+  }
+}
+
+// The solve's outcome and SAVEd state (block 0, thread 0).
+__device__ __forceinline__ void jcg_finish(const ItLocal& L, const JcgArgs& a, bool halted) {
+  const int in_final = halted ? L.in : a.itmax;
+  a.flags[F_NUMITR] = in_final;
+  a.flags[F_IER] = halted ? 0 : 13;
+  a.saved[0] = L.q1 ? 1.0 : 0.0;
+  a.saved[1] = L.gamold;
+  a.saved[2] = L.rhoold;
+  a.saved[3] = L.cmold;
+  a.saved[4] = L.cme;
+  a.saved[5] = L.stptst;
+  a.flags[F_ITSUM] += in_final;
+  if (!halted) a.flags[F_NITMAX] += 1;
+}
+
+// The per-row partials of one 2048-row chunk (threads g*512 .. g*512+511 of
+// the block, chunk c) into the chunk's slot of the partials by parity:
+// ((w0+w1)+(w2+w3))+((w4+w5)+(w6+w7)) over the chunk's 8 waves.  Every
+// thread of the block calls it.
+__device__ __forceinline__ void jcg_chunk_partials(const JcgArgs& a, double (*wsum)[24], int c,
+                                                   int nch, int in, double add, double adt,
+                                                   double auu) {
+  const int t = threadIdx.x;
+  const double s0 = wave_sum(add), s1 = wave_sum(adt), s2 = wave_sum(auu);
+  const int tt = t % RED_THREADS;
+  double* ws = wsum[t / RED_THREADS];
+  if ((tt & 63) == 0) {
+    ws[0 + (tt >> 6)] = s0;
+    ws[8 + (tt >> 6)] = s1;
+    ws[16 + (tt >> 6)] = s2;
+  }
+  __syncthreads();
+  if (tt < 3 && c < nch) {
+    const double* v = ws + 8 * tt;
+    a.part[(size_t)(in & 1) * nch * 4 + (size_t)c * 4 + tt] =
+        ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+  }
+}
+
 // DUAL: two initial guesses, a.u (the reference's) and a.u2 (the learned
 // one); the pseudo-residuals of both are formed in the scaling pass and the
 // iteration starts from the one with the smaller norm (the reference's on a
@@ -388,27 +509,7 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
   const int hlo_row = staged ? a.hrange[2 * tile] : 0;
   const int hhi_row = staged ? a.hrange[2 * tile + 1] : 0;
 
-  if (t == 0) {
-    L.in = 0;
-    L.adapt = 1;
-    L.halt = 0;
-    L.q1 = a.saved[0] != 0.0;
-    L.gamold = a.saved[1];
-    L.rhoold = a.saved[2];
-    L.cmold = a.saved[3];
-    L.itmax = a.itmax;
-    L.zeta = a.zeta;
-    L.cme = 0.0;
-    L.gamma = 0.0;
-    L.rho = 0.0;
-    L.delnnm = 0.0;
-    L.delsnm = 0.0;
-    L.udnm = 0.0;
-    L.stptst = 0.0;
-    L.maxfn = a.maxfn;
-    L.tz = a.tz;
-    L.stop = 0;
-  }
+  if (t == 0) jcg_init_state(L, a);
   __syncthreads();
 
   // per-row iterates, register resident
@@ -604,14 +705,7 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
   for (int in = 0; in <= a.itmax; ++in) {
     const bool need_u = !(L.q1 || (in > 5 && (in % 5) != 0));
     if (eig_block) {
-      // adaptive eigenvalue estimate for this iteration
-      if (t == 0) {
-        L.in = in;
-        if (L.adapt) jcg_chgcon(L, tri1, tri2);
-        // by parity: a block may still read the last iteration's pair
-        a.saved[6 + 2 * (in & 1)] = L.cme;
-        a.saved[7 + 2 * (in & 1)] = L.cmold;
-      }
+      jcg_eig_step(L, a, tri1, tri2, in);
     } else {
       // N d, rows in groups of four (bounded register pressure); per row
       // the neighbours are taken in slot order.  The next slot's column
@@ -672,65 +766,12 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
           if (need_u) auu = auu + uc[r] * uc[r];
         }
       }
-      const double s0 = wave_sum(add), s1 = wave_sum(adt), s2 = wave_sum(auu);
-      // chunk value: ((w0+w1)+(w2+w3))+((w4+w5)+(w6+w7)) over the 8 waves
       __shared__ double wsum[JCG_CPB][24];
-      const int tt = t % RED_THREADS;
-      double* ws = wsum[t / RED_THREADS];
-      if ((tt & 63) == 0) {
-        ws[0 + (tt >> 6)] = s0;
-        ws[8 + (tt >> 6)] = s1;
-        ws[16 + (tt >> 6)] = s2;
-      }
-      __syncthreads();
-      if (tt < 3 && c < nch) {
-        const double* v = ws + 8 * tt;
-        a.part[(size_t)(in & 1) * nch * 4 + (size_t)c * 4 + tt] =
-            ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-      }
+      jcg_chunk_partials(a, wsum, c, nch, in, add, adt, auu);
     }
     if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks, STAGED)) return;
 
-    // the eigenvalue block's estimate for this iteration, loaded alongside
-    // the partials
-    double cme_in = 0.0, cmold_in = 0.0;
-    if (t == 0) {
-      cme_in = ld_agent(a.saved + 6 + 2 * (in & 1));
-      cmold_in = ld_agent(a.saved + 7 + 2 * (in & 1));
-    }
-    // reduce the chunk partials (every block, identical tree); the loads of
-    // two chunks are issued before their sums
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-    const double* part = a.part + (size_t)(in & 1) * nch * 4;
-    if (t < RED_XTHREADS) {
-      int cc = t;
-      for (; cc + RED_XTHREADS < nch; cc += 2 * RED_XTHREADS) {
-        const double* pa = part + (size_t)cc * 4;
-        const double* pb = part + (size_t)(cc + RED_XTHREADS) * 4;
-        const double a0 = ld_agent(pa), a1 = ld_agent(pa + 1), a2 = ld_agent(pa + 2);
-        const double b0 = ld_agent(pb), b1 = ld_agent(pb + 1), b2 = ld_agent(pb + 2);
-        r0 = (r0 + a0) + b0;
-        r1 = (r1 + a1) + b1;
-        r2 = (r2 + a2) + b2;
-      }
-      if (cc < nch) {
-        const double* pa = part + (size_t)cc * 4;
-        r0 = r0 + ld_agent(pa);
-        r1 = r1 + ld_agent(pa + 1);
-        r2 = r2 + ld_agent(pa + 2);
-      }
-    }
-    block_tree3(r0, r1, r2, lds);
-    const double delnnm = r0, dtnrm = r1, unorm_s = r2;
-    if (t == 0) {
-      L.in = in;
-      L.cme = cme_in;
-      L.cmold = cmold_in;
-      L.delnnm = delnnm;
-      jcg_pstop(L, need_u ? unorm_s : 1.0, delnnm);
-      if (!L.halt) jcg_parcon(L, dtnrm);
-    }
-    __syncthreads();
+    jcg_scalars(L, a, lds, in, nch, need_u);
     if (L.halt) break;
     const double c1 = L.c1, c2 = L.c2, c3 = L.c3, c4 = L.c4;
     if (!eig_block) {
@@ -804,19 +845,7 @@ __device__ __forceinline__ void jcg_solve(const JcgArgs& a, int bid, int nblocks
     }
   }
 #undef NN
-  if (bid == 0 && t == 0) {
-    const int in_final = halted ? L.in : a.itmax;
-    a.flags[F_NUMITR] = in_final;
-    a.flags[F_IER] = halted ? 0 : 13;
-    a.saved[0] = L.q1 ? 1.0 : 0.0;
-    a.saved[1] = L.gamold;
-    a.saved[2] = L.rhoold;
-    a.saved[3] = L.cmold;
-    a.saved[4] = L.cme;
-    a.saved[5] = L.stptst;
-    a.flags[F_ITSUM] += in_final;
-    if (!halted) a.flags[F_NITMAX] += 1;
-  }
+  if (bid == 0 && t == 0) jcg_finish(L, a, halted);
 }
 
 template <bool DUAL, bool STAGED>
@@ -835,6 +864,154 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg_batch(const JcgArgs* __restri
   int m = 0;
   while (m + 1 < nsys && first[m + 1] <= b) ++m;
   jcg_solve<DUAL, STAGED>(args[m], b - first[m], first[m + 1] - first[m]);
+}
+
+// ITPACK jcg for systems with more row tiles than blocks (more than the
+// resident blocks allow, or a caller's limit): each row block works through
+// the tiles bid, bid + nrb, ... with the iterates in memory (u in a.u, the
+// previous u in a.uo, d in a.d, the previous d in a.dpv, N d in a.ndv), the
+// products gathered from memory and two grid barriers per iteration (after
+// the products, after the update).  Every row's operations and the
+// reduction tree are those of jcg_solve, so the results are the same.
+__device__ void jcg_solve_mem(const JcgArgs& a, int bid, int nblocks) {
+  __shared__ ItLocal L;
+  __shared__ double lds[12];
+  __shared__ double tri1[JCG_MAX_ITMAX], tri2[JCG_MAX_ITMAX];
+  __shared__ double wsum[JCG_CPB][24];
+  const int np = a.n;
+  const int t = threadIdx.x;
+  const int nch = (np + RED_CHUNK - 1) / RED_CHUNK;
+  const int ntile = (nch + JCG_CPB - 1) / JCG_CPB;
+  const int nrb = nblocks - 1;  // row blocks; the last block estimates the eigenvalue
+  const bool eig_block = bid == nrb;
+  const int g = t / RED_THREADS, tt = t % RED_THREADS;
+  unsigned gen = 0;
+  int* err = a.flags + F_ERROR;
+  double* u = a.u;
+  double* d = a.d;
+  if (t == 0) jcg_init_state(L, a);
+  __syncthreads();
+
+  // ---- scal, part 1 (src/itpackv.F:2374-2393), unless already done
+  if (!eig_block && !a.prescaled)
+    for (int tl = bid; tl < ntile; tl += nrb)
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        const int i = (tl * JCG_CPB + g) * RED_CHUNK + tt + RED_THREADS * r;
+        if (i >= np) continue;
+        const double dg = at(a.coef, i);
+        if (!(dg > 0.0)) atomicOr(err, ERR_DIAG);
+        const double sq = __builtin_sqrt(dg);
+        at(u, i) = at(u, i) * sq;
+        const double w = 1.0 / sq;
+        at(a.w, i) = w;
+        at(a.rhs, i) = at(a.rhs, i) * w;
+      }
+  if (!a.prescaled)
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
+
+  // ---- scal, part 2 and d0 = rhs - N u - u (src/itpackv.F:2394-2400, 346-353)
+  if (!eig_block)
+    for (int tl = bid; tl < ntile; tl += nrb)
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        const int i = (tl * JCG_CPB + g) * RED_CHUNK + tt + RED_THREADS * r;
+        if (i >= np) continue;
+        const double wi = at(a.w, i);
+        double dci = at(a.rhs, i);
+        const int nn = at(a.nneigh, i);
+        for (int j = 1; j < nn; ++j) {
+          const unsigned q = (unsigned)(j * np + i);
+          const unsigned cc = (unsigned)at(a.col, q);
+          const double cs = at(a.coef, q) * wi * at(a.w, cc);
+          at(a.cs, q) = cs;
+          dci = dci - cs * at(u, cc);
+        }
+        at(d, i) = dci - at(u, i);
+        at(a.dpv, i) = 0.0;
+        at(a.uo, i) = 0.0;
+      }
+  if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
+
+  // ---- iterations
+  for (int in = 0; in <= a.itmax; ++in) {
+    const bool need_u = !(L.q1 || (in > 5 && (in % 5) != 0));
+    if (eig_block) {
+      jcg_eig_step(L, a, tri1, tri2, in);
+    } else {
+      for (int tl = bid; tl < ntile; tl += nrb) {
+        const int c = tl * JCG_CPB + g;
+        double add = 0.0, adt = 0.0, auu = 0.0;
+        for (int r = 0; r < RED_PER_THREAD; ++r) {
+          const int i = c * RED_CHUNK + tt + RED_THREADS * r;
+          if (i >= np) continue;
+          const int nn = at(a.nneigh, i);
+          double ndi = 0.0;
+          for (int j = 1; j < nn; ++j) {
+            const unsigned q = (unsigned)(j * np + i);
+            ndi = ndi - at(a.cs, q) * at(d, (unsigned)at(a.col, q));
+          }
+          at(a.ndv, i) = ndi;
+          const double dci = at(d, i);
+          add = add + dci * dci;
+          adt = adt + dci * ndi;
+          if (need_u) {
+            const double uci = at(u, i);
+            auu = auu + uci * uci;
+          }
+        }
+        jcg_chunk_partials(a, wsum, c, nch, in, add, adt, auu);
+        __syncthreads();  // wsum is reused by the next tile
+      }
+    }
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
+    jcg_scalars(L, a, lds, in, nch, need_u);
+    if (L.halt) break;
+    const double c1 = L.c1, c2 = L.c2, c3 = L.c3, c4 = L.c4;
+    if (!eig_block)
+      for (int tl = bid; tl < ntile; tl += nrb)
+        for (int r = 0; r < RED_PER_THREAD; ++r) {
+          const int i = (tl * JCG_CPB + g) * RED_CHUNK + tt + RED_THREADS * r;
+          if (i >= np) continue;
+          const double dci = at(d, i), uci = at(u, i);
+          const double un = c1 * dci + c2 * uci + c3 * at(a.uo, i);
+          const double dn = c1 * at(a.ndv, i) + c4 * dci + c3 * at(a.dpv, i);
+          at(a.uo, i) = uci;
+          at(u, i) = un;
+          at(a.dpv, i) = dci;
+          at(d, i) = dn;
+        }
+    if (!grid_sync(a.barrier, gen, err, a.barrier_ticks, bid, nblocks)) return;
+  }
+  // the iterate of the last iteration index: u at the halt, or the one
+  // before the final update when ITMAX ran out
+  const bool halted = L.halt != 0;
+
+  // ---- unscal (src/itpackv.F:2424-2479), ETAS and ETA2 (src/gwce.F:2006-2008)
+  if (!eig_block)
+    for (int tl = bid; tl < ntile; tl += nrb)
+      for (int r = 0; r < RED_PER_THREAD; ++r) {
+        const int i = (tl * JCG_CPB + g) * RED_CHUNK + tt + RED_THREADS * r;
+        if (i >= np) continue;
+        const double wi = at(a.w, i);
+        const double ur = (halted ? at(u, i) : at(a.uo, i)) * wi;
+        const double ri = 1.0 / wi;
+        at(a.coef, i) = ri * ri;
+        at(a.rhs, i) = at(a.rhs, i) * ri;
+        at(u, i) = ur;
+        if (a.eta2)
+          at(a.eta2, i) = (double)at(a.nodecode, i) * (ur + 0.0) + at(a.eta1, i);
+        if (!a.defer_unscal) {
+          const int nn = at(a.nneigh, i);
+          for (int j = 1; j < nn; ++j) {
+            const unsigned q = (unsigned)(j * np + i);
+            at(a.coef, q) = at(a.cs, q) * ri * (1.0 / at(a.w, (unsigned)at(a.col, q)));
+          }
+        }
+      }
+  if (bid == 0 && t == 0) jcg_finish(L, a, halted);
+}
+
+__global__ __launch_bounds__(JCG_BLOCK) void k_jcg_mem(JcgArgs a) {
+  jcg_solve_mem(a, (int)blockIdx.x, (int)gridDim.x);
 }
 
 template __global__ void k_jcg<false, false>(JcgArgs);

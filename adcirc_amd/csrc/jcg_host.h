@@ -62,7 +62,8 @@ inline hipError_t jcg_resident_cap_of(const void* fn, int* per_cu_out) {
 inline hipError_t jcg_resident_cap(int device, int* cap) {
   const void* fns[] = {(const void*)k_jcg<false, false>,       (const void*)k_jcg<true, false>,
                        (const void*)k_jcg<false, true>,        (const void*)k_jcg_batch<false, false>,
-                       (const void*)k_jcg_batch<true, false>,  (const void*)k_jcg_batch<false, true>};
+                       (const void*)k_jcg_batch<true, false>,  (const void*)k_jcg_batch<false, true>,
+                       (const void*)k_jcg_mem};
   int per_cu = 1 << 30;
   hipError_t e = hipSuccess;
   for (const void* fn : fns) {

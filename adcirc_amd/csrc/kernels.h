@@ -227,6 +227,9 @@ struct JcgArgs {
   double* eta2;       // optional: ETA2 = NODECODE*(ETAS+0)+ETA1
   const double* eta1;
   const int* nodecode;
+  double* uo;         // k_jcg_mem: [n] the previous u
+  double* dpv;        //             [n] the previous d
+  double* ndv;        //             [n] N d
   const double* u2;   // k_jcg<true>: the second (learned) start, scaled like u
   double* d2;         //              and its initial pseudo-residual
 };
@@ -300,6 +303,8 @@ __global__ void k_jcg(JcgArgs a);
 // nsys systems in one launch: blocks first[m] .. first[m+1]-1 solve args[m]
 template <bool DUAL, bool STAGED>
 __global__ void k_jcg_batch(const JcgArgs* args, const int* first, int nsys);
+// more row tiles than blocks: the iterates in memory (a.uo, a.dpv, a.ndv)
+__global__ void k_jcg_mem(JcgArgs a);
 __global__ void k_wd_d1(DevMesh m, DevState s, int* flags, double h0, double habsmin);
 __global__ void k_wd_elem(DevMesh m, DevState s, WdPrm q, WdWork w);
 __global__ void k_wd_index(const int* bcnt, int nb, int* bofs, int* flags);

@@ -360,6 +360,11 @@ class Model:
         check(self.lib.adcgpu_jcg_shape(self.h, _p(out, ip_)), "jcg_shape")
         return tuple(int(v) for v in out)
 
+    def set_jcg_blocks(self, blocks: int):
+        """Limits JCG launches to `blocks` blocks (0: the device's resident
+        count); larger systems are then solved memory-resident."""
+        check(self.lib.adcgpu_set_jcg_blocks(self.h, int(blocks)), "set_jcg_blocks")
+
     def synchronize(self):
         check(self.lib.adcgpu_synchronize(self.h), "synchronize")
 

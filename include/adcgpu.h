@@ -197,6 +197,15 @@ int adcgpu_run_stats(adcgpu_ctx *ctx, int64_t *out5);
  * (warm_start = 2; the smaller one was used), 0 otherwise.  Synchronises the
  * context's stream.  No counterpart in the reference: a reporting call. */
 int adcgpu_last_solve(adcgpu_ctx *ctx, double *out4);
+/* Limits the blocks of the context's JCG launches to `blocks` (>= 2; 0: as
+ * many as are resident on the device, the default).  A system with more
+ * 4,096-row tiles than the blocks allow, by this limit or beyond 1,044,480
+ * rows on MI355X, is solved memory-resident (iterates in HBM, products
+ * gathered, two grid barriers per iteration) with the same results.  Not
+ * with warm_start = 2, nor for contexts that belong to a batch.  No
+ * counterpart in the reference: for sharing the device and for tests of the
+ * memory-resident solve. */
+int adcgpu_set_jcg_blocks(adcgpu_ctx *ctx, int blocks);
 
 /* ---- batches: several contexts stepped together (ensembles, uneven meshes)
  * No counterpart in the reference (an ensemble there is separate runs of
