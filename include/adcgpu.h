@@ -41,7 +41,8 @@ extern "C" {
 #define ADCGPU_E_UNSUPPORTED (-2) /* configuration outside the implemented subset */
 #define ADCGPU_E_HIP (-3)         /* HIP runtime error (message via adcgpu_last_error) */
 #define ADCGPU_E_DIAG (-4)        /* GWCE diagonal <= 0 (src/gwce.F:678-707 terminates) */
-#define ADCGPU_E_CAPACITY (-5)    /* the mesh has more rows than the resident JCG grid holds */
+#define ADCGPU_E_CAPACITY (-5)    /* the mesh exceeds the kernels' 32-bit offsets, or a JCG
+                                     block limit below two */
 #define ADCGPU_E_NONFINITE (-6)   /* a non-finite velocity (UU2 or VV2) at some node: the
                                      reference's NaN velocity check, src/timestep.F:1413-1471 */
 #define ADCGPU_E_ERRORELEV (-7)   /* |ETA2| above ErrorElev at a wet node: the reference
