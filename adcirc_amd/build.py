@@ -54,6 +54,7 @@ def build(verbose: bool = False) -> str:
 
 FORTRAN = os.path.join(HERE, "fortran")
 FORTRAN_CHECK = os.path.join(FORTRAN, "jcg_dropin_check")
+FORTRAN_RESIDENT = os.path.join(FORTRAN, "resident_check")
 
 
 def flang() -> str:
@@ -64,8 +65,10 @@ def flang() -> str:
 
 
 def build_fortran(verbose: bool = False) -> None:
-    """The ISO_C_BINDING interface module and the program that calls the
-    drop-in JCG through it (skipped, with a message, when no flang exists)."""
+    """The ISO_C_BINDING interface module and the programs that call the
+    library through it: the drop-in JCG, and the per-step path with the
+    bind(C) mesh and parameter types (skipped, with a message, when no flang
+    exists)."""
     fc = flang()
     if not fc:
         print("flang not found: Fortran interface check not built", file=sys.stderr)
@@ -78,6 +81,8 @@ def build_fortran(verbose: bool = False) -> None:
          "-o", iface_o],
         [fc, "-O2", os.path.join(FORTRAN, "jcg_dropin_check.F90"), iface_o, "-I", bdir,
          "-L", HERE, "-ladcgpu", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_CHECK],
+        [fc, "-O2", os.path.join(FORTRAN, "resident_check.F90"), iface_o, "-I", bdir,
+         "-L", HERE, "-ladcgpu", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_RESIDENT],
     ]
     for cmd in cmds:
         if verbose:

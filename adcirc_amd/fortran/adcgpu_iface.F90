@@ -17,7 +17,8 @@ module adcgpu_iface
   integer(c_int), parameter, public :: ADCGPU_FP64 = 0, ADCGPU_MIXED = 1
   integer(c_int), parameter, public :: F_ETA1 = 0, F_ETA2 = 1, F_ETAS = 2, F_UU1 = 3, &
        F_VV1 = 4, F_UU2 = 5, F_VV2 = 6, F_QX1 = 7, F_QY1 = 8, F_QX2 = 9, F_QY2 = 10, &
-       F_H1 = 11, F_H2 = 12, F_TK = 13, F_ETAMAX = 14, F_TOTALAREA = 15, F_UMAX = 23, &
+       F_H1 = 11, F_H2 = 12, F_TK = 13, F_ETAMAX = 14, F_TOTALAREA = 15, F_WSX1 = 16, &
+       F_WSY1 = 17, F_WSX2 = 18, F_WSY2 = 19, F_PR1 = 20, F_PR2 = 21, F_UMAX = 23, &
        F_NODECODE = 100, F_NOFF = 101, F_MJU = 102, F_ETAMAX_IT = 103, F_UMAX_IT = 104, F_COEF = 200
 
   type, bind(C), public :: adcgpu_mesh
