@@ -290,9 +290,6 @@ JcgArgs make_jcg_args(adcgpu_ctx* c, double* coef, const int* col, const int* nn
   // the learned second start, formed by the pass before (learned mode)
   a.u2 = u2;
   a.d2 = c->jd2;
-  // the two-start kernel runs unstaged: staging spills its registers further
-  // (measured 559 against 516 us per launch mid-storm at 1M nodes)
-  if (u2) a.hrows = -1;
   return a;
 }
 
@@ -315,8 +312,8 @@ int launch_jcg(adcgpu_ctx* c, double* coef, const int* col, const int* nneigh, d
     HIPCHK(hipEventRecord(c->jcg_events[c->ev_used].first, c->stream));
   }
   const void* fn =
-      c->jcg_mem ? (const void*)k_jcg_mem
-      : a.hrows >= 0 ? (const void*)k_jcg<false, true>
+      c->jcg_mem     ? (const void*)k_jcg_mem
+      : a.hrows >= 0 ? (u2 ? (const void*)k_jcg<true, true> : (const void*)k_jcg<false, true>)
                      : (u2 ? (const void*)k_jcg<true, false> : (const void*)k_jcg<false, false>);
   HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->jcg_grid), dim3(JCG_BLOCK), args,
                                     (unsigned)jcg_stage_bytes(a.hrows), c->stream));
@@ -1539,7 +1536,7 @@ int adcgpu_batch_run(adcgpu_batch* b, int it0, int nsteps, const double* const* 
       }
       for (size_t gi = 0; gi + 1 < b->gm.size(); ++gi) {
         // a group runs staged only if all its members can
-        bool st = !b->gdual[gi];
+        bool st = true;
         for (int q = b->gm[gi]; q < b->gm[gi + 1]; ++q) st = st && cur[q].hrows >= 0;
         if (!st)
           for (int q = b->gm[gi]; q < b->gm[gi + 1]; ++q) cur[q].hrows = -1;
@@ -1559,7 +1556,8 @@ int adcgpu_batch_run(adcgpu_batch* b, int it0, int nsteps, const double* const* 
           hmax = std::max(hmax, cur[q].hrows);
         }
         const bool st = cur[q0].hrows >= 0;
-        const void* fn = b->gdual[gi] ? (const void*)k_jcg_batch<true, false>
+        const void* fn = b->gdual[gi] ? (st ? (const void*)k_jcg_batch<true, true>
+                                            : (const void*)k_jcg_batch<true, false>)
                                       : (st ? (const void*)k_jcg_batch<false, true>
                                             : (const void*)k_jcg_batch<false, false>);
         const JcgArgs* pa = b->dargs + par * ns + q0;

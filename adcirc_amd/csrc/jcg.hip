@@ -1017,8 +1017,10 @@ __global__ __launch_bounds__(JCG_BLOCK) void k_jcg_mem(JcgArgs a) {
 template __global__ void k_jcg<false, false>(JcgArgs);
 template __global__ void k_jcg<true, false>(JcgArgs);
 template __global__ void k_jcg<false, true>(JcgArgs);
+template __global__ void k_jcg<true, true>(JcgArgs);
 template __global__ void k_jcg_batch<false, false>(const JcgArgs*, const int*, int);
 template __global__ void k_jcg_batch<true, false>(const JcgArgs*, const int*, int);
 template __global__ void k_jcg_batch<false, true>(const JcgArgs*, const int*, int);
+template __global__ void k_jcg_batch<true, true>(const JcgArgs*, const int*, int);
 
 }  // namespace adcgpu

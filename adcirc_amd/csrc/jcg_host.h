@@ -60,9 +60,10 @@ inline hipError_t jcg_resident_cap_of(const void* fn, int* per_cu_out) {
 // Every variant of the kernel that is launched must fit: one start or two,
 // and one start with the staged pseudo-residual; single and batched.
 inline hipError_t jcg_resident_cap(int device, int* cap) {
-  const void* fns[] = {(const void*)k_jcg<false, false>,       (const void*)k_jcg<true, false>,
-                       (const void*)k_jcg<false, true>,        (const void*)k_jcg_batch<false, false>,
-                       (const void*)k_jcg_batch<true, false>,  (const void*)k_jcg_batch<false, true>,
+  const void* fns[] = {(const void*)k_jcg<false, false>,      (const void*)k_jcg<true, false>,
+                       (const void*)k_jcg<false, true>,       (const void*)k_jcg<true, true>,
+                       (const void*)k_jcg_batch<false, false>, (const void*)k_jcg_batch<true, false>,
+                       (const void*)k_jcg_batch<false, true>, (const void*)k_jcg_batch<true, true>,
                        (const void*)k_jcg_mem};
   int per_cu = 1 << 30;
   hipError_t e = hipSuccess;

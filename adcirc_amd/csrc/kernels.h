@@ -297,7 +297,7 @@ __global__ void k_rhs_node(DevMesh m, DevState s, const CT* contrib, const doubl
                            const double* scal, JcgPre pre);
 // DUAL: start from the better (smaller initial pseudo-residual) of a.u and a.u2;
 // STAGED: the products read the pseudo-residual from LDS (a.hrows >= 0).
-// Instantiated: <false, false>, <true, false>, <false, true>.
+// Instantiated for all four combinations.
 template <bool DUAL, bool STAGED>
 __global__ void k_jcg(JcgArgs a);
 // nsys systems in one launch: blocks first[m] .. first[m+1]-1 solve args[m]
