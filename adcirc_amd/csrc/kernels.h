@@ -34,6 +34,7 @@ constexpr int WD_BLOCK = 1024;
 constexpr int WD_XMAX = 256;                 // four 64-bit mask words per candidate
 constexpr int WD_XW = WD_XMAX / 64;
 constexpr int WD_HASH = 2 * WD_XMAX;         // LDS hash slots for a block's external names
+constexpr int WD_INC_MAX = 12;               // incidence lists a wetting candidate scans at once
 constexpr int WD_CLOSE_LDS = 2 * WD_BLOCK * (8 + 8 * WD_XW + 4) + 2 * WD_HASH * 4;
 constexpr int WD_LINK_BLOCKS = 64;           // blocks whose link values fit the LDS of k_wd_link
 // serial fallback of k_wd_link: candidates staged through LDS in chunks, the

@@ -1027,21 +1027,53 @@ __global__ __launch_bounds__(256) void k_wd_pred(DevMesh m, DevState s, WdWork w
     const int l = (lc >> (2 * j)) & 3;
     const int x = m.nm[3 * e + l];
     const int pos = m.slot[3 * e + l];
+    const int p0 = m.inc_ptr[x], p1 = m.inc_ptr[x + 1];
     int pk = -1;
-    for (int r = pos - 1; r >= m.inc_ptr[x]; --r) {
-      const int code = m.inc[r], e2 = code / 3;
-      if (wd_touches(w, e2, code - 3 * e2)) {
-        pk = wd_index(w, e2, w.mask[e2 >> 6]);
-        break;
-      }
-    }
     bool last = true;
-    const int end = m.inc_ptr[x + 1];
-    for (int r = pos + 1; r < end; ++r) {
-      const int code = m.inc[r], e2 = code / 3;
-      if (wd_touches(w, e2, code - 3 * e2)) {
-        last = false;
-        break;
+    if (p1 - p0 <= WD_INC_MAX) {
+      // the node's whole list at once: its codes, then their mask words,
+      // then the candidates' vertex codes, each round of loads independent
+      int code[WD_INC_MAX];
+#pragma unroll
+      for (int q = 0; q < WD_INC_MAX; ++q) code[q] = p0 + q < p1 ? m.inc[p0 + q] : -1;
+      unsigned long long mw[WD_INC_MAX];
+#pragma unroll
+      for (int q = 0; q < WD_INC_MAX; ++q) mw[q] = code[q] >= 0 ? w.mask[(code[q] / 3) >> 6] : 0ull;
+      unsigned tch = 0;  // bit q: entry q's element is a candidate that updates node x
+      int lq[WD_INC_MAX];
+#pragma unroll
+      for (int q = 0; q < WD_INC_MAX; ++q) {
+        const int e2 = code[q] / 3;
+        const bool cand = code[q] >= 0 && ((mw[q] >> (e2 & 63)) & 1ull);
+        lq[q] = cand ? w.lc[e2] : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < WD_INC_MAX; ++q) {
+        const int k2 = code[q] - 3 * (code[q] / 3);
+        if (lq[q] >= 0 && (k2 == (lq[q] & 3) || k2 == ((lq[q] >> 2) & 3))) tch |= 1u << q;
+      }
+      const int me = pos - p0;
+      const unsigned before = tch & ((1u << me) - 1u), after = tch >> (me + 1);
+      if (before) {
+        const int q = 31 - __clz((int)before);  // the nearest earlier candidate
+        const int e2 = m.inc[p0 + q] / 3;
+        pk = wd_index(w, e2, w.mask[e2 >> 6]);
+      }
+      last = after == 0u;
+    } else {
+      for (int r = pos - 1; r >= p0; --r) {
+        const int code = m.inc[r], e2 = code / 3;
+        if (wd_touches(w, e2, code - 3 * e2)) {
+          pk = wd_index(w, e2, w.mask[e2 >> 6]);
+          break;
+        }
+      }
+      for (int r = pos + 1; r < p1; ++r) {
+        const int code = m.inc[r], e2 = code / 3;
+        if (wd_touches(w, e2, code - 3 * e2)) {
+          last = false;
+          break;
+        }
       }
     }
     if (pk < 0) v = fmax(v, s.tk[x]);
