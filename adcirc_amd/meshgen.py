@@ -64,6 +64,13 @@ class RunConfig:
     face: Tuple[float, ...] = (0.0,)      # degrees
     # met forcing: None, or the 13 Holland parameters of adco_holland
     holland: Optional[Tuple[float, ...]] = None
+    # tidal potential (NTIP = 1): per constituent (TPK, AMIGT rad/s, ETRF, FFT,
+    # FACET degrees), fort.15's NTIF block
+    tipot: Tuple[Tuple[float, float, float, float, float], ...] = ()
+    # centre of the CPP projection (degrees) that maps the Cartesian mesh to
+    # longitude/latitude (ICS = 1)
+    slam0: float = 0.0
+    sfea0: float = 0.0
 
     @property
     def nbfr(self) -> int:
@@ -72,6 +79,10 @@ class RunConfig:
     @property
     def nws(self) -> int:
         return 0 if self.holland is None else 8
+
+    @property
+    def ntip(self) -> int:
+        return 1 if self.tipot else 0
 
 
 @dataclasses.dataclass
@@ -138,6 +149,26 @@ def basin(seed: int = 1, nx: int = 101, ny: int = 51, lx: float = 100.0e3,
     emo = np.full((neta, 1), amp)
     efa = np.zeros((neta, 1))
     return Case(mesh, run, emo, efa)
+
+
+# fort.15 tidal potential lines (TPK, AMIGT rad/s, ETRF, FFT, FACET degrees) of
+# the four main constituents; FFT and FACET as for an arbitrary start date
+TIPOT_M2S2K1O1 = (
+    (0.242334, 0.000140518902509, 0.693, 0.9722, 212.5),
+    (0.112841, 0.000145444104333, 0.693, 1.0, 0.0),
+    (0.141565, 0.0000729211583579, 0.736, 1.0877, 351.8),
+    (0.100514, 0.0000675977441508, 0.695, 1.1513, 119.2),
+)
+
+
+def basin_tide(seed: int = 1) -> Case:
+    """The basin with the tidal potential of M2, S2, K1 and O1 besides its
+    boundary tide; the Cartesian mesh is placed at 75 W, 35 N for the
+    potential's longitude and latitude (CPP projection, ICS = 1)."""
+    case = basin(seed)
+    case.run.tipot = TIPOT_M2S2K1O1
+    case.run.slam0, case.run.sfea0 = -75.0, 35.0
+    return case
 
 
 def beach(seed: int = 3, nx: int = 41, ny: int = 21, lx: float = 8.0e3,

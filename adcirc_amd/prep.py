@@ -180,6 +180,22 @@ def flow_boundaries(x: np.ndarray, y: np.ndarray, land, costset: float):
             np.array(siii), np.array(me2gw, np.int32))
 
 
+REARTH = 6378206.4  # the reference's Rearth (src/constants.F90:79)
+
+
+def lonlat_cpp(x: np.ndarray, y: np.ndarray, slam0_deg: float, sfea0_deg: float):
+    """Longitude and latitude (radians) of Cartesian nodes by the inverse CPP
+    projection about (SLAM0, SFEA0) degrees, as the reference computes SLAM
+    and SFEA for ICS = 1 (src/mesh.F:1494-1512, INVCP src/mesh.F:3409-3420)."""
+    slam0 = slam0_deg * (3.141592653589793 / 180.0)
+    sfea0 = sfea0_deg * (3.141592653589793 / 180.0)
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    slam = slam0 + x / (REARTH * math.cos(sfea0))
+    sfea = y / REARTH
+    return slam, sfea
+
+
 def tau0_nodal(tau0: float, dp: np.ndarray) -> np.ndarray:
     """TAU0VAR from fort.15's TAU0 without a nodal-attribute file
     (src/nodalattr.F:1858-1895).  TAU0 >= 0 holds everywhere.  A negative

@@ -15,6 +15,7 @@
  *     GWCE LHS/RHS/BC   src/gwce.F:419-743, 1000-1862, 2000-2008
  *     wet/dry           src/wetdry.F:140-1470 (+ computeAlpha :1561-1652)
  *     momentum          src/momentum.F:298-1182
+ *     tidal potential   src/timestep.F:412-413, 841, 1507-1562 (NTIP = 1)
  *     peak surge        src/write_output.F:6357-6366
  *
  * A second reduction order (ADCO_ORDER_BLOCKED) replaces the reference's
@@ -110,6 +111,18 @@ int *adco_iarray(adco_model *m, const char *name); /* nodecode, noff, mju, etama
 int adco_numitr(const adco_model *m);
 int adco_ncchange(const adco_model *m);
 uint64_t adco_state_hash(const adco_model *m);
+
+/* Tidal potential (NTIP = 1): the constituents (FACET in radians), the nodes'
+ * longitude and latitude SLAM/SFEA in radians.  From the next step on, TIP2 is
+ * formed per step and enters the GWCE and momentum pressure terms
+ * (src/timestep.F:412-413, 841, 1507-1562; src/gwce.F:1181-1185;
+ * src/momentum.F:443-450).  Returns 0, or -1 for NTIF outside 1..ADCO_MAXTIF. */
+#define ADCO_MAXTIF 32
+int adco_set_tidepot(adco_model *m, int ntif, const double *tpk, const double *amigt,
+                     const double *etrf, const double *fft, const double *facet,
+                     const double *slam, const double *sfea);
+/* RampTip of the last step */
+double adco_ramptip(const adco_model *m);
 
 /* elevation boundary value for one boundary node at one time (gwce.F:1638-1651) */
 void adco_eta_bc(const adco_setup *s, double timeh, double rampelev, double *out);

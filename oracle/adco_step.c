@@ -46,6 +46,12 @@ struct adco_model {
   double habsmin, hoff;
   double dt, dto2, go2;
   double ramp, rampelev, rampmete;
+  /* tidal potential (NTIP = 1, src/timestep.F:1507-1562): constituents, the
+   * species factors L_N(0:2, I) (src/adcirc.F:295-308), longitudes, TIP1/2 */
+  int ctip, ntif;
+  double tpk[ADCO_MAXTIF], amigt[ADCO_MAXTIF], etrf[ADCO_MAXTIF], fft[ADCO_MAXTIF];
+  double facet[ADCO_MAXTIF], pert[ADCO_MAXTIF];
+  double *ln, *slam, *tip1, *tip2;
 };
 
 #define NM(ie, k) (m->s.nm[(size_t)(ie)*3 + (k)] - 1)
@@ -160,10 +166,66 @@ void adco_destroy(adco_model *m) {
                m->mom_lv_x, m->mom_lv_y, m->wsx1, m->wsy1, m->wsx2, m->wsy2, m->pr1,
                m->pr2, m->etamax, m->totalarea, m->tk_tmp, m->tkm, m->auv, m->nodecode,
                m->nnodecode, m->mju, m->etamax_it, m->umax_it, m->umax, m->nibcnt, m->noff, m->noffold, m->alphal, m->coef,
-               m->obccoef, m->wksp};
+               m->obccoef, m->wksp, m->ln, m->slam, m->tip1, m->tip2};
   for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
   free(m);
 }
+
+int adco_set_tidepot(adco_model *m, int ntif, const double *tpk, const double *amigt,
+                     const double *etrf, const double *fft, const double *facet,
+                     const double *slam, const double *sfea) {
+  if (ntif < 1 || ntif > ADCO_MAXTIF) return -1;
+  const int np = m->np;
+  m->ctip = 1;
+  m->ntif = ntif;
+  for (int j = 0; j < ntif; ++j) {
+    m->tpk[j] = tpk[j];
+    m->amigt[j] = amigt[j];
+    m->etrf[j] = etrf[j];
+    m->fft[j] = fft[j];
+    m->facet[j] = facet[j];
+    /* src/read_input.F:3349-3353 */
+    m->pert[j] = amigt[j] == 0.0 ? 0.0 : 2.0 * 3.141592653589793 / amigt[j];
+  }
+  if (!m->ln) {
+    m->ln = dalloc(3 * (size_t)np);
+    m->slam = dalloc(np);
+    m->tip1 = dalloc(np);
+    m->tip2 = dalloc(np);
+  }
+  for (int i = 0; i < np; ++i) {
+    /* src/adcirc.F:303-307 */
+    m->ln[i] = 1.5 * cos(sfea[i]) * cos(sfea[i]) - 1.0;
+    m->ln[np + i] = sin(2.0 * sfea[i]);
+    m->ln[2 * (size_t)np + i] = cos(sfea[i]) * cos(sfea[i]);
+    m->slam[i] = slam[i];
+    m->tip1[i] = 0.0;
+    m->tip2[i] = 0.0;
+  }
+  return 0;
+}
+
+/* applyTidePotentialForcing, the sum over constituents without the full
+ * formula and without self-attraction/loading (NTIP = 1; SALTAMP is zero,
+ * src/read_input.F:6460, so its term adds a zero to a sum that starts at +0
+ * and changes no bit) (src/timestep.F:1527-1557) */
+static void tide_potential(adco_model *m, double timeh, double ramptip) {
+  const int np = m->np;
+  for (int j = 0; j < m->ntif; ++j) {
+    const int ncyc = m->pert[j] == 0.0 ? 0 : (int)(timeh / m->pert[j]);
+    const double argt = m->amigt[j] * (timeh - ncyc * m->pert[j]) + m->facet[j];
+    const double tpmul = ramptip * m->etrf[j] * m->tpk[j] * m->fft[j];
+    int na = (int)lround(m->amigt[j] / 7e-5);
+    if (na > 2) na = 2;
+    const double *l = m->ln + (size_t)na * np;
+    for (int i = 0; i < np; ++i) {
+      const double argtp = argt + na * m->slam[i];
+      m->tip2[i] = m->tip2[i] + tpmul * l[i] * cos(argtp);
+    }
+  }
+}
+
+double adco_ramptip(const adco_model *m) { return m->ramp; }
 
 /* flang's NORM2: running maximum with scaled sum of squares */
 static double norm2_3(const double *v) {
@@ -357,6 +419,11 @@ static void gwce_rhs(adco_model *m) {
       PR1N3 = m->pr1[n3];
     }
     double Off1N1 = 0, Off1N2 = 0, Off1N3 = 0, TiPN1 = 0, TiPN2 = 0, TiPN3 = 0;
+    if (m->ctip) { /* src/gwce.F:1181-1185 */
+      TiPN1 = m->tip1[n1];
+      TiPN2 = m->tip1[n2];
+      TiPN3 = m->tip1[n3];
+    }
     const double *tkm = m->tkm;
     double BSXN1 = tkm[3 * n1 + 0] * QX1N1 + tkm[3 * n1 + 2] * QY1N1;
     double BSYN1 = tkm[3 * n1 + 2] * QX1N1 + tkm[3 * n1 + 1] * QY1N1;
@@ -764,6 +831,14 @@ static void momentum(adco_model *m) {
       B1N3 = B1N3 + m->pr1[n3];
       B2N3 = B2N3 + m->pr2[n3];
     }
+    if (m->ctip) { /* src/momentum.F:443-450 */
+      B1N1 = B1N1 - m->tip1[n1];
+      B2N1 = B2N1 - m->tip2[n1];
+      B1N2 = B1N2 - m->tip1[n2];
+      B2N2 = B2N2 - m->tip2[n2];
+      B1N3 = B1N3 - m->tip1[n3];
+      B2N3 = B2N3 - m->tip2[n3];
+    }
     double DBTPDX1A = B1N1 * FDX1 + B1N2 * FDX2 + B1N3 * FDX3;
     double DBTPDX2A = B2N1 * FDX1 + B2N2 * FDX2 + B2N3 * FDX3;
     double DBTPDY1A = B1N1 * FDY1 + B1N2 * FDY2 + B1N3 * FDY3;
@@ -914,7 +989,14 @@ int adco_step(adco_model *m, int it, const double *wsx2, const double *wsy2,
     } else {
       m->pr2[i] = m->s.prbckgrnd_mh2o;
     }
+    if (m->ctip) {
+      /* src/timestep.F:412-413 */
+      m->tip1[i] = m->tip2[i];
+      m->tip2[i] = 0.0;
+    }
   }
+  /* src/timestep.F:841; RampTip shares the elevation ramp's length here */
+  if (m->ctip) tide_potential(m, timeh, m->ramp);
   friction(m);
   if (m->s.nws != 0 && wsx2 && wsy2 && pr2) {
     memcpy(m->wsx2, wsx2, sizeof(double) * np);
@@ -956,7 +1038,8 @@ double *adco_array(adco_model *m, const char *name) {
       {"gwce_lv", m->gwce_lv}, {"h2", m->h2}, {"h1", m->h1}, {"alphal", m->alphal},
       {"uu1", m->uu1}, {"vv1", m->vv1}, {"wsx2", m->wsx2}, {"wsy2", m->wsy2},
       {"pr2", m->pr2}, {"wsx1", m->wsx1}, {"wsy1", m->wsy1}, {"pr1", m->pr1},
-      {"tkm", m->tkm}, {"mom_lv_x", m->mom_lv_x}, {"mom_lv_y", m->mom_lv_y}};
+      {"tkm", m->tkm}, {"mom_lv_x", m->mom_lv_x}, {"mom_lv_y", m->mom_lv_y},
+      {"tip1", m->tip1}, {"tip2", m->tip2}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); ++k)
     if (!strcmp(t[k].n, name)) return t[k].p;
   if (!strcmp(name, "ep")) return &m->ep;

@@ -43,6 +43,11 @@ def beach_lumped_golden():
 
 
 @pytest.fixture(scope="session")
+def basin_tide_golden():
+    return load_golden("basin_tide")
+
+
+@pytest.fixture(scope="session")
 def basin_golden():
     return load_golden("basin")
 

@@ -8,7 +8,7 @@ in the build container only; the fixtures it writes are what the tests use
 everywhere else.
 
     python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [basin_linear] [jcg] [tau0]
-                                       [coastal20k]
+                                       [coastal20k] [basin_tide]
 """
 from __future__ import annotations
 
@@ -208,6 +208,11 @@ def main(argv):
         r = case.run
         r.nolibf, r.cf, r.nolifa, r.nolica, r.nolicat = 0, 1.0e-4, 0, 0, 0
         print(run_case("basin_linear", case, nsteps=3600, early=10, dump_every=1200))
+    if "basin_tide" in todo:
+        # the basin with the tidal potential of M2, S2, K1, O1 (NTIP = 1) at
+        # 75 W, 35 N; one simulated hour
+        print(run_case("basin_tide", meshgen.basin_tide(), nsteps=1800, early=10,
+                       dump_every=600))
     if "jcg" in todo:
         print(make_jcg())
     if "coastal20k" in todo:

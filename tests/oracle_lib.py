@@ -67,6 +67,8 @@ def load() -> C.CDLL:
     lib.adco_state_hash.restype = C.c_uint64
     lib.adco_state_hash.argtypes = [C.c_void_p]
     lib.adco_holland.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
+    lib.adco_set_tidepot.restype = C.c_int
+    lib.adco_set_tidepot.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.adco_itpack_init.argtypes = [C.POINTER(Itpack), C.c_int]
     lib.adco_dfault.argtypes = [_ip, _dp]
     lib.adco_jcg.restype = C.c_int
@@ -134,6 +136,19 @@ class OracleModel:
         self.setup = s
         self.run = run
         self.h = self.lib.adco_create(C.byref(s), order)
+        tip = list(getattr(run, "tipot", ()))
+        if tip:
+            from adcirc_amd.prep import lonlat_cpp
+            slam, sfea = lonlat_cpp(d["x"], d["y"], run.slam0, run.sfea0)
+            cols = [np.array([c[k] for c in tip], np.float64) for k in range(5)]
+            cols[4] = cols[4] * (3.141592653589793 / 180.0)  # FACET in radians
+            for k, name in enumerate(("tpk", "amigt", "etrf", "fft", "facet")):
+                self.keep["tip_" + name] = cols[k]
+            self.keep["slam"], self.keep["sfea"] = slam, sfea
+            rc = self.lib.adco_set_tidepot(self.h, len(tip),
+                                           *[_p(c, _dp) for c in cols],
+                                           _p(slam, _dp), _p(sfea, _dp))
+            assert rc == 0, "adco_set_tidepot"
         self.hol = None
         if run.holland is not None:
             self.hol = np.array(list(run.holland) + [0.0] * (16 - len(run.holland)))

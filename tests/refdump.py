@@ -47,7 +47,14 @@ def write_inputs(case, workdir: str, nsteps: int, dump_every: int = 0,
         "amig_in": ",".join(fl(v) for v in r.amig), "ff_in": ",".join(fl(v) for v in r.ff),
         "face_in": ",".join(fl(v) for v in r.face), "hprm": ",".join(fl(v) for v in hol),
         "ilump_in": str(r.ilump), "hash_steps": "1" if hash_steps else "0",
+        "slam0_in": fl(getattr(r, "slam0", 0.0)), "sfea0_in": fl(getattr(r, "sfea0", 0.0)),
     }
+    tip = list(getattr(r, "tipot", ()))
+    if tip:
+        nml["ntip_in"] = "1"
+        nml["ntif_in"] = str(len(tip))
+        for k, name in enumerate(("tpk_in", "amigt_in", "etrf_in", "fft_in", "facet_in")):
+            nml[name] = ",".join(fl(c[k]) for c in tip)
     with open(os.path.join(workdir, "run.nml"), "w") as f:
         f.write("&run\n")
         for k, v in nml.items():

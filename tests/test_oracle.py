@@ -274,3 +274,36 @@ def test_peak_surge_and_speed_tracking(beach_golden):
     assert np.array_equal(m.iarray("etamax_it"), eit)
     assert np.array_equal(m.iarray("umax_it"), uit)
     assert (uit > 0).any() and (eit > 0).any()
+
+
+def test_basin_tidal_potential_matches_reference(basin_tide_golden):
+    """The basin with the tidal potential of M2, S2, K1 and O1 (NTIP = 1,
+    src/timestep.F:1507-1562; TIP1/TIP2 in src/gwce.F:1181-1185 and
+    src/momentum.F:443-450), one hour: bit-identical every step."""
+    d = basin_tide_golden
+    _run(d, meshgen.basin_tide().run, int(d["run_nsteps"]))
+
+
+def test_tidal_potential_closed_form(basin_tide_golden):
+    """TIP2 against the equilibrium tide written out independently: for each
+    constituent the species factor from the latitude (long-period
+    1.5 cos^2 - 1, diurnal sin 2phi, semidiurnal cos^2) times
+    ramp ETRF TPK FFT cos(omega t + FACET + n lambda), the species n the
+    frequency's nearest multiple of one cycle per day."""
+    from adcirc_amd.prep import lonlat_cpp
+    d = basin_tide_golden
+    run = meshgen.basin_tide().run
+    m = oracle_lib.OracleModel(d, run, oracle_lib.SEQUENTIAL)
+    lam, phi = lonlat_cpp(d["x"], d["y"], run.slam0, run.sfea0)
+    for it in range(1, 4):
+        m.step(it)
+    t = 3 * run.dtdp + (run.statim - run.reftim) * 86400.0
+    ramp = math.tanh((2.0 * (3 * run.dtdp) / 86400.0) / run.drampelev)
+    want = np.zeros_like(lam)
+    for tpk, om, etrf, fft, facet in run.tipot:
+        n = 2 if om > 1.3e-4 else (1 if om > 5e-5 else 0)
+        fac = {0: 1.5 * np.cos(phi) ** 2 - 1.0, 1: np.sin(2.0 * phi), 2: np.cos(phi) ** 2}[n]
+        want += ramp * etrf * tpk * fft * fac * np.cos(om * t + math.radians(facet) + n * lam)
+    got = m.array("tip2")
+    assert np.abs(got - want).max() <= 1e-15 * max(1.0, np.abs(want).max() * 1e3)
+    assert np.abs(want).max() > 0.0
