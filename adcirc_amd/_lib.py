@@ -17,7 +17,8 @@ ip_ = C.POINTER(C.c_int32)
 FP64, MIXED = 0, 1
 FIELDS = dict(eta1=0, eta2=1, etas=2, uu1=3, vv1=4, uu2=5, vv2=6, qx1=7, qy1=8, qx2=9, qy2=10,
               h1=11, h2=12, tk=13, etamax=14, totalarea=15, wsx1=16, wsy1=17, wsx2=18,
-              wsy2=19, pr1=20, pr2=21, gwce_lv=22, umax=23, etas_prev=24, nodecode=100, noff=101,
+              wsy2=19, pr1=20, pr2=21, gwce_lv=22, umax=23, etas_prev=24, tip1=25, tip2=26,
+              nodecode=100, noff=101,
               mju=102, etamax_it=103, umax_it=104,
               coef=200, alphal=300)
 INT_FIELDS = ("nodecode", "noff", "mju", "etamax_it", "umax_it")
@@ -33,7 +34,7 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_ml_collect", "adcgpu_ml_samples", "adcgpu_ml_train", "adcgpu_ml_grad",
            "adcgpu_ml_features", "adcgpu_ml_correction", "adcgpu_state_bytes",
            "adcgpu_state_save", "adcgpu_state_load", "adcgpu_last_solve",
-           "adcgpu_set_jcg_blocks", "adcgpu_batch_create", "adcgpu_batch_run", "adcgpu_batch_destroy")
+           "adcgpu_set_jcg_blocks", "adcgpu_set_tide_potential", "adcgpu_batch_create", "adcgpu_batch_run", "adcgpu_batch_destroy")
 ML_NPARAM = 22913
 # error codes (ADCGPU_E_*)
 E_ARG, E_UNSUPPORTED, E_HIP, E_DIAG, E_CAPACITY, E_NONFINITE, E_ERRORELEV = range(-1, -8, -1)
@@ -102,6 +103,8 @@ def load() -> C.CDLL:
     lib.adcgpu_run_stats.argtypes = [vp, C.POINTER(C.c_int64)]
     lib.adcgpu_last_solve.argtypes = [vp, dp_]
     lib.adcgpu_set_jcg_blocks.argtypes = [vp, C.c_int]
+    lib.adcgpu_set_tide_potential.argtypes = [vp, C.c_int, dp_, dp_, dp_, dp_, dp_, dp_, dp_,
+                                              C.c_int, C.c_double, C.c_double, C.c_double]
     lib.adcgpu_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     lib.adcgpu_batch_run.argtypes = [vp, C.c_int, C.c_int, C.POINTER(dp_), C.POINTER(dp_)]
     lib.adcgpu_batch_destroy.argtypes = [vp]

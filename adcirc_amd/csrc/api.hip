@@ -102,6 +102,13 @@ struct adcgpu_ctx {
   bool jcg_mem = false;         // more row tiles than blocks: k_jcg_mem
   double *juo = nullptr, *jdpv = nullptr, *jndv = nullptr;  // k_jcg_mem's iterates
   int jmem_n = 0;               // rows those hold
+  // tidal potential (NTIP = 1): constituents, ramp and time origin, the
+  // species factors L_N [3][np] and longitudes on the device, TIP1/TIP2
+  bool ctip = false;
+  std::vector<double> tip_tpk, tip_amigt, tip_etrf, tip_fft, tip_facet, tip_pert;
+  int tip_nramp = 0;
+  double tip_dramp = 0.0, tip_statim = 0.0, tip_reftim = 0.0;
+  double *tip1 = nullptr, *tip2 = nullptr, *tip_ln = nullptr, *tip_slam = nullptr;
   int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // the GWCE matrix's unchanging terms per incidence slot, and each element's
@@ -167,6 +174,8 @@ struct adcgpu_ctx {
     s.alphal = alphal; s.coef = coef; s.obc = obc; s.fric = fric; s.nodecode = nodecode;
     s.nnodecode = nnodecode; s.noff = noff; s.noffold = noffold; s.mju = mju;
     s.etamax_it = etamax_it; s.umax = umax; s.umax_it = umax_it; s.ncele = ncele;
+    s.tip1 = ctip ? tip1 : nullptr;
+    s.tip2 = ctip ? tip2 : nullptr;
     return s;
   }
 };
@@ -888,6 +897,8 @@ static double** field_ptr(adcgpu_ctx* c, int f) {
     case ADCGPU_PR2: return &c->pr2;
     case ADCGPU_GWCE_LV: return &c->gwce_lv;
     case ADCGPU_ETAS_PREV: return &c->etas_prev;
+    case ADCGPU_TIP1: return c->ctip ? &c->tip1 : nullptr;
+    case ADCGPU_TIP2: return c->ctip ? &c->tip2 : nullptr;
     case ADCGPU_UMAX: return &c->umax;
     case ADCGPU_COEF: return &c->coef;
     case ADCGPU_ALPHAL: return &c->alphal;
@@ -1006,6 +1017,29 @@ int adcgpu_begin_step(adcgpu_ctx* c, int it) {
   std::swap(c->uu1, c->uu2); std::swap(c->vv1, c->vv2);
   if (c->prm.nws != 0) {
     std::swap(c->wsx1, c->wsx2); std::swap(c->wsy1, c->wsy2); std::swap(c->pr1, c->pr2);
+  }
+  if (c->ctip) {
+    // TIP1 <- TIP2 (src/timestep.F:412-413), then this step's TIP2
+    // (src/timestep.F:841, 1507-1562): the per-constituent phases and
+    // multipliers on the host, the nodal sum on the device
+    std::swap(c->tip1, c->tip2);
+    const double timeloc = it * c->prm.dtdp + c->tip_statim * 86400.0;
+    const double timeh = it * c->prm.dtdp + (c->tip_statim - c->tip_reftim) * 86400.0;
+    double ramptip = 1.0;
+    if (c->tip_nramp != 0) ramptip = std::tanh((2.0 * timeloc / 86400.0) / c->tip_dramp);
+    if (c->tip_dramp < 1.0e-6) ramptip = 1.0;
+    TipPrm tp;
+    std::memset(&tp, 0, sizeof(tp));
+    tp.ntif = (int)c->tip_tpk.size();
+    for (int j = 0; j < tp.ntif; ++j) {
+      const double pert = c->tip_pert[j];
+      const int ncyc = pert == 0.0 ? 0 : (int)(timeh / pert);
+      tp.argt[j] = c->tip_amigt[j] * (timeh - ncyc * pert) + c->tip_facet[j];
+      tp.tpmul[j] = ramptip * c->tip_etrf[j] * c->tip_tpk[j] * c->tip_fft[j];
+      tp.na[j] = std::min((int)std::lround(c->tip_amigt[j] / 7e-5), 2);
+    }
+    hipLaunchKernelGGL(k_tide_pot, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, tp,
+                       (const double*)c->tip_ln, (const double*)c->tip_slam, c->tip2);
   }
   return check_launch("begin_step");
 }
@@ -1626,6 +1660,58 @@ int adcgpu_set_jcg_blocks(adcgpu_ctx* c, int blocks) {
   return 0;
 }
 
+int adcgpu_set_tide_potential(adcgpu_ctx* c, int ntif, const double* tpk, const double* amigt,
+                              const double* etrf, const double* fft, const double* facet,
+                              const double* slam, const double* sfea, int nramp,
+                              double dramptip, double statim, double reftim) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  if (ntif < 0 || ntif > TIP_MAX) return fail(ADCGPU_E_ARG, "NTIF out of range");
+  if (ntif > 0 && (!tpk || !amigt || !etrf || !fft || !facet || !slam || !sfea))
+    return fail(ADCGPU_E_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (ntif == 0) {
+    c->ctip = false;
+    return 0;
+  }
+  const int np = c->np;
+  c->tip_tpk.assign(tpk, tpk + ntif);
+  c->tip_amigt.assign(amigt, amigt + ntif);
+  c->tip_etrf.assign(etrf, etrf + ntif);
+  c->tip_fft.assign(fft, fft + ntif);
+  c->tip_facet.assign(facet, facet + ntif);
+  c->tip_pert.resize(ntif);
+  for (int j = 0; j < ntif; ++j)  // src/read_input.F:3349-3353
+    c->tip_pert[j] = amigt[j] == 0.0 ? 0.0 : 2.0 * 3.141592653589793 / amigt[j];
+  c->tip_nramp = nramp;
+  c->tip_dramp = dramptip;
+  c->tip_statim = statim;
+  c->tip_reftim = reftim;
+  // the species factors (src/adcirc.F:303-307), on the host with the C
+  // library's cos and sin like the reference
+  std::vector<double> ln(3 * (size_t)np);
+  for (int i = 0; i < np; ++i) {
+    ln[i] = 1.5 * std::cos(sfea[i]) * std::cos(sfea[i]) - 1.0;
+    ln[np + i] = std::sin(2.0 * sfea[i]);
+    ln[2 * (size_t)np + i] = std::cos(sfea[i]) * std::cos(sfea[i]);
+  }
+  if (!c->tip1) {
+    for (double** p : {&c->tip1, &c->tip2, &c->tip_slam}) {
+      if (int rc = dmalloc(p, (size_t)np)) return rc;
+      c->allocs.push_back((void*)*p);
+    }
+    if (int rc = dmalloc(&c->tip_ln, 3 * (size_t)np)) return rc;
+    c->allocs.push_back((void*)c->tip_ln);
+  }
+  HIPCHK(hipMemcpy(c->tip_ln, ln.data(), ln.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->tip_slam, slam, (size_t)np * 8, hipMemcpyHostToDevice));
+  // the cold start's TIP2 (src/cstart.F:358-361)
+  HIPCHK(hipMemset(c->tip1, 0, (size_t)np * 8));
+  HIPCHK(hipMemset(c->tip2, 0, (size_t)np * 8));
+  c->ctip = true;
+  return 0;
+}
+
 // ---- learned start
 int adcgpu_ml_params(adcgpu_ctx* c, const float* in, float* out) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
@@ -1783,6 +1869,10 @@ std::vector<StatePart> state_parts(adcgpu_ctx* c) {
   v.push_back({c->scal, S_COUNT * 8});
   v.push_back({c->jsaved, 12 * 8});
   v.push_back({c->flags, F_COUNT * 4});
+  if (c->ctip) {
+    v.push_back({c->tip1, np * 8});
+    v.push_back({c->tip2, np * 8});
+  }
   return v;
 }
 }  // namespace

@@ -105,6 +105,17 @@ struct DevState {
   double *umax;    // peak depth-averaged speed at wet nodes
   int *umax_it;    // step of the last UMAX rise (0: never)
   unsigned char* ncele;  // [ne] NCEle of each element for the next GWCE matrix
+  double *tip1, *tip2;   // tidal potential at time levels s and s+1 (null: none)
+};
+
+// The tidal potential of one step (src/timestep.F:1527-1557, NTIP = 1), per
+// constituent: its species, ARGT (the phase at the step, reduced by whole
+// periods) and TPMUL = RampTip*ETRF*TPK*FFT, formed on the host.
+constexpr int TIP_MAX = 32;
+struct TipPrm {
+  int ntif;
+  int na[TIP_MAX];
+  double argt[TIP_MAX], tpmul[TIP_MAX];
 };
 
 // The GWCE matrix terms of every (element, vertex) slot that do not change
@@ -237,6 +248,9 @@ struct JcgArgs {
 
 __global__ void k_init_h(int np, const double* dp, const double* eta2, double ifnlfa, double* h1,
                          double* h2);
+// TIP2 = sum over constituents of TPMUL*L_N(NA, i)*cos(ARGT + NA*SLAM(i)); ln is
+// [3][np] (long-period, diurnal, semidiurnal species)
+__global__ void k_tide_pot(int np, TipPrm p, const double* ln, const double* slam, double* tip2);
 __global__ void k_friction(int np, FricPrm f, double* tk);
 __global__ void k_holland(int np, const double* x, const double* y, HollandPrm p, double* wsx,
                           double* wsy, double* pr);

@@ -27,6 +27,23 @@ __global__ __launch_bounds__(256) void k_init_h(int np, const double* __restrict
   h1[i] = h;
 }
 
+// applyTidePotentialForcing's constituent sum (src/timestep.F:1527-1557): TIP2
+// starts at zero and takes each constituent's term in order; the
+// self-attraction term of NTIP = 2 is absent (its amplitudes are zero for
+// NTIP = 1, src/read_input.F:6460, and adding a zero to this sum changes no bit).
+__global__ __launch_bounds__(256) void k_tide_pot(int np, TipPrm p, const double* __restrict__ ln,
+                                                  const double* __restrict__ slam,
+                                                  double* __restrict__ tip2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double v = 0.0;
+  for (int j = 0; j < p.ntif; ++j) {
+    const double argtp = p.argt[j] + p.na[j] * slam[i];
+    v = v + p.tpmul[j] * ln[(size_t)p.na[j] * np + i] * cos(argtp);
+  }
+  tip2[i] = v;
+}
+
 // Apply2DBottomFriction (src/nodalattr.F:2400-2411) for IFHYBF = 0, and the
 // TKM = (TK, TK, 0) of Apply2DInternalWaveDrag without wave drag; TKM is
 // represented by TK alone.
@@ -546,6 +563,11 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
       WSXAvg = (1.0 * s.wsx1[n1] + 1.0 * s.wsx1[n2] + 1.0 * s.wsx1[n3]) / 3.0;
       WSYAvg = (1.0 * s.wsy1[n1] + 1.0 * s.wsy1[n2] + 1.0 * s.wsy1[n3]) / 3.0;
     }
+    // tidal potential at time level s (src/gwce.F:1181-1185); PR - 0 is PR
+    double TiPN1 = 0.0, TiPN2 = 0.0, TiPN3 = 0.0;
+    if (s.tip1) {
+      TiPN1 = s.tip1[n1]; TiPN2 = s.tip1[n2]; TiPN3 = s.tip1[n3];
+    }
     // TKM(1)=TKM(2)=TK, TKM(3)=0
     const double tk1 = s.tk[n1], tk2 = s.tk[n2], tk3 = s.tk[n3];
     const double BSXN1 = tk1 * QX1N1 + 0.0 * QY1N1, BSYN1 = 0.0 * QX1N1 + tk1 * QY1N1;
@@ -593,11 +615,13 @@ __global__ __launch_bounds__(256) void k_rhs_elem(DevMesh m, DevState s, RhsPrm 
     const double A00pB00 = A00 + B00;
     double JXAvg = CorifAvg * QY1Avg -
                    q.ifnlfa * GOAreaIE4 * 1.0 * (E1N1SQ * FDX1 + E1N2SQ * FDX2 + E1N3SQ * FDX3) -
-                   GHAvgOAreaIE2 * (PR1N1 * FDX1 + PR1N2 * FDX2 + PR1N3 * FDX3) +
+                   GHAvgOAreaIE2 * ((PR1N1 - TiPN1) * FDX1 + (PR1N2 - TiPN2) * FDX2 +
+                                    (PR1N3 - TiPN3) * FDX3) +
                    WSXAvg - BSXAvg + MX + Tau0QXAvg;
     double JYAvg = -CorifAvg * QX1Avg -
                    q.ifnlfa * GOAreaIE4 * 1.0 * (E1N1SQ * FDY1 + E1N2SQ * FDY2 + E1N3SQ * FDY3) -
-                   GHAvgOAreaIE2 * (PR1N1 * FDY1 + PR1N2 * FDY2 + PR1N3 * FDY3) +
+                   GHAvgOAreaIE2 * ((PR1N1 - TiPN1) * FDY1 + (PR1N2 - TiPN2) * FDY2 +
+                                    (PR1N3 - TiPN3) * FDY3) +
                    WSYAvg - BSYAvg + MY + Tau0QYAvg;
     JXAvg = JXAvg -
             q.ifnlct * (QX1Avg * (U1N1 * FDX1 + U1N2 * FDX2 + U1N3 * FDX3) +
@@ -686,10 +710,19 @@ __global__ __launch_bounds__(256) void k_rhs_elem32(DevMesh m, DevState s, RhsPr
     const double eva = m.evm[n1], evb = m.evm[n2], evc = m.evm[n3];
     const double t0a = m.tau0[n1], t0b = m.tau0[n2], t0c = m.tau0[n3];
     float PR1X = 0.0f, PR1Y = 0.0f, WSXAvg = 0.0f, WSYAvg = 0.0f;
-    if (q.nws) {
-      const double pa = s.pr1[n1], pb = s.pr1[n2], pc = s.pr1[n3];
+    if (q.nws || s.tip1) {
+      // the pressure head less the tidal potential, in FP64 before the gradient
+      double pa = 0.0, pb = 0.0, pc = 0.0;
+      if (q.nws) {
+        pa = s.pr1[n1]; pb = s.pr1[n2]; pc = s.pr1[n3];
+      }
+      if (s.tip1) {
+        pa = pa - s.tip1[n1]; pb = pb - s.tip1[n2]; pc = pc - s.tip1[n3];
+      }
       PR1X = grad32(pa, pb, pc, FDX2, FDX3);
       PR1Y = grad32(pa, pb, pc, FDY2, FDY3);
+    }
+    if (q.nws) {
       WSXAvg = ((float)s.wsx1[n1] + (float)s.wsx1[n2] + (float)s.wsx1[n3]) / 3.0f;
       WSYAvg = ((float)s.wsy1[n1] + (float)s.wsy1[n2] + (float)s.wsy1[n3]) / 3.0f;
     }
@@ -1510,6 +1543,11 @@ __global__ __launch_bounds__(256) void k_mom_elem(DevMesh m, DevState s, MomPrm 
       B1N2 = B1N2 + s.pr1[n2]; B2N2 = B2N2 + s.pr2[n2];
       B1N3 = B1N3 + s.pr1[n3]; B2N3 = B2N3 + s.pr2[n3];
     }
+    if (s.tip1) {  // src/momentum.F:443-450
+      B1N1 = B1N1 - s.tip1[n1]; B2N1 = B2N1 - s.tip2[n1];
+      B1N2 = B1N2 - s.tip1[n2]; B2N2 = B2N2 - s.tip2[n2];
+      B1N3 = B1N3 - s.tip1[n3]; B2N3 = B2N3 - s.tip2[n3];
+    }
     const double DBTPDX1A = B1N1 * FDX1 + B1N2 * FDX2 + B1N3 * FDX3;
     const double DBTPDX2A = B2N1 * FDX1 + B2N2 * FDX2 + B2N3 * FDX3;
     const double DBTPDY1A = B1N1 * FDY1 + B1N2 * FDY2 + B1N3 * FDY3;
@@ -1581,6 +1619,11 @@ __global__ __launch_bounds__(256) void k_mom_elem32(DevMesh m, DevState s, MomPr
       b1a = b1a + s.pr1[n1]; b2a = b2a + s.pr2[n1];
       b1b = b1b + s.pr1[n2]; b2b = b2b + s.pr2[n2];
       b1c = b1c + s.pr1[n3]; b2c = b2c + s.pr2[n3];
+    }
+    if (s.tip1) {
+      b1a = b1a - s.tip1[n1]; b2a = b2a - s.tip2[n1];
+      b1b = b1b - s.tip1[n2]; b2b = b2b - s.tip2[n2];
+      b1c = b1c - s.tip1[n3]; b2c = b2c - s.tip2[n3];
     }
     const float DBTPDXA = 0.5f * (grad32(b1a, b1b, b1c, FDX2, FDX3) +
                                   grad32(b2a, b2b, b2c, FDX2, FDX3));

@@ -177,6 +177,10 @@ class Model:
             self.emo = np.zeros((self.neta, self.nbfr))
             self.efa = np.zeros((self.neta, self.nbfr))
         self._bc = np.zeros(max(self.neta, 1))
+        if getattr(run, "tipot", ()):
+            from .prep import lonlat_cpp
+            slam, sfea = lonlat_cpp(t["x"], t["y"], run.slam0, run.sfea0)
+            self.set_tide_potential(run.tipot, slam, sfea, run)
 
     @staticmethod
     def radians(deg: np.ndarray) -> np.ndarray:
@@ -234,9 +238,39 @@ class Model:
             self.eta_bc(it0 + s, tab[s])
         return tab
 
+    def set_tide_potential(self, constituents, slam, sfea, run) -> None:
+        """Tidal potential (NTIP = 1): constituents as fort.15's (TPK, AMIGT,
+        ETRF, FFT, FACET degrees), node longitudes/latitudes in radians; the
+        ramp and time origin from the run."""
+        c = [np.array([x[k] for x in constituents], np.float64) for k in range(5)]
+        c[4] = c[4] * DEG2RAD
+        sl = np.ascontiguousarray(slam, np.float64)
+        sf = np.ascontiguousarray(sfea, np.float64)
+        check(self.lib.adcgpu_set_tide_potential(
+            self.h, len(constituents), *[_p(a, dp_) for a in c], _p(sl, dp_), _p(sf, dp_),
+            int(run.nramp), float(run.drampelev), float(run.statim), float(run.reftim)),
+            "set_tide_potential")
+
     # ------------------------------------------------------------- stepping
-    def timestep(self, it: int, forcing: Optional[Sequence[np.ndarray]] = None) -> None:
+    def timestep(self, it: int, forcing: Optional[Sequence[np.ndarray]] = None,
+                 tip2: Optional[np.ndarray] = None) -> None:
+        """One step; `tip2` replaces the step's device-formed tidal potential
+        with the caller's values (the per-step calls of adcgpu_timestep with
+        ADCGPU_TIP2 set after adcgpu_begin_step)."""
         bc = self.eta_bc(it, self._bc)
+        if tip2 is not None:
+            lib = self.lib
+            check(lib.adcgpu_begin_step(self.h, it), "begin_step")
+            self.set("tip2", tip2)
+            check(lib.adcgpu_friction(self.h), "friction")
+            if forcing is not None and self.run.nws != 0:
+                w, v, p = (np.ascontiguousarray(a, np.float64) for a in forcing)
+                check(lib.adcgpu_set_forcing(self.h, _p(w, dp_), _p(v, dp_), _p(p, dp_)),
+                      "set_forcing")
+            check(lib.adcgpu_solve_gwce(self.h, it, _p(bc, dp_)), "solve_gwce")
+            check(lib.adcgpu_wetdry(self.h, it), "wetdry")
+            check(lib.adcgpu_momentum(self.h), "momentum")
+            return
         if forcing is not None:
             w, v, p = (np.ascontiguousarray(a, np.float64) for a in forcing)
             rc = self.lib.adcgpu_timestep(self.h, it, _p(bc, dp_), _p(w, dp_), _p(v, dp_),

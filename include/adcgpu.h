@@ -104,6 +104,8 @@ enum adcgpu_field {
                        (src/write_output.F:6367-6376) */
   ADCGPU_ETAS_PREV, /* the increment before ETAS (kept when warm_start != 0 or
                        the learned start is set up) */
+  ADCGPU_TIP1,      /* tidal potential at time levels s and s+1 (only after */
+  ADCGPU_TIP2,      /* adcgpu_set_tide_potential; TIP1, TIP2 in src/global.F) */
   ADCGPU_NODECODE = 100, ADCGPU_NOFF, ADCGPU_MJU,
   ADCGPU_ETAMAX_IT, /* step IT of each node's last ETAMAX rise, 0 if none: the
                        reference's ETAMAX_Time is IT*DTDP + STATIM*86400
@@ -207,6 +209,22 @@ int adcgpu_last_solve(adcgpu_ctx *ctx, double *out4);
  * counterpart in the reference: for sharing the device and for tests of the
  * memory-resident solve. */
 int adcgpu_set_jcg_blocks(adcgpu_ctx *ctx, int blocks);
+
+/* Tidal potential forcing, NTIP = 1 (replaces applyTidePotentialForcing,
+ * src/timestep.F:1507-1562, with CTIP's terms in src/gwce.F:1181-1185 and
+ * src/momentum.F:443-450).  ntif constituents as fort.15 gives them (TPK,
+ * AMIGT rad/s, ETRF, FFT) with FACET in radians; slam/sfea: every node's
+ * longitude and latitude in radians (SLAM, SFEA); the ramp as NRAMP and
+ * DRampTip (days), and STATIM/REFTIM (days) for the step's times.  From the
+ * next adcgpu_begin_step on, each step moves TIP2 to TIP1 and forms TIP2 on
+ * the device (the C library's cos and the device's differ in the last bit of
+ * some arguments; a caller that needs the host's values overwrites
+ * ADCGPU_TIP2 after adcgpu_begin_step).  ntif = 0 turns the forcing off.
+ * Keeps the current TIP1/TIP2 at zero, as a cold start does. */
+int adcgpu_set_tide_potential(adcgpu_ctx *ctx, int ntif, const double *tpk, const double *amigt,
+                              const double *etrf, const double *fft, const double *facet,
+                              const double *slam, const double *sfea, int nramp,
+                              double dramptip, double statim, double reftim);
 
 /* ---- batches: several contexts stepped together (ensembles, uneven meshes)
  * No counterpart in the reference (an ensemble there is separate runs of
