@@ -12,7 +12,8 @@ and tag lines (RUNDES, RUNID, BOUNTAG, ALPHA, TIPOTAG, NAMEFR) are read whole.
 Options outside the subset raise ``Fort15Error`` naming the option and the
 line: IM other than 0, 111111 (the same defaults spelled out) or 111112
 (lumped GWCE); spherical coordinates (ICS = 2) and NCOR = 1; nodal attributes
-(NWP > 0); tidal potential (NTIP != 0); met forcing other than NWS = 0 or the
+(NWP > 0); the tidal potential with self-attraction (NTIP = 2, fort.24); a
+tidal-potential ramp other than the elevation ramp; met forcing other than NWS = 0 or the
 parametric vortex NWS = 8; TAU0 schemes other than a constant, -1 and -2;
 hybrid friction (NOLIBF 2, 3); NRS/ice variants of NWS; flux boundaries
 (NFFR); station lists read from a separate file (NSTA < 0).
@@ -203,8 +204,8 @@ def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
     if ncor != 0:
         r.fail("NCOR = 1 needs spherical coordinates, which are not supported")
     ntip = r.int1("NTIP")
-    if ntip != 0:
-        r.fail(f"NTIP = {ntip}: tidal potential forcing is not supported")
+    if ntip not in (0, 1):
+        r.fail(f"NTIP = {ntip}: supported are 0 and 1 (no self-attraction/loading)")
     nws = r.int1("NWS")
     if nws not in (0, 8, -8):
         r.fail(f"NWS = {nws}: supported are 0 and the parametric vortex 8")
@@ -226,6 +227,8 @@ def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
     # NRAMP 0, 1: one value; 2: DRAMP DRAMPExtFlux FluxSettlingTime; 3 adds
     # DRAMPIntFlux; 4 DRAMPElev; ... (src/read_input.F:2893-2950)
     drampelev = ramps[4] if nramp >= 4 else ramps[0]
+    if ntip and nramp >= 5 and ramps[5] != drampelev:
+        r.fail("DRAMPTip differs from DRAMPElev: one ramp length is supported")
     a00, b00, c00 = r.floats(3, "A00 B00 C00")
     if nolifa in (2, 3):
         v = r.values(4, "H0 NODEDRYMIN NODEWETMIN VELMIN")
@@ -244,9 +247,13 @@ def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
         r.fail("ESLM < 0 (Smagorinsky viscosity) is not supported")
     cori = r.float1("CORI")
     ntif = r.int1("NTIF")
+    tipot = []
     for _ in range(ntif):
-        r.raw()
-        r.floats(5, "TPK AMIGT ETRF FFT FACET")
+        r.raw()  # TIPOTAG
+        tipot.append(tuple(r.floats(5, "TPK AMIGT ETRF FFT FACET")))
+    # src/read_input.F:4818-4825
+    if (ntip == 0) != (ntif == 0):
+        r.fail(f"NTIP = {ntip} with NTIF = {ntif}: both zero or both non-zero")
     nbfr = r.int1("NBFR")
     tags, amig, ff, face = [], [], [], []
     for _ in range(nbfr):
@@ -289,7 +296,8 @@ def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
                     nolica=nolica, nolicat=nolicat, nolibf=nolibf, cf=cf, tau0=tau0, eslm=eslm,
                     cori=cori, h0=h0, velmin=velmin, a00=a00, b00=b00, c00=c00, itmax=itmax,
                     convcr=convcr, ilump=ilump, anginn=anginn, g=g, amig=tuple(amig),
-                    ff=tuple(ff), face=tuple(face), holland=None)
+                    ff=tuple(ff), face=tuple(face), holland=None, tipot=tuple(tipot),
+                    slam0=slam0, sfea0=sfea0)
     return Fort15(rundes, runid, run, emo, efa, tags, elev, vel, nfover=nfover, nabout=nabout,
                   nscreen=nscreen, ihot=ihot, ics=ics, im=im, nws=nws, nramp=nramp, ncor=ncor,
                   ntip=ntip, slam0=slam0, sfea0=sfea0, ititer=ititer, isldia=isldia,
@@ -324,7 +332,7 @@ def write_fort15(path: str, f: Fort15) -> None:
     add([r.nolicat], "NOLICAT")
     add([0], "NWP")
     add([0], "NCOR")
-    add([0], "NTIP")
+    add([1 if r.tipot else 0], "NTIP")
     add([f.nws], "NWS")
     add([r.nramp], "NRAMP")
     add([r.g], "G")
@@ -354,7 +362,10 @@ def write_fort15(path: str, f: Fort15) -> None:
     add([r.cf], "TAU" if r.nolibf == 0 else "CF")
     add([r.eslm], "ESLM")
     add([r.cori], "CORI")
-    add([0], "NTIF")
+    add([len(r.tipot)], "NTIF")
+    for k, c in enumerate(r.tipot):
+        L.append(f"T{k + 1}")
+        add(list(c), "TPK AMIGT ETRF FFT FACET")
     add([r.nbfr], "NBFR")
     tags = list(f.bountag) + [f"F{k + 1}" for k in range(len(f.bountag), r.nbfr)]
     for k in range(r.nbfr):

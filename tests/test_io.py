@@ -66,7 +66,7 @@ test run                                                 ! RUNID
 1                       ! NOLICAT
 0                       ! NWP
 0                       ! NCOR
-0                       ! NTIP
+1                       ! NTIP
 0                       ! NWS
 8                       ! NRAMP
 9.81                    ! G
@@ -75,7 +75,7 @@ test run                                                 ! RUNID
 1.5                     ! STATIM
 0.25                    ! REFTIM
 2.0                     ! RNDAY
-1.0, 2.0, 0.0, 3.0, 0.75, 5.0, 6.0, 7.0, 8.0  ! DRAMP ... DRAMPElev ... DUnRampMete
+1.0, 2.0, 0.0, 3.0, 0.75, 0.75, 6.0, 7.0, 8.0  ! DRAMP ... DRAMPElev DRAMPTip ... DUnRampMete
 0.35 0.30 0.35          ! A00 B00 C00
 0.1 0 0 0.02            ! H0 NODEDRYMIN NODEWETMIN VELMIN
 -79.0 35.0              ! SLAM0 SFEA0
@@ -118,7 +118,7 @@ M2
 def test_fort15_documented_layout(tmp_path):
     """A fort.15 typed by hand in the documented layout: comments after the
     values, D exponents, comma separators, the nine-value NRAMP = 8 line,
-    tidal potential constituents (NTIF > 0 read and skipped), stations."""
+    the tidal potential (NTIP = 1 and its constituent), stations."""
     p = tmp_path / "fort.15"
     p.write_text(DOCUMENTED)
     f = fort15.read_fort15(str(p), 2)
@@ -136,6 +136,8 @@ def test_fort15_documented_layout(tmp_path):
     assert np.array_equal(f.emo, [[0.5, 0.1], [0.6, 0.2]])
     assert np.array_equal(f.efa, [[10.0, 20.0], [11.0, 21.0]])
     assert r.anginn == 90.0
+    assert f.ntip == 1 and r.tipot == ((0.242334, 1.405189e-4, 0.693, 1.0, 0.0),)
+    assert (r.slam0, r.sfea0) == (-79.0, 35.0)
     assert f.elev_sta == fort15.Output(1, 1.5, 2.5, 60)
     assert np.array_equal(f.stations_e, [[100.0, 200.0], [300.0, 400.0]])
     assert f.elev == fort15.Output(1, 1.5, 3.5, 120)
@@ -154,7 +156,7 @@ def test_fort15_documented_layout(tmp_path):
 
 
 @pytest.mark.parametrize("line,value,word", [
-    (7, "2", "ICS"), (8, "1", "IM"), (8, "511112", "IM"), (13, "1", "NWP"), (15, "1", "NTIP"),
+    (7, "2", "ICS"), (8, "1", "IM"), (8, "511112", "IM"), (13, "1", "NWP"), (15, "2", "NTIP"),
     (16, "12", "NWS"), (19, "-3", "TAU0"), (9, "2", "NOLIBF")])
 def test_fort15_unsupported_options_raise(tmp_path, line, value, word):
     lines = DOCUMENTED.splitlines()
