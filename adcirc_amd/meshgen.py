@@ -71,6 +71,10 @@ class RunConfig:
     # longitude/latitude (ICS = 1)
     slam0: float = 0.0
     sfea0: float = 0.0
+    # periodic specified normal flux (NFFR > 0) on IBTYPE 2/12/22 boundaries:
+    # per constituent (FAMIG rad/s, FFF, FFACE degrees); the amplitudes and
+    # phases per boundary node are Case.qnam / Case.qnph
+    fluxf: Tuple[Tuple[float, float, float], ...] = ()
 
     @property
     def nbfr(self) -> int:
@@ -84,6 +88,10 @@ class RunConfig:
     def ntip(self) -> int:
         return 1 if self.tipot else 0
 
+    @property
+    def nffr(self) -> int:
+        return len(self.fluxf)
+
 
 @dataclasses.dataclass
 class Case:
@@ -91,6 +99,10 @@ class Case:
     run: RunConfig
     emo: np.ndarray   # [neta, nbfr] amplitudes (m)
     efa: np.ndarray   # [neta, nbfr] phases (degrees)
+    # normal flux per node of the IBTYPE 2/12/22 segments in fort.14 order
+    # (fort.15's QNAM, QNPH per constituent): [entries, nffr], m^2/s and degrees
+    qnam: Optional[np.ndarray] = None
+    qnph: Optional[np.ndarray] = None
 
 
 def _cell_split(i: int, j: int) -> bool:
@@ -168,6 +180,42 @@ def basin_tide(seed: int = 1) -> Case:
     case = basin(seed)
     case.run.tipot = TIPOT_M2S2K1O1
     case.run.slam0, case.run.sfea0 = -75.0, 35.0
+    return case
+
+
+def basin_river(seed: int = 1) -> Case:
+    """The basin with two rivers besides its boundary tide: 8 km of the
+    shallow west edge take a specified normal flux as an essential condition
+    (IBTYPE 2), 10 km of the north edge as a natural one (IBTYPE 22).  Each
+    carries a steady discharge and a tidal (M2) part, positive into the
+    domain; the land segments meet the rivers at shared end nodes."""
+    nx, ny, lx, ly = 101, 51, 100.0e3, 50.0e3
+    case = basin(seed, nx, ny, lx, ly)
+    ny_ = ny
+
+    def nid(i, j):
+        return i * ny_ + j + 1
+
+    # land runs north edge east -> west, west edge north -> south, south edge
+    # west -> east (as _structured builds it); cut it at the rivers
+    north = [nid(i, ny - 1) for i in range(nx - 1, -1, -1)]
+    west = [nid(0, j) for j in range(ny - 2, -1, -1)]
+    south = [nid(i, 0) for i in range(1, nx)]
+    ring = north + west + south
+    r_n0, r_n1 = ring.index(nid(40, ny - 1)), ring.index(nid(30, ny - 1))
+    r_w0, r_w1 = ring.index(nid(0, 29)), ring.index(nid(0, 21))
+    segs = [(0, ring[:r_n0 + 1]), (22, ring[r_n0:r_n1 + 1]), (0, ring[r_n1:r_w0 + 1]),
+            (2, ring[r_w0:r_w1 + 1]), (0, ring[r_w1:])]
+    case.mesh.land_bnd = [(ib, np.array(v, np.int32)) for ib, v in segs]
+    case.mesh.name = f"basin river {nx}x{ny} seed {seed}"
+    case.run.fluxf = ((0.0, 1.0, 0.0), (M2_OMEGA, 1.0, 30.0))
+    n22, n2 = r_n1 - r_n0 + 1, r_w1 - r_w0 + 1
+    # m^2/s per unit width: steady 0.5 (north) and 1.2 (west), M2 part 0.2/0.4
+    q = np.concatenate([np.column_stack([np.full(n22, 0.5), np.full(n22, 0.2)]),
+                        np.column_stack([np.full(n2, 1.2), np.full(n2, 0.4)])])
+    ph = np.concatenate([np.column_stack([np.zeros(n22), np.linspace(0.0, 20.0, n22)]),
+                         np.column_stack([np.zeros(n2), np.linspace(10.0, 0.0, n2)])])
+    case.qnam, case.qnph = q, ph
     return case
 
 

@@ -113,12 +113,17 @@ def csr_positions(neitab: np.ndarray, nneigh: np.ndarray, nm0: np.ndarray) -> np
 _ME_OVERLAP_CODES = (2, 12, 22, 3, 13, 23, 30, 32, 40, 41, 52, 64)
 
 
+FLOW_TYPES = (0, 10, 20, 2, 12, 22)   # land and specified-normal-flux boundaries
+FLUX_TYPES = (2, 12, 22)
+
+
 def flow_boundaries(x: np.ndarray, y: np.ndarray, land, costset: float):
-    """NBV, LBCODEI, CSII, SIII, ME2GW (1-based) for external flow boundaries."""
-    nbv, lbc, csii, siii, me2gw = [], [], [], [], []
+    """NBV, LBCODEI, CSII, SIII, ME2GW (1-based) and BNDLEN2O3 for external
+    flow boundaries (src/mesh.F:2230-2420)."""
+    nbv, lbc, csii, siii, me2gw, bndlen = [], [], [], [], [], []
     jgw = 0
     for ib, seg in land:
-        if ib not in (0, 10, 20):
+        if ib not in FLOW_TYPES:
             raise NotImplementedError(f"flow boundary type {ib}")
         v = [int(n) for n in seg]
         nvell = len(v)
@@ -134,6 +139,10 @@ def flow_boundaries(x: np.ndarray, y: np.ndarray, land, costset: float):
             nbv.append(ni)
             xgi, xgj, xgk = x[ni - 1], x[nj - 1], x[nk - 1]
             ygi, ygj, ygk = y[ni - 1], y[nj - 1], y[nk - 1]
+            # two thirds of the edge to the next node; zero for the last node
+            # of a segment, whose next node is itself (src/mesh.F:2300-2303)
+            delx, dely = xgj - xgi, ygj - ygi
+            bndlen.append(2.0 * math.sqrt(delx * delx + dely * dely) / 3.0)
             calc = True
             if i == 1 and nbvv[1] == nbvv[0]:
                 theta, cross, costheta = -9999999.0, 0.0, costset
@@ -177,7 +186,7 @@ def flow_boundaries(x: np.ndarray, y: np.ndarray, land, costset: float):
                 if nbvv[i] == nbv[0] and code in _ME_OVERLAP_CODES:
                     me2gw[0] = jgw
     return (np.array(nbv, np.int32), np.array(lbc, np.int32), np.array(csii),
-            np.array(siii), np.array(me2gw, np.int32))
+            np.array(siii), np.array(me2gw, np.int32), np.array(bndlen, np.float64))
 
 
 REARTH = 6378206.4  # the reference's Rearth (src/constants.F90:79)
@@ -228,7 +237,7 @@ def build_tables(mesh: Mesh, nolifa: int, h0: float, anginn: float = 110.0) -> D
     csr = csr_positions(neitab0, nneigh, nm0)
     nbd = np.concatenate(mesh.open_bnd).astype(np.int32) if mesh.open_bnd else np.zeros(0, np.int32)
     costset = math.cos(anginn * (3.141592653589793 / 180.0))
-    nbv, lbc, csii, siii, me2gw = flow_boundaries(x, y, mesh.land_bnd, costset)
+    nbv, lbc, csii, siii, me2gw, bndlen = flow_boundaries(x, y, mesh.land_bnd, costset)
     npn, ne = x.shape[0], nm.shape[0]
     return dict(
         np=npn, ne=ne, mnei=mnei, neta=int(nbd.shape[0]), nvel=int(nbv.shape[0]),
@@ -237,4 +246,5 @@ def build_tables(mesh: Mesh, nolifa: int, h0: float, anginn: float = 110.0) -> D
         # reference layout: NEITAB(NP,MNEI) column-major, 1-based, 0 padded
         neitab=(neitab0 + 1).T.astype(np.int32).ravel(),
         csr=csr, nbd=nbd, nbv=nbv, lbcodei=lbc, csii=csii, siii=siii, me2gw=me2gw,
+        bndlen2o3=bndlen,
     )

@@ -124,6 +124,20 @@ int adco_set_tidepot(adco_model *m, int ntif, const double *tpk, const double *a
 /* RampTip of the last step */
 double adco_ramptip(const adco_model *m);
 
+/* Specified normal flux boundaries (NFLUXF = 1; LBCODEI 2, 12, 22 among land
+ * codes, all <= 29): BNDLEN2O3(NVEL) from the mesh set-up (src/mesh.F:2303).
+ * Each later step shifts QN0 <- QN1 <- QN2 (src/timestep.F:605-612), takes QN2
+ * from the last adco_set_qn2 (zero without one), adds the flux line integrals
+ * to GWCE_LV (src/gwce.F:1737-1834) and uses -QN2/H2 as the normal velocity of
+ * codes 0-19 (src/momentum.F:810-848).  Returns 0, or -1 for a code above 29. */
+int adco_set_flux(adco_model *m, const double *bndlen2o3);
+void adco_set_qn2(adco_model *m, const double *qn2);
+/* QN2 of periodic flux forcing at one time (src/timestep.F:861-878, ICS = 1):
+ * qnam/qnph (NFFR, NVEL) column-major, FFACE and QNPH in radians. */
+void adco_flux_bc(int nvel, const int *lbcodei, int nffr, const double *famig,
+                  const double *fff, const double *fface, const double *qnam,
+                  const double *qnph, double timeh, double rampextflux, double *out);
+
 /* elevation boundary value for one boundary node at one time (gwce.F:1638-1651) */
 void adco_eta_bc(const adco_setup *s, double timeh, double rampelev, double *out);
 

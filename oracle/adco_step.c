@@ -8,8 +8,9 @@
  * (consistent GWCE, Kolar-Gray lateral stress, non-conservative advection,
  * IBPV momentum lateral stress, corrected area integration), Cartesian mesh
  * (ICS = 1: every spherical factor is exactly 1, TANPHI = 0), periodic
- * elevation boundaries, no-normal-flow land boundaries (IBTYPE 0), NOLIFA
- * 0/1/2, NOLIBF 0/1/2 with nodal friction, optional met forcing passed in.
+ * elevation boundaries, no-normal-flow land boundaries (IBTYPE 0, 10, 20),
+ * specified normal flux boundaries (IBTYPE 2, 12, 22), NOLIFA 0/1/2, NOLIBF
+ * 0/1/2 with nodal friction, optional met forcing passed in.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -52,6 +53,11 @@ struct adco_model {
   double tpk[ADCO_MAXTIF], amigt[ADCO_MAXTIF], etrf[ADCO_MAXTIF], fft[ADCO_MAXTIF];
   double facet[ADCO_MAXTIF], pert[ADCO_MAXTIF];
   double *ln, *slam, *tip1, *tip2;
+  /* specified normal flux boundaries (NFLUXF = 1): BNDLEN2O3 and the flux
+   * at three time levels QN0/QN1/QN2 per flow-boundary entry
+   * (src/global.F, src/boundaries.F); qn_next is the next step's QN2 */
+  int nfluxf, qn_pending;
+  double *bndlen, *qn0, *qn1, *qn2, *qn_next;
 };
 
 #define NM(ie, k) (m->s.nm[(size_t)(ie)*3 + (k)] - 1)
@@ -166,7 +172,8 @@ void adco_destroy(adco_model *m) {
                m->mom_lv_x, m->mom_lv_y, m->wsx1, m->wsy1, m->wsx2, m->wsy2, m->pr1,
                m->pr2, m->etamax, m->totalarea, m->tk_tmp, m->tkm, m->auv, m->nodecode,
                m->nnodecode, m->mju, m->etamax_it, m->umax_it, m->umax, m->nibcnt, m->noff, m->noffold, m->alphal, m->coef,
-               m->obccoef, m->wksp, m->ln, m->slam, m->tip1, m->tip2};
+               m->obccoef, m->wksp, m->ln, m->slam, m->tip1, m->tip2, m->bndlen, m->qn0,
+               m->qn1, m->qn2, m->qn_next};
   for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
   free(m);
 }
@@ -226,6 +233,76 @@ static void tide_potential(adco_model *m, double timeh, double ramptip) {
 }
 
 double adco_ramptip(const adco_model *m) { return m->ramp; }
+
+int adco_set_flux(adco_model *m, const double *bndlen2o3) {
+  const int nvel = m->nvel;
+  if (nvel < 1) return -1;
+  /* the codes whose fluxes are specified or zero (LBCODEI <= 29) */
+  for (int k = 0; k < nvel; ++k)
+    if (m->s.lbcodei[k] < 0 || m->s.lbcodei[k] > 29) return -1;
+  if (!m->bndlen) {
+    m->bndlen = dalloc(nvel);
+    m->qn0 = dalloc(nvel);
+    m->qn1 = dalloc(nvel);
+    m->qn2 = dalloc(nvel);
+    m->qn_next = dalloc(nvel);
+  }
+  memcpy(m->bndlen, bndlen2o3, sizeof(double) * nvel);
+  m->nfluxf = 1;
+  m->qn_pending = 0;
+  return 0;
+}
+
+void adco_set_qn2(adco_model *m, const double *qn2) {
+  if (!m->nfluxf) return;
+  memcpy(m->qn_next, qn2, sizeof(double) * m->nvel);
+  m->qn_pending = 1;
+}
+
+/* Periodic specified normal flux (src/timestep.F:861-878, NFFR > 0, ICS = 1 so
+ * rotate_normal_flux returns its argument): QN2 starts at zero (the shift of
+ * src/timestep.F:605-612) and sums each constituent in turn at the entries
+ * whose LBCODEI is 2, 12, 22 or 52.  qnam/qnph are (NFFR, NVEL) column-major,
+ * phases in radians. */
+void adco_flux_bc(int nvel, const int *lbcodei, int nffr, const double *famig,
+                  const double *fff, const double *fface, const double *qnam,
+                  const double *qnph, double timeh, double rampextflux, double *out) {
+  for (int i = 0; i < nvel; ++i) out[i] = 0.0;
+  for (int j = 0; j < nffr; ++j) {
+    /* src/read_input.F:3551-3555 */
+    const double fper = famig[j] == 0.0 ? 0.0 : 2.0 * 3.141592653589793 / famig[j];
+    const int ncyc = fper == 0.0 ? 0 : (int)(timeh / fper);
+    const double argj = famig[j] * (timeh - ncyc * fper) + fface[j];
+    const double rff = fff[j] * rampextflux;
+    for (int i = 0; i < nvel; ++i) {
+      const int c = lbcodei[i];
+      if (c == 2 || c == 12 || c == 22 || c == 52)
+        out[i] = out[i] + qnam[(size_t)i * nffr + j] * rff * cos(argj - qnph[(size_t)i * nffr + j]);
+    }
+  }
+}
+
+/* The specified-flux line integrals into GWCE_LV (src/gwce.F:1737-1834) for
+ * boundary codes up to 29: each boundary edge J-1..J adds
+ * NCI*NCJ*BNDLEN2O3(J-1)/4 * (2 QFORCE(own) + QFORCE(other)) to both of its
+ * nodes, in edge order, I's node first. */
+static void gwce_flux(adco_model *m) {
+  const int *nbv = m->s.nbv;
+  const double *t0 = m->s.tau0var;
+  const double dt2 = m->dt * 2.0;
+  int nbdj = nbv[0] - 1;
+  double qforcej = (m->qn2[0] - m->qn0[0]) / dt2 + t0[nbdj] * m->qn1[0];
+  for (int J = 1; J < m->nvel; ++J) {
+    const int nbdi = nbdj;
+    nbdj = nbv[J] - 1;
+    const double qforcei = qforcej;
+    qforcej = (m->qn2[J] - m->qn0[J]) / dt2 + t0[nbdj] * m->qn1[J];
+    const int nci = m->nodecode[nbdi], ncj = m->nodecode[nbdj];
+    const double w = (double)(nci * ncj) * m->bndlen[J - 1] / 4.0;
+    m->gwce_lv[nbdi] = m->gwce_lv[nbdi] + w * (2.0 * qforcei + qforcej);
+    m->gwce_lv[nbdj] = m->gwce_lv[nbdj] + w * (2.0 * qforcej + qforcei);
+  }
+}
 
 /* flang's NORM2: running maximum with scaled sum of squares */
 static double norm2_3(const double *v) {
@@ -601,6 +678,7 @@ static int gwce_new(adco_model *m, double timeh) {
     m->eta2[i] = 0.0;
   }
   eta_bc_periodic(m, timeh, m->eta2);
+  if (m->nfluxf) gwce_flux(m);
   const int *nbd = m->s.nbd, *nneigh = m->s.nneigh;
   if (m->s.ilump) {
     /* lumped: boundary rows from COEFD, explicit solve (src/gwce.F:1856-1862,
@@ -909,8 +987,9 @@ static void momentum(adco_model *m) {
     int code = m->s.lbcodei[I];
     double CS = m->s.csii[I], SI = m->s.siii[I];
     double *A = auv + 4 * b;
+    const double qn2 = m->nfluxf ? m->qn2[I] : 0.0;
     if (code >= 0 && code <= 9) {
-      double VelNorm = -0.0 / m->h2[b];
+      double VelNorm = -qn2 / m->h2[b];
       double eps = 2.220446049250313e-16;
       if (fabs(norm2_3(m->tkm + 3 * b) - sqrt(2.0) * m->tk[b]) < eps) {
         m->mom_lv_x[b] = (SI * m->mom_lv_x[b] - CS * m->mom_lv_y[b] - VelNorm * A[2]) * nci;
@@ -929,7 +1008,7 @@ static void momentum(adco_model *m) {
       }
     }
     if (code >= 10 && code <= 19) {
-      double VelNorm = -0.0 / m->h2[b];
+      double VelNorm = -qn2 / m->h2[b];
       double VelTan = 0.0;
       m->mom_lv_x[b] = VelTan * nci;
       m->mom_lv_y[b] = VelNorm * nci;
@@ -998,6 +1077,15 @@ int adco_step(adco_model *m, int it, const double *wsx2, const double *wsy2,
   /* src/timestep.F:841; RampTip shares the elevation ramp's length here */
   if (m->ctip) tide_potential(m, timeh, m->ramp);
   friction(m);
+  if (m->nfluxf) {
+    /* src/timestep.F:605-612, then this step's QN2 (src/timestep.F:861-903) */
+    for (int k = 0; k < m->nvel; ++k) {
+      m->qn0[k] = m->qn1[k];
+      m->qn1[k] = m->qn2[k];
+      m->qn2[k] = m->qn_pending ? m->qn_next[k] : 0.0;
+    }
+    m->qn_pending = 0;
+  }
   if (m->s.nws != 0 && wsx2 && wsy2 && pr2) {
     memcpy(m->wsx2, wsx2, sizeof(double) * np);
     memcpy(m->wsy2, wsy2, sizeof(double) * np);
@@ -1039,7 +1127,8 @@ double *adco_array(adco_model *m, const char *name) {
       {"uu1", m->uu1}, {"vv1", m->vv1}, {"wsx2", m->wsx2}, {"wsy2", m->wsy2},
       {"pr2", m->pr2}, {"wsx1", m->wsx1}, {"wsy1", m->wsy1}, {"pr1", m->pr1},
       {"tkm", m->tkm}, {"mom_lv_x", m->mom_lv_x}, {"mom_lv_y", m->mom_lv_y},
-      {"tip1", m->tip1}, {"tip2", m->tip2}};
+      {"tip1", m->tip1}, {"tip2", m->tip2}, {"qn0", m->qn0}, {"qn1", m->qn1},
+      {"qn2", m->qn2}};
   for (size_t k = 0; k < sizeof(t) / sizeof(t[0]); ++k)
     if (!strcmp(t[k].n, name)) return t[k].p;
   if (!strcmp(name, "ep")) return &m->ep;

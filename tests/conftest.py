@@ -48,6 +48,11 @@ def basin_tide_golden():
 
 
 @pytest.fixture(scope="session")
+def basin_river_golden():
+    return load_golden("basin_river")
+
+
+@pytest.fixture(scope="session")
 def basin_golden():
     return load_golden("basin")
 

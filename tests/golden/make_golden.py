@@ -8,7 +8,7 @@ in the build container only; the fixtures it writes are what the tests use
 everywhere else.
 
     python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [basin_linear] [jcg] [tau0]
-                                       [coastal20k] [basin_tide]
+                                       [coastal20k] [basin_tide] [basin_river]
 """
 from __future__ import annotations
 
@@ -66,7 +66,10 @@ def run_case(name: str, case, nsteps: int, early: int, dump_every: int,
             out[f"s{s}_{k}"] = snaps[s][k]
     if snap_keys is SNAP_KEYS:
         out["s1_coef"] = snaps[1]["coef"]
-    r = case.run
+    if case.qnam is not None:
+        # the flux forcing per flux-boundary node, and the reference's edge lengths
+        out["qnam"], out["qnph"] = case.qnam, case.qnph
+        out["bndlen2o3"] = b["bndlen2o3"]
     out["run_nsteps"] = np.array(nsteps)
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **out)
@@ -212,6 +215,11 @@ def main(argv):
         # the basin with the tidal potential of M2, S2, K1, O1 (NTIP = 1) at
         # 75 W, 35 N; one simulated hour
         print(run_case("basin_tide", meshgen.basin_tide(), nsteps=1800, early=10,
+                       dump_every=600))
+    if "basin_river" in todo:
+        # the basin with two rivers (IBTYPE 2 and 22), steady and M2 flux;
+        # one simulated hour
+        print(run_case("basin_river", meshgen.basin_river(), nsteps=1800, early=10,
                        dump_every=600))
     if "jcg" in todo:
         print(make_jcg())

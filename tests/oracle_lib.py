@@ -6,6 +6,7 @@ baseline leg use the oracle as the checker; the product never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from typing import Dict, Optional
 
@@ -77,7 +78,30 @@ def load() -> C.CDLL:
     lib.adco_dot.restype = C.c_double
     lib.adco_dot.argtypes = [C.c_int, C.c_int, _dp, _dp]
     lib.adco_eta_bc.argtypes = [C.POINTER(Setup), C.c_double, C.c_double, _dp]
+    lib.adco_set_flux.restype = C.c_int
+    lib.adco_set_flux.argtypes = [C.c_void_p, _dp]
+    lib.adco_set_qn2.argtypes = [C.c_void_p, _dp]
+    lib.adco_flux_bc.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                 C.c_double, _dp]
     return lib
+
+
+FLUX_CODES = (2, 12, 22, 52)
+
+
+def flux_arrays(lbcodei: np.ndarray, qnam: np.ndarray, qnph_deg: np.ndarray):
+    """QNAM/QNPH (NFFR, NVEL) column-major as the reference holds them
+    (src/read_input.F:3562-3586): the per-entry rows, in fort.15's order, go
+    to the flow-boundary entries whose code is a specified flux; phases in
+    radians."""
+    nvel, nffr = lbcodei.shape[0], qnam.shape[1]
+    am = np.zeros((nvel, nffr))
+    ph = np.zeros((nvel, nffr))
+    idx = np.flatnonzero(np.isin(lbcodei, FLUX_CODES))
+    assert idx.shape[0] == qnam.shape[0], "one QNAM row per flux-boundary entry"
+    am[idx] = qnam
+    ph[idx] = qnph_deg * (3.141592653589793 / 180.0)
+    return am.ravel(), ph.ravel()
 
 
 def _p(a, t):
@@ -149,6 +173,17 @@ class OracleModel:
                                            *[_p(c, _dp) for c in cols],
                                            _p(slam, _dp), _p(sfea, _dp))
             assert rc == 0, "adco_set_tidepot"
+        self.flux = None
+        if getattr(run, "fluxf", ()):
+            lbc = np.asarray(d["lbcodei"], np.int32)
+            am, ph = flux_arrays(lbc, np.asarray(d["qnam"]), np.asarray(d["qnph"]))
+            fc = [np.array([c[k] for c in run.fluxf], np.float64) for k in range(3)]
+            fc[2] = fc[2] * (3.141592653589793 / 180.0)  # FFACE in radians
+            self.flux = dict(lbc=lbc, am=am, ph=ph, famig=fc[0], fff=fc[1], fface=fc[2],
+                             qn2=np.zeros(lbc.shape[0]))
+            self.keep["bndlen2o3"] = np.ascontiguousarray(d["bndlen2o3"], np.float64)
+            rc = self.lib.adco_set_flux(self.h, _p(self.keep["bndlen2o3"], _dp))
+            assert rc == 0, "adco_set_flux"
         self.hol = None
         if run.holland is not None:
             self.hol = np.array(list(run.holland) + [0.0] * (16 - len(run.holland)))
@@ -166,7 +201,24 @@ class OracleModel:
                               _p(self.pr, _dp))
         return self.wsx, self.wsy, self.pr
 
+    def flux_bc(self, it: int) -> np.ndarray:
+        """This step's QN2 (periodic flux forcing; RampExtFlux shares the
+        elevation ramp's length)."""
+        r, f = self.run, self.flux
+        timeh = it * r.dtdp + (r.statim - r.reftim) * 86400.0
+        timeloc = it * r.dtdp + r.statim * 86400.0
+        ramp = 1.0 if r.nramp == 0 else math.tanh((2.0 * timeloc / 86400.0) / r.drampelev)
+        out = np.zeros(f["lbc"].shape[0])
+        self.lib.adco_flux_bc(f["lbc"].shape[0], _p(f["lbc"], _ip), f["famig"].shape[0],
+                              _p(f["famig"], _dp), _p(f["fff"], _dp), _p(f["fface"], _dp),
+                              _p(f["am"], _dp), _p(f["ph"], _dp), timeh, ramp, _p(out, _dp))
+        return out
+
     def step(self, it: int) -> int:
+        if self.flux is not None:
+            q = self.flux_bc(it)
+            self.flux["qn2"] = q
+            self.lib.adco_set_qn2(self.h, _p(q, _dp))
         if self.hol is not None:
             t = it * self.run.dtdp + self.run.statim * 86400.0
             wsx, wsy, pr = self.forcing(t)

@@ -49,6 +49,18 @@ def write_inputs(case, workdir: str, nsteps: int, dump_every: int = 0,
         "ilump_in": str(r.ilump), "hash_steps": "1" if hash_steps else "0",
         "slam0_in": fl(getattr(r, "slam0", 0.0)), "sfea0_in": fl(getattr(r, "sfea0", 0.0)),
     }
+    fluxf = list(getattr(r, "fluxf", ()))
+    if fluxf:
+        nml["nffr_in"] = str(len(fluxf))
+        nml["famig_in"] = ",".join(fl(c[0]) for c in fluxf)
+        nml["fff_in"] = ",".join(fl(c[1]) for c in fluxf)
+        nml["fface_in"] = ",".join(fl(c[2]) for c in fluxf)
+        nml["fluxfile"] = "'fluxforcing.txt'"
+        # fort.15 order: per constituent, every flux-boundary node in turn
+        with open(os.path.join(workdir, "fluxforcing.txt"), "w") as f:
+            for j in range(len(fluxf)):
+                for e in range(case.qnam.shape[0]):
+                    f.write(f"{case.qnam[e, j]:.17g} {case.qnph[e, j]:.17g}\n")
     tip = list(getattr(r, "tipot", ()))
     if tip:
         nml["ntip_in"] = "1"
@@ -127,6 +139,7 @@ def read_output(path: str) -> Dict[str, np.ndarray]:
     d["eta2_0"] = s.arr(np.float64); d["totalarea0"] = s.arr(np.float64)
     d["mju0"] = s.arr(np.int32)
     d["prbckgrnd_mh2o"], d["costset"] = (float(v) for v in s.f8(2))
+    d["bndlen2o3"] = s.arr(np.float64)[:nvel]
     steps: List[np.ndarray] = []
     snaps: Dict[int, Dict[str, np.ndarray]] = {}
     while True:

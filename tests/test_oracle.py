@@ -284,6 +284,46 @@ def test_basin_tidal_potential_matches_reference(basin_tide_golden):
     _run(d, meshgen.basin_tide().run, int(d["run_nsteps"]))
 
 
+def test_basin_rivers_match_reference(basin_river_golden):
+    """The basin with two rivers, a specified normal flux as an essential
+    condition (IBTYPE 2) and as a natural one (IBTYPE 22), each with a steady
+    and an M2 constituent (NFFR = 2, src/timestep.F:605-612, 861-878; the
+    flux line integrals of src/gwce.F:1737-1834 and the normal velocity of
+    src/momentum.F:810-848), one hour: bit-identical every step."""
+    d = basin_river_golden
+    m = _run(d, meshgen.basin_river().run, int(d["run_nsteps"]))
+    assert np.abs(m.flux["qn2"]).max() > 0.2  # the ramp is still rising
+
+
+def test_river_tables_match_reference(basin_river_golden):
+    """The product's flow-boundary tables for a mesh with flux segments
+    (LBCODEI 2, 22, the ME2GW precedence of flux codes at shared end nodes,
+    BNDLEN2O3 with zero after each segment's last node) equal the reference's
+    (src/mesh.F:2230-2420)."""
+    from adcirc_amd import prep
+    d = basin_river_golden
+    case = meshgen.basin_river()
+    t = prep.build_tables(case.mesh, case.run.nolifa, case.run.h0, case.run.anginn)
+    for k in ("nbv", "lbcodei", "csii", "siii", "me2gw", "bndlen2o3"):
+        assert np.array_equal(t[k], d[k]), k
+    assert set(np.unique(d["lbcodei"])) >= {2, 22}
+    assert (d["bndlen2o3"] == 0).sum() == len(case.mesh.land_bnd)
+
+
+def test_river_flux_raises_the_water(basin_river_golden):
+    """Mass balance sign: with the rivers flowing in, the basin's mean
+    elevation after the hour is above the same basin's without them."""
+    d = basin_river_golden
+    river = d["s1800_eta2"]
+    base = meshgen.basin()
+    tb = _small_case_tables(base)
+    mb = oracle_lib.OracleModel(tb, base.run, oracle_lib.SEQUENTIAL)
+    for it in range(1, 1801):
+        mb.step(it)
+    dv = float(np.sum((river - mb.array("eta2")) * d["totalarea0"]))
+    assert dv > 0.0
+
+
 def test_tidal_potential_closed_form(basin_tide_golden):
     """TIP2 against the equilibrium tide written out independently: for each
     constituent the species factor from the latitude (long-period
