@@ -34,7 +34,8 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_ml_collect", "adcgpu_ml_samples", "adcgpu_ml_train", "adcgpu_ml_grad",
            "adcgpu_ml_features", "adcgpu_ml_correction", "adcgpu_state_bytes",
            "adcgpu_state_save", "adcgpu_state_load", "adcgpu_last_solve",
-           "adcgpu_set_jcg_blocks", "adcgpu_set_tide_potential", "adcgpu_batch_create", "adcgpu_batch_run", "adcgpu_batch_destroy")
+           "adcgpu_set_jcg_blocks", "adcgpu_set_tide_potential", "adcgpu_set_flux_boundary",
+           "adcgpu_set_flux", "adcgpu_set_flux_table", "adcgpu_flux_bc", "adcgpu_batch_create", "adcgpu_batch_run", "adcgpu_batch_destroy")
 ML_NPARAM = 22913
 # error codes (ADCGPU_E_*)
 E_ARG, E_UNSUPPORTED, E_HIP, E_DIAG, E_CAPACITY, E_NONFINITE, E_ERRORELEV = range(-1, -8, -1)
@@ -105,6 +106,11 @@ def load() -> C.CDLL:
     lib.adcgpu_set_jcg_blocks.argtypes = [vp, C.c_int]
     lib.adcgpu_set_tide_potential.argtypes = [vp, C.c_int, dp_, dp_, dp_, dp_, dp_, dp_, dp_,
                                               C.c_int, C.c_double, C.c_double, C.c_double]
+    lib.adcgpu_set_flux_boundary.argtypes = [vp, dp_]
+    lib.adcgpu_set_flux.argtypes = [vp, dp_]
+    lib.adcgpu_set_flux_table.argtypes = [vp, C.c_int, dp_]
+    lib.adcgpu_flux_bc.argtypes = [C.c_int, ip_, C.c_int, dp_, dp_, dp_, dp_, dp_, C.c_double,
+                                   C.c_double, dp_]
     lib.adcgpu_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     lib.adcgpu_batch_run.argtypes = [vp, C.c_int, C.c_int, C.POINTER(dp_), C.POINTER(dp_)]
     lib.adcgpu_batch_destroy.argtypes = [vp]

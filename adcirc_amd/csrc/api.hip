@@ -109,6 +109,16 @@ struct adcgpu_ctx {
   int tip_nramp = 0;
   double tip_dramp = 0.0, tip_statim = 0.0, tip_reftim = 0.0;
   double *tip1 = nullptr, *tip2 = nullptr, *tip_ln = nullptr, *tip_slam = nullptr;
+  // specified normal flux boundaries (NFLUXF = 1): the flow-boundary tables
+  // kept on the host, each flux node's edge terms, QN0/QN1/QN2 per entry and
+  // the resident run's QN2 rows
+  int nvel = 0;
+  std::vector<int> h_nbv, h_bnd_index;
+  bool nfluxf = false;
+  int *fb_ptr = nullptr, *fb_ent = nullptr, *nbv0 = nullptr;
+  double *bndlen = nullptr, *qn0 = nullptr, *qn1 = nullptr, *qn2 = nullptr;
+  double* qn_table = nullptr;
+  int qn_table_steps = 0, qn_table_cap = 0;
   int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
   bool coef_scaled = false;  // off-diagonals of COEF left scaled by a deferred solve
   // the GWCE matrix's unchanging terms per incidence slot, and each element's
@@ -162,6 +172,8 @@ struct adcgpu_ctx {
     m.nneigh = nneigh; m.col = col; m.csrpos = csrpos; m.bnd_index = bnd_index; m.nbd = nbd;
     m.obc_q = obc_q; m.obc_s = obc_s; m.nobc = nobc; m.obcg_ptr = obcg_ptr; m.obcg = obcg; m.vbc_ptr = vbc_ptr;
     m.vbc = vbc; m.lbcodei = lbcodei; m.csii = csii; m.siii = siii;
+    m.fb_ptr = nfluxf ? fb_ptr : nullptr; m.fb_ent = nfluxf ? fb_ent : nullptr;
+    m.nbv = nfluxf ? nbv0 : nullptr; m.bndlen = nfluxf ? bndlen : nullptr;
     m.fdx32 = fdx32; m.fdy32 = fdy32; m.areas32 = areas32;
     return m;
   }
@@ -176,6 +188,9 @@ struct adcgpu_ctx {
     s.etamax_it = etamax_it; s.umax = umax; s.umax_it = umax_it; s.ncele = ncele;
     s.tip1 = ctip ? tip1 : nullptr;
     s.tip2 = ctip ? tip2 : nullptr;
+    s.qn0 = nfluxf ? qn0 : nullptr;
+    s.qn1 = nfluxf ? qn1 : nullptr;
+    s.qn2 = nfluxf ? qn2 : nullptr;
     return s;
   }
 };
@@ -641,6 +656,8 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     if (b < 0 || b >= np) return fail(ADCGPU_E_ARG, "NBV out of range");
     vbl[b].push_back(I);
   }
+  std::vector<int> h_nbv(std::max(M->nvel, 1), 0);
+  for (int I = 0; I < M->nvel; ++I) h_nbv[I] = M->nbv[I] - 1;
   std::vector<int> vbc_ptr(np + 1, 0), vbc;
   for (int i = 0; i < np; ++i) {
     vbc_ptr[i + 1] = vbc_ptr[i] + (int)vbl[i].size();
@@ -709,6 +726,9 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   if (!lcol.empty()) U(&c->jlcol, lcol);
   U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_q, obc_q);
   U(&c->obc_s, obc_s); c->nobc = nobc;
+  c->nvel = M->nvel;
+  c->h_nbv = h_nbv;
+  c->h_bnd_index = bnd_index;
   U(&c->obcg_ptr, obcg_ptr); U(&c->obcg, obcg); U(&c->vbc_ptr, vbc_ptr); U(&c->vbc, vbc);
   U(&c->lbcodei, lbc); U(&c->fdx, fdx); U(&c->fdy, fdy); U(&c->areas, var); U(&c->x, vx);
   U(&c->y, vy); U(&c->dp, vdp); U(&c->tau0, vt); U(&c->evm, ve); U(&c->corif, vcor);
@@ -867,6 +887,9 @@ void adcgpu_destroy(adcgpu_ctx* c) {
   }
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->eta_bc_table) (void)hipFree(c->eta_bc_table);
+  for (void* p : {(void*)c->fb_ptr, (void*)c->fb_ent, (void*)c->nbv0, (void*)c->bndlen,
+                  (void*)c->qn_table})
+    if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -981,6 +1004,9 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
   HIPCHK(hipMemsetAsync(c->etamax_it, 0, np * 4, c->stream));
   HIPCHK(hipMemsetAsync(c->umax_it, 0, np * 4, c->stream));
   HIPCHK(hipMemsetAsync(c->jsaved, 0, 12 * sizeof(double), c->stream));
+  if (c->nfluxf)
+    for (double* p : {c->qn0, c->qn1, c->qn2})
+      HIPCHK(hipMemsetAsync(p, 0, (size_t)c->nvel * 8, c->stream));
   {
     std::vector<double> pr(np, c->prm.prbckgrnd_mh2o), low(np, -99999.0), ones(ne, 1.0);
     HIPCHK(hipMemcpyAsync(c->pr1, pr.data(), np * 8, hipMemcpyHostToDevice, c->stream));
@@ -1006,9 +1032,13 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
   return check_launch("cold start");
 }
 
-int adcgpu_begin_step(adcgpu_ctx* c, int it) {
-  if (!c) return fail(ADCGPU_E_ARG, "null context");
+// The time-level shifts of the step's start; qn_next: this step's QN2 on the
+// device (null: zero until adcgpu_set_flux).
+static int begin_step(adcgpu_ctx* c, int it, const double* qn_next) {
   c->cur_it = it;
+  if (c->nfluxf)
+    hipLaunchKernelGGL(k_flux_shift, dim3(nblk(c->nvel, 256)), dim3(256), 0, c->stream, c->qn0,
+                       c->qn1, c->qn2, qn_next, c->nvel);
   if (it == 1)
     hipLaunchKernelGGL(k_init_h, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->dp,
                        c->eta2, c->ifnlfa, c->h1, c->h2);
@@ -1042,6 +1072,18 @@ int adcgpu_begin_step(adcgpu_ctx* c, int it) {
                        (const double*)c->tip_ln, (const double*)c->tip_slam, c->tip2);
   }
   return check_launch("begin_step");
+}
+
+int adcgpu_begin_step(adcgpu_ctx* c, int it) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  return begin_step(c, it, nullptr);
+}
+
+int adcgpu_set_flux(adcgpu_ctx* c, const double* qn2) {
+  if (!c || !qn2) return fail(ADCGPU_E_ARG, "null argument");
+  if (!c->nfluxf) return fail(ADCGPU_E_ARG, "no flux boundaries set up (adcgpu_set_flux_boundary)");
+  HIPCHK(hipMemcpyAsync(c->qn2, qn2, (size_t)c->nvel * 8, hipMemcpyHostToDevice, c->stream));
+  return 0;
 }
 
 static FricPrm fric_prm(const adcgpu_ctx* c) {
@@ -1349,6 +1391,8 @@ int adcgpu_holland(adcgpu_ctx* c, double t, const double* prm13) {
 static int run_prepare(adcgpu_ctx* c, int nsteps, const double* eta_bc_table,
                        const double* holland) {
   if (c->prm.nws != 0 && !holland) return fail(ADCGPU_E_ARG, "met forcing required");
+  if (c->nfluxf && c->qn_table_steps < nsteps)
+    return fail(ADCGPU_E_ARG, "flux boundaries: adcgpu_set_flux_table rows for every step required");
   if (c->neta > 0) {
     if (!eta_bc_table) return fail(ADCGPU_E_ARG, "eta_bc_table required");
     if (c->eta_bc_table_steps < nsteps) {
@@ -1370,7 +1414,7 @@ static int run_prepare(adcgpu_ctx* c, int nsteps, const double* eta_bc_table,
 // A resident step up to the GWCE solve: begin, friction (with this step's
 // Holland forcing in the same pass), the assembly.
 static int run_step_assemble(adcgpu_ctx* c, int it, int s, const double* holland, GwceStep* g) {
-  int rc = adcgpu_begin_step(c, it);
+  int rc = begin_step(c, it, c->nfluxf ? c->qn_table + (size_t)s * c->nvel : nullptr);
   if (!rc) {
     if (holland) {
       // friction and this step's forcing in one pass
@@ -1660,6 +1704,82 @@ int adcgpu_set_jcg_blocks(adcgpu_ctx* c, int blocks) {
   return 0;
 }
 
+int adcgpu_set_flux_boundary(adcgpu_ctx* c, const double* bndlen2o3) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int np = c->np, nvel = c->nvel;
+  for (void** p : {(void**)&c->fb_ptr, (void**)&c->fb_ent, (void**)&c->nbv0, (void**)&c->bndlen})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  c->nfluxf = false;
+  c->qn_table_steps = 0;
+  // the elevation-boundary indices without flux nodes
+  HIPCHK(hipMemcpy(c->bnd_index, c->h_bnd_index.data(), (size_t)np * sizeof(int),
+                   hipMemcpyHostToDevice));
+  if (!bndlen2o3) return 0;
+  if (nvel < 2) return fail(ADCGPU_E_ARG, "flux boundaries need at least two flow-boundary nodes");
+  for (int k = 0; k < nvel; ++k) {
+    if (c->h_nbv[k] < 0 || c->h_nbv[k] >= np) return fail(ADCGPU_E_ARG, "NBV out of range");
+    if (!(bndlen2o3[k] >= 0.0 && bndlen2o3[k] <= 1.7976931348623157e308))
+      return fail(ADCGPU_E_ARG, "BNDLEN2O3 must be finite and non-negative");
+  }
+  // each node's edge terms in the reference's edge order (src/gwce.F:1777-1832):
+  // edge J adds to NBV(J-1) first, then to NBV(J).  Zero-length edges (after
+  // each segment's last node) add exactly +0 and are left out, and so are the
+  // elevation-boundary rows, which the boundary condition overwrites.
+  std::vector<std::vector<int>> terms(np);
+  for (int J = 1; J < nvel; ++J) {
+    if (bndlen2o3[J - 1] == 0.0) continue;
+    const int a = c->h_nbv[J - 1], b = c->h_nbv[J];
+    if (c->h_bnd_index[a] < 0) terms[a].push_back(2 * J);
+    if (c->h_bnd_index[b] < 0) terms[b].push_back(2 * J + 1);
+  }
+  std::vector<int> bidx = c->h_bnd_index, ptr(1, 0), ent;
+  for (int i = 0; i < np; ++i) {
+    if (terms[i].empty()) continue;
+    bidx[i] = -2 - (int)(ptr.size() - 1);
+    ent.insert(ent.end(), terms[i].begin(), terms[i].end());
+    ptr.push_back((int)ent.size());
+  }
+  std::vector<double> len(bndlen2o3, bndlen2o3 + nvel);
+  if (int rc = upload(&c->fb_ptr, ptr)) return rc;
+  if (int rc = upload(&c->fb_ent, ent)) return rc;
+  if (int rc = upload(&c->nbv0, c->h_nbv)) return rc;
+  if (int rc = upload(&c->bndlen, len)) return rc;
+  if (!c->qn0) {
+    for (double** p : {&c->qn0, &c->qn1, &c->qn2}) {
+      if (int rc = dmalloc(p, (size_t)nvel)) return rc;
+      c->allocs.push_back((void*)*p);
+    }
+  }
+  // QN0 = QN1 = QN2 = 0, as a cold start leaves them (src/cstart.F)
+  for (double* p : {c->qn0, c->qn1, c->qn2}) HIPCHK(hipMemset(p, 0, (size_t)nvel * 8));
+  HIPCHK(hipMemcpy(c->bnd_index, bidx.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
+  c->nfluxf = true;
+  return 0;
+}
+
+int adcgpu_set_flux_table(adcgpu_ctx* c, int nsteps, const double* qn2_table) {
+  if (!c || nsteps < 0 || (nsteps > 0 && !qn2_table)) return fail(ADCGPU_E_ARG, "invalid argument");
+  if (!c->nfluxf) return fail(ADCGPU_E_ARG, "no flux boundaries set up (adcgpu_set_flux_boundary)");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->qn_table_cap < nsteps) {
+    if (c->qn_table) (void)hipFree(c->qn_table);
+    c->qn_table = nullptr;
+    c->qn_table_cap = 0;
+    if (int rc = dmalloc(&c->qn_table, (size_t)nsteps * c->nvel)) return rc;
+    c->qn_table_cap = nsteps;
+  }
+  if (nsteps > 0)
+    HIPCHK(hipMemcpy(c->qn_table, qn2_table, (size_t)nsteps * c->nvel * 8, hipMemcpyHostToDevice));
+  c->qn_table_steps = nsteps;
+  return 0;
+}
+
 int adcgpu_set_tide_potential(adcgpu_ctx* c, int ntif, const double* tpk, const double* amigt,
                               const double* etrf, const double* fft, const double* facet,
                               const double* slam, const double* sfea, int nramp,
@@ -1873,6 +1993,8 @@ std::vector<StatePart> state_parts(adcgpu_ctx* c) {
     v.push_back({c->tip1, np * 8});
     v.push_back({c->tip2, np * 8});
   }
+  if (c->nfluxf)
+    for (double* p : {c->qn0, c->qn1, c->qn2}) v.push_back({p, (size_t)c->nvel * 8});
   return v;
 }
 }  // namespace
@@ -1990,6 +2112,30 @@ int adcgpu_elevation_bc(int neta, int nbfr, const double* amig, const double* ff
 }
 
 // ---------------------------------------------------------------- drop-in JCG
+int adcgpu_flux_bc(int nvel, const int32_t* lbcodei, int nffr, const double* famig,
+                   const double* fff, const double* fface, const double* qnam,
+                   const double* qnph, double timeh, double rampextflux, double* out) {
+  if (nvel < 0 || nffr < 0 || !out || (nvel > 0 && !lbcodei) ||
+      (nffr > 0 && (!famig || !fff || !fface || !qnam || !qnph)))
+    return fail(ADCGPU_E_ARG, "invalid argument");
+  for (int i = 0; i < nvel; ++i) out[i] = 0.0;
+  for (int j = 0; j < nffr; ++j) {
+    // FPER (src/read_input.F:3551-3555); the phase reduced by whole periods
+    const double fper = famig[j] == 0.0 ? 0.0 : 2.0 * 3.141592653589793 / famig[j];
+    const int ncyc = fper == 0.0 ? 0 : (int)(timeh / fper);
+    const double argj = famig[j] * (timeh - ncyc * fper) + fface[j];
+    const double rff = fff[j] * rampextflux;
+    for (int i = 0; i < nvel; ++i) {
+      const int code = lbcodei[i];
+      if (code == 2 || code == 12 || code == 22 || code == 52) {
+        const size_t q = (size_t)i * nffr + j;
+        out[i] = out[i] + qnam[q] * rff * std::cos(argj - qnph[q]);
+      }
+    }
+  }
+  return 0;
+}
+
 int adcgpu_jcg(int n, int ndim, int maxnz, int32_t* jcoef, double* coef, double* rhs, double* u,
                int32_t* iparm, double* rparm) {
   if (n <= 0) return 11;

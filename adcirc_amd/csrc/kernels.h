@@ -77,7 +77,8 @@ struct DevMesh {
   const int* nneigh;    // [np]
   const int* col;       // [mnei*np] column-major, 0-based (slot 0 = self)
   const int* csrpos;    // [ne*3*2] 0-based column slots
-  const int* bnd_index; // [np] elevation-boundary index or -1
+  const int* bnd_index; // [np] elevation-boundary index, -1, or -2-k for the
+                        //      k-th node that takes specified-flux edge terms
   const int* nbd;       // [neta] 0-based
   const int* obc_q;     // [nobc] COEF positions j*np+i that couple row i to an
   const int* obc_s;     //        elevation-boundary node, and their OBCCOEF slot
@@ -90,6 +91,14 @@ struct DevMesh {
   const int* lbcodei;
   const double* csii;
   const double* siii;
+  // specified normal flux boundaries (null without): per flux node k its
+  // edge terms fb_ent[fb_ptr[k] .. fb_ptr[k+1]) in the reference's edge
+  // order, each 2*J + r for the edge NBV(J-1)-NBV(J) (0-based J >= 1) with
+  // r = 0 when the node is NBV(J-1), 1 when it is NBV(J)
+  const int* fb_ptr;
+  const int* fb_ent;
+  const int* nbv;       // [nvel] 0-based
+  const double* bndlen; // [nvel] BNDLEN2O3
   // mixed precision: FP32 copies of the element geometry (null in FP64 runs)
   const float* fdx32;
   const float* fdy32;
@@ -106,7 +115,11 @@ struct DevState {
   int *umax_it;    // step of the last UMAX rise (0: never)
   unsigned char* ncele;  // [ne] NCEle of each element for the next GWCE matrix
   double *tip1, *tip2;   // tidal potential at time levels s and s+1 (null: none)
+  double *qn0, *qn1, *qn2;  // normal flux per flow-boundary entry at three time
+                            // levels (null: no flux boundaries)
 };
+
+__global__ void k_flux_shift(double* qn0, double* qn1, double* qn2, const double* next, int nvel);
 
 // The tidal potential of one step (src/timestep.F:1527-1557, NTIP = 1), per
 // constituent: its species, ARGT (the phase at the step, reduced by whole

@@ -804,6 +804,41 @@ __global__ __launch_bounds__(256) void k_rhs_elem32(DevMesh m, DevState s, RhsPr
   contrib[m.slot[3 * e + 2]] = R3 * nce;
 }
 
+// QFORCE of flow-boundary entry K for a specified (or zero) flux, codes up to
+// 29 (src/gwce.F:1741, 1783): (QN2 - QN0)/DT2 + Tau0Var*QN1.
+__device__ __forceinline__ double qforce(const DevMesh& m, const DevState& s, int K) {
+  return (s.qn2[K] - s.qn0[K]) / (m.dt * 2.0) + m.tau0[m.nbv[K]] * s.qn1[K];
+}
+
+// The specified-flux line integrals a node receives (src/gwce.F:1777-1832):
+// for each boundary edge NBV(J-1)-NBV(J) it lies on, in edge order,
+// NCI*NCJ*BNDLEN2O3(J-1)/4 * (2 QFORCE(own) + QFORCE(other)) added to its
+// GWCE_LV.  The edges whose terms are exactly +0 (zero length) are left out.
+__device__ __forceinline__ double flux_terms(const DevMesh& m, const DevState& s, int k, double acc) {
+  for (int p = m.fb_ptr[k]; p < m.fb_ptr[k + 1]; ++p) {
+    const int e = m.fb_ent[p];
+    const int J = e >> 1;
+    const int ni = m.nbv[J - 1], nj = m.nbv[J];
+    const double qi = qforce(m, s, J - 1), qj = qforce(m, s, J);
+    const double w = (double)(s.nodecode[ni] * s.nodecode[nj]) * m.bndlen[J - 1] / 4.0;
+    acc = acc + ((e & 1) ? w * (2.0 * qj + qi) : w * (2.0 * qi + qj));
+  }
+  return acc;
+}
+
+// The shift of the specified normal flux to the previous time levels and the
+// step's new QN2 (src/timestep.F:605-612, then 861-903): next == null leaves
+// QN2 zero, as the reference's shift does.
+__global__ __launch_bounds__(256) void k_flux_shift(double* __restrict__ qn0, double* __restrict__ qn1,
+                                                    double* __restrict__ qn2,
+                                                    const double* __restrict__ next, int nvel) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nvel) return;
+  qn0[k] = qn1[k];
+  qn1[k] = qn2[k];
+  qn2[k] = next ? next[k] : 0.0;
+}
+
 // Node gather of GWCE_LV, the time-level shift of ETA and the elevation
 // boundary conditions (src/gwce.F:1613-1631, 1638-1651, 1846-1855).  `eta1`
 // already points at the previous ETA2; `eta2` is rewritten.
@@ -848,6 +883,7 @@ __global__ __launch_bounds__(256) void k_rhs_node(DevMesh m, DevState s, const C
     } else {
       for (int p = m.inc_ptr[i]; p < m.inc_ptr[i + 1]; ++p) acc = acc + contrib[p];
     }
+    if (bi < -1) acc = flux_terms(m, s, -2 - bi, acc);
     for (int p = m.obcg_ptr[i]; p < (LUMPED ? m.obcg_ptr[i] : m.obcg_ptr[i + 1]); ++p) {
       const int slot = m.obcg[p];
       const int I = slot / m.mnei;
@@ -1733,7 +1769,7 @@ __global__ __launch_bounds__(256) void k_mom_node(DevMesh m, DevState s, MomPrm 
     const int I = m.vbc[p];
     const int code = m.lbcodei[I];
     const double CS = m.csii[I], SI = m.siii[I];
-    const double veln = -0.0 / s.h2[i];
+    const double veln = -(s.qn2 ? s.qn2[I] : 0.0) / s.h2[i];
     if (code >= 0 && code <= 9) {
       if (isotropic) {
         mx = (SI * mx - CS * my - veln * A2) * dnci;

@@ -104,6 +104,26 @@ def write_case(path: str, tables, run, eta2, nodecode, noff, eta_bc_table, holla
             f.write(a.tobytes())
 
 
+FLUX_CODES = (2, 12, 22, 52)
+
+
+def flux_amplitudes(lbcodei, qnam, qnph_deg):
+    """QNAM and QNPH (radians) laid out (NFFR, NVEL) column-major as the
+    reference holds them (src/read_input.F:3562-3586): fort.15's rows, one per
+    flow-boundary entry whose code takes a specified flux, in NBV order; zero
+    at the other entries."""
+    lbc = np.asarray(lbcodei, np.int32)
+    qnam = np.asarray(qnam, np.float64)
+    idx = np.flatnonzero(np.isin(lbc, FLUX_CODES))
+    if qnam.ndim != 2 or idx.shape[0] != qnam.shape[0]:
+        raise ValueError(f"{idx.shape[0]} flux-boundary entries, QNAM has {qnam.shape[0]} rows")
+    am = np.zeros((lbc.shape[0], qnam.shape[1]))
+    ph = np.zeros_like(am)
+    am[idx] = qnam
+    ph[idx] = np.asarray(qnph_deg, np.float64) * DEG2RAD
+    return am, ph
+
+
 def run_times(run, it: int):
     """(timeloc, timeh, ramp) of step it: TimeLoc (src/timestep.F:257), the
     harmonic time TIMEH of the elevation forcing, and RampElev, the tanh ramp
@@ -141,7 +161,7 @@ def elevation_table(run, emo, efa, it0: int, nsteps: int) -> np.ndarray:
 class Model:
     def __init__(self, tables: Dict[str, np.ndarray], run, device: int = 0,
                  tau0var=None, evm=None, fric=None, corif=None, emo=None, efa=None,
-                 precision: int = _lib.FP64):
+                 precision: int = _lib.FP64, qnam=None, qnph=None):
         lib = _lib.load()
         self.lib, self.run = lib, run
         t = tables
@@ -181,6 +201,11 @@ class Model:
             from .prep import lonlat_cpp
             slam, sfea = lonlat_cpp(t["x"], t["y"], run.slam0, run.sfea0)
             self.set_tide_potential(run.tipot, slam, sfea, run)
+        self.flux = None
+        if getattr(run, "fluxf", ()):
+            if qnam is None or qnph is None or "bndlen2o3" not in t:
+                raise ValueError("flux boundaries need QNAM/QNPH and the tables' BNDLEN2O3")
+            self.set_flux_boundary(t["lbcodei"], t["bndlen2o3"], run.fluxf, qnam, qnph)
 
     @staticmethod
     def radians(deg: np.ndarray) -> np.ndarray:
@@ -251,6 +276,41 @@ class Model:
             int(run.nramp), float(run.drampelev), float(run.statim), float(run.reftim)),
             "set_tide_potential")
 
+    def set_flux_boundary(self, lbcodei, bndlen2o3, constituents, qnam, qnph) -> None:
+        """Specified normal flux boundaries with periodic forcing (NFFR > 0):
+        constituents as fort.15's (FAMIG rad/s, FFF, FFACE degrees), QNAM and
+        QNPH (degrees) per flux-boundary entry and constituent."""
+        bl = np.ascontiguousarray(bndlen2o3, np.float64)
+        check(self.lib.adcgpu_set_flux_boundary(self.h, _p(bl, dp_)), "set_flux_boundary")
+        am, ph = flux_amplitudes(lbcodei, qnam, qnph)
+        c = [np.array([x[k] for x in constituents], np.float64) for k in range(3)]
+        c[2] = c[2] * DEG2RAD
+        self.flux = dict(lbc=np.ascontiguousarray(lbcodei, np.int32), am=am, ph=ph,
+                         famig=c[0], fff=c[1], fface=c[2])
+        self._qn = np.zeros(am.shape[0])
+
+    def flux_bc(self, it: int, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """QN2 of step it (adcgpu_flux_bc; RampExtFlux shares the elevation
+        ramp's length)."""
+        f = self.flux
+        _, timeh, ramp = self.times(it)
+        out = np.empty(f["lbc"].shape[0]) if out is None else out
+        check(self.lib.adcgpu_flux_bc(f["lbc"].shape[0], _p(f["lbc"], ip_), f["famig"].shape[0],
+                                      _p(f["famig"], dp_), _p(f["fff"], dp_), _p(f["fface"], dp_),
+                                      _p(f["am"], dp_), _p(f["ph"], dp_), timeh, ramp,
+                                      _p(out, dp_)), "flux_bc")
+        return out
+
+    def flux_rows(self, it0: int, nsteps: int) -> None:
+        """QN2 of steps it0 .. it0+nsteps-1 for the next resident run
+        (adcgpu_set_flux_table); nothing without flux boundaries."""
+        if self.flux is None:
+            return
+        qt = np.empty((nsteps, self.flux["lbc"].shape[0]))
+        for s in range(nsteps):
+            self.flux_bc(it0 + s, qt[s])
+        check(self.lib.adcgpu_set_flux_table(self.h, nsteps, _p(qt, dp_)), "set_flux_table")
+
     # ------------------------------------------------------------- stepping
     def timestep(self, it: int, forcing: Optional[Sequence[np.ndarray]] = None,
                  tip2: Optional[np.ndarray] = None) -> None:
@@ -258,10 +318,16 @@ class Model:
         with the caller's values (the per-step calls of adcgpu_timestep with
         ADCGPU_TIP2 set after adcgpu_begin_step)."""
         bc = self.eta_bc(it, self._bc)
-        if tip2 is not None:
+        if tip2 is not None or self.flux is not None:
             lib = self.lib
             check(lib.adcgpu_begin_step(self.h, it), "begin_step")
-            self.set("tip2", tip2)
+            if tip2 is not None:
+                self.set("tip2", tip2)
+            if self.flux is not None:
+                # a fresh array each step: the copy may still read it after
+                # the call returns
+                self._qn = q = self.flux_bc(it)
+                check(lib.adcgpu_set_flux(self.h, _p(q, dp_)), "set_flux")
             check(lib.adcgpu_friction(self.h), "friction")
             if forcing is not None and self.run.nws != 0:
                 w, v, p = (np.ascontiguousarray(a, np.float64) for a in forcing)
@@ -288,6 +354,7 @@ class Model:
                      table: Optional[np.ndarray] = None) -> int:
         tab = self.eta_bc_table(it0, nsteps) if table is None else table
         tab = np.ascontiguousarray(tab, np.float64)
+        self.flux_rows(it0, nsteps)
         hol = None
         if holland is not None:
             hol = np.ascontiguousarray(list(holland) + [0.0] * (13 - len(holland)), np.float64)
@@ -469,6 +536,8 @@ class Batch:
                                      else tables[k], np.float64)
                 for k, m in enumerate(self.models)]
         tp = (dp_ * n)(*[_p(t, dp_) for t in tabs])
+        for m in self.models:
+            m.flux_rows(it0, nsteps)
         hp = None
         hols = []
         if holland is not None:

@@ -117,7 +117,9 @@ enum adcgpu_field {
 
 /* Create a context on `device` (HIP ordinal) from the mesh and parameters.
  * Returns ADCGPU_E_UNSUPPORTED for configurations outside the implemented
- * subset (hybrid friction, 3D, sponge, subgrid, weirs). */
+ * subset (hybrid friction, 3D, sponge, subgrid, weirs, flow-boundary codes
+ * above 29).  Specified-flux codes (2, 12, 22) carry a zero flux until
+ * adcgpu_set_flux_boundary. */
 int adcgpu_create(const adcgpu_mesh *mesh, const adcgpu_params *prm, int device,
                   adcgpu_ctx **out);
 void adcgpu_destroy(adcgpu_ctx *ctx);
@@ -225,6 +227,35 @@ int adcgpu_set_tide_potential(adcgpu_ctx *ctx, int ntif, const double *tpk, cons
                               const double *etrf, const double *fft, const double *facet,
                               const double *slam, const double *sfea, int nramp,
                               double dramptip, double statim, double reftim);
+
+/* Specified normal flux boundaries (NFLUXF = 1; LBCODEI 2, 12 and 22 among the
+ * land codes).  bndlen2o3: BNDLEN2O3(NVEL), two thirds of each flow-boundary
+ * edge to the next entry, zero after each segment's last node, as the
+ * reference's mesh set-up forms it (src/mesh.F:2300-2303); NULL turns the
+ * flux boundaries off.  From then on each adcgpu_begin_step shifts
+ * QN0 <- QN1 <- QN2 and zeroes QN2 (src/timestep.F:605-612), the GWCE takes
+ * the flux line integrals of every boundary edge (src/gwce.F:1737-1834,
+ * codes up to 29) and momentum imposes -QN2/H2 as the normal velocity of
+ * codes 0-19 (src/momentum.F:810-848).  QN0..QN2 start at zero. */
+int adcgpu_set_flux_boundary(adcgpu_ctx *ctx, const double *bndlen2o3);
+/* This step's QN2(NVEL) (src/timestep.F:861-903: the caller's periodic or
+ * fort.20 flux, e.g. from adcgpu_flux_bc), host pointer; between
+ * adcgpu_begin_step and adcgpu_solve_gwce.  Entries whose code takes no
+ * specified flux should hold zero, as in the reference. */
+int adcgpu_set_flux(adcgpu_ctx *ctx, const double *qn2);
+/* QN2 rows for the resident runs (adcgpu_run, adcgpu_batch_run) of a context
+ * with flux boundaries: qn2_table[s*NVEL + I] is QN2 of the run's step
+ * it0 + s (host pointer, copied at once); a run of more steps than the table
+ * holds is refused. */
+int adcgpu_set_flux_table(adcgpu_ctx *ctx, int nsteps, const double *qn2_table);
+/* Periodic specified normal flux at one time (src/timestep.F:861-878, ICS = 1):
+ * QN2(I) = sum over constituents J of QNAM(J,I)*FFF(J)*RampExtFlux*
+ * cos(FAMIG(J)*(timeh - NCYC*FPER(J)) + FFACE(J) - QNPH(J,I)) at the entries
+ * whose code is 2, 12, 22 or 52, zero elsewhere; qnam/qnph are (NFFR, NVEL)
+ * column-major, fface/qnph in radians.  Host computation (C library cos). */
+int adcgpu_flux_bc(int nvel, const int32_t *lbcodei, int nffr, const double *famig,
+                   const double *fff, const double *fface, const double *qnam,
+                   const double *qnph, double timeh, double rampextflux, double *out);
 
 /* ---- batches: several contexts stepped together (ensembles, uneven meshes)
  * No counterpart in the reference (an ensemble there is separate runs of

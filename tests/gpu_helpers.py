@@ -19,6 +19,8 @@ def golden_tables(d):
                            "csr", "nbd", "nbv", "lbcodei", "csii", "siii", "me2gw")}
     for k in ("np", "ne", "mnei", "neta", "nvel", "nvelme"):
         t[k] = int(d[k])
+    if "bndlen2o3" in d:
+        t["bndlen2o3"] = d["bndlen2o3"]
     return t
 
 
@@ -29,7 +31,8 @@ def build_pair(d, run, order=oracle_lib.BLOCKED):
     emo = np.asarray(d["emo"]).reshape(t["neta"], nbfr) if t["neta"] else None
     efa = np.asarray(d["efa"]).reshape(t["neta"], nbfr) if t["neta"] else None
     g = Model(t, run, tau0var=d["tau0var"], evm=d["evm"], fric=d["fric"],
-              corif=np.full(np_, run.cori), emo=emo, efa=efa)
+              corif=np.full(np_, run.cori), emo=emo, efa=efa, qnam=d.get("qnam"),
+              qnph=d.get("qnph"))
     g.cold_start(d["eta2_0"], d["nodecode0"], d["noff0"])
     o = oracle_lib.OracleModel(d, run, order)
     if run.holland is not None:

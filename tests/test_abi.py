@@ -59,3 +59,34 @@ def test_elevation_bc_matches_oracle(beach_golden):
         ref = np.zeros(neta)
         o.lib.adco_eta_bc(C.byref(o.setup), timeh, ramp, P(ref))
         assert np.array_equal(out, ref)
+
+
+def test_flux_bc_matches_oracle(basin_river_golden):
+    """Host periodic flux forcing (src/timestep.F:861-878) equals the oracle's,
+    bit for bit, at the steps the fixture's run takes and at long times (the
+    phase reduced by whole periods)."""
+    import math
+
+    import oracle_lib
+    from adcirc_amd import meshgen
+    from adcirc_amd.model import flux_amplitudes
+    d = basin_river_golden
+    run = meshgen.basin_river().run
+    o = oracle_lib.OracleModel(d, run)
+    lib = _lib.load()
+    lbc = np.ascontiguousarray(d["lbcodei"], np.int32)
+    am, ph = flux_amplitudes(lbc, d["qnam"], d["qnph"])
+    c = [np.array([x[k] for x in run.fluxf]) for k in range(3)]
+    c[2] = c[2] * (3.141592653589793 / 180.0)
+    P = lambda a: a.ctypes.data_as(_lib.dp_)
+    for it in (1, 2, 900, 1800, 123457):
+        out = np.zeros(lbc.shape[0])
+        timeh = it * run.dtdp
+        ramp = math.tanh((2.0 * timeh / 86400.0) / run.drampelev)
+        assert lib.adcgpu_flux_bc(lbc.shape[0], lbc.ctypes.data_as(_lib.ip_), len(run.fluxf),
+                                  P(c[0]), P(c[1]), P(c[2]), P(am), P(ph), timeh,
+                                  float(ramp), P(out)) == 0
+        assert np.array_equal(out, o.flux_bc(it)), it
+        assert np.count_nonzero(out) == np.isin(lbc, (2, 12, 22)).sum()
+    # the amplitudes land on the flux entries only, in NBV order
+    assert np.array_equal(am[np.isin(lbc, (2, 12, 22))], d["qnam"])
