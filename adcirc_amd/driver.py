@@ -31,7 +31,14 @@ import numpy as np
 from . import fort14, fort15, outputs, prep
 from .model import Model, cold_wetdry_state
 
-FLUX_TYPES = (2, 12, 22, 32, 52)
+FLUX_CODES = (2, 12, 22)
+
+
+def flux_entries(mesh) -> int:
+    """Flow-boundary entries of fort.14's specified-flux segments: the lines
+    per constituent of fort.15's NFFR block.  The segments' types fix the
+    count (the sharp-angle rule turns 2 into 12, still a flux code)."""
+    return sum(int(v.shape[0]) for ib, v in mesh.land_bnd if ib in FLUX_CODES)
 
 
 def _check(f: fort15.Fort15, holland) -> None:
@@ -64,15 +71,18 @@ def run_directory(case_dir: str, out_dir: Optional[str] = None,
     os.makedirs(out_dir, exist_ok=True)
     mesh = fort14.read_fort14(os.path.join(case_dir, "fort.14"))
     neta = sum(int(s.shape[0]) for s in mesh.open_bnd)
-    flux = any(ib in FLUX_TYPES for ib, _ in mesh.land_bnd)
-    f = fort15.read_fort15(os.path.join(case_dir, "fort.15"), neta, flux)
+    bad = sorted({ib for ib, _ in mesh.land_bnd} - set(prep.FLOW_TYPES))
+    if bad:
+        raise fort15.Fort15Error(f"fort.14: flow boundary types {bad} are not supported "
+                                 f"(supported: {', '.join(map(str, prep.FLOW_TYPES))})")
+    f = fort15.read_fort15(os.path.join(case_dir, "fort.15"), neta, flux_entries(mesh))
     _check(f, holland)
     run = f.run
     if f.nws != 0:
         run.holland = tuple(float(v) for v in holland)
     tables = prep.build_tables(mesh, run.nolifa, run.h0, run.anginn)
     eta2, nodecode, noff = cold_wetdry_state(tables, run)
-    g = Model(tables, run, emo=f.emo, efa=Model.radians(f.efa))
+    g = Model(tables, run, emo=f.emo, efa=Model.radians(f.efa), qnam=f.qnam, qnph=f.qnph)
     try:
         g.cold_start(eta2, nodecode, noff)
         if run.holland is not None:

@@ -64,7 +64,8 @@ def write_fort14(path: str, m: Mesh) -> None:
 
 def read_fort14(path: str) -> Mesh:
     """Read the subset of fort.14 this package supports (IBTYPE 0/10/20 land
-    boundaries, elevation-specified open boundaries)."""
+    and 2/12/22 specified-flux boundaries, elevation-specified open
+    boundaries)."""
     with open(path) as f:
         tokens_lines = f.read().splitlines()
     title = tokens_lines[0]

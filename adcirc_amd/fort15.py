@@ -15,8 +15,10 @@ line: IM other than 0, 111111 (the same defaults spelled out) or 111112
 (NWP > 0); the tidal potential with self-attraction (NTIP = 2, fort.24); a
 tidal-potential ramp other than the elevation ramp; met forcing other than NWS = 0 or the
 parametric vortex NWS = 8; TAU0 schemes other than a constant, -1 and -2;
-hybrid friction (NOLIBF 2, 3); NRS/ice variants of NWS; flux boundaries
-(NFFR); station lists read from a separate file (NSTA < 0).
+hybrid friction (NOLIBF 2, 3); NRS/ice variants of NWS; normal flux read from
+fort.20 (NFFR 0 or -1; the periodic flux of NFFR > 0 on IBTYPE 2, 12, 22
+boundaries is read); a flux-forcing ramp other than the elevation ramp;
+station lists read from a separate file (NSTA < 0).
 """
 from __future__ import annotations
 
@@ -104,6 +106,11 @@ class Fort15:
     nfreq: int = 0
     nhstar: int = 0
     nhsinc: int = 0
+    # periodic normal flux (NFFR > 0): FBOUNTAG per constituent, QNAM/QNPH
+    # (degrees) per flux-boundary entry and constituent
+    fbountag: List[str] = dataclasses.field(default_factory=list)
+    qnam: Optional[np.ndarray] = None
+    qnph: Optional[np.ndarray] = None
 
 
 _NUM = re.compile(r"[\s,]+")
@@ -171,10 +178,12 @@ def _stations(r: _Lines, what: str) -> np.ndarray:
     return np.array([r.floats(2, f"{what} station") for _ in range(n)]).reshape(n, 2)
 
 
-def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
+def read_fort15(path: str, neta: int, flux_entries: int = 0) -> Fort15:
     """Read fort.15 for a mesh with ``neta`` elevation-boundary nodes (NETA of
-    fort.14); ``flux_boundaries`` says whether fort.14 has IBTYPE 2, 12, 22,
-    32 or 52 boundaries (whose NFFR block is not supported)."""
+    fort.14) and ``flux_entries`` flow-boundary entries of IBTYPE 2, 12 or 22
+    (the NVEL entries whose QNAM/QNPH lines the NFFR block holds,
+    src/read_input.F:3562-3586; 0: fort.14 has no flux boundaries and no
+    NFFR block is read)."""
     with open(path) as f:
         r = _Lines(f.read(), path)
     rundes = r.raw().rstrip("\n")[:80].strip()
@@ -229,6 +238,9 @@ def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
     drampelev = ramps[4] if nramp >= 4 else ramps[0]
     if ntip and nramp >= 5 and ramps[5] != drampelev:
         r.fail("DRAMPTip differs from DRAMPElev: one ramp length is supported")
+    if flux_entries and nramp >= 2 and (ramps[1] != drampelev or ramps[2] != 0.0):
+        r.fail("DRAMPExtFlux differs from DRAMPElev, or FluxSettlingTime is not zero: "
+               "one ramp length is supported")
     a00, b00, c00 = r.floats(3, "A00 B00 C00")
     if nolifa in (2, 3):
         v = r.values(4, "H0 NODEDRYMIN NODEWETMIN VELMIN")
@@ -267,8 +279,21 @@ def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
         for j in range(neta):
             emo[j, k], efa[j, k] = r.floats(2, "EMO EFA")
     anginn = r.float1("ANGINN")
-    if flux_boundaries:
-        r.fail("flux boundaries (NFFR) are not supported")
+    fluxf, fbountag, qnam, qnph = [], [], None, None
+    if flux_entries:
+        # src/read_input.F:3503-3588
+        nffr = r.int1("NFFR")
+        if nffr <= 0:
+            r.fail(f"NFFR = {nffr}: normal flux from fort.20 is not supported (NFFR > 0)")
+        for _ in range(nffr):
+            fbountag.append(r.raw().strip()[:5])
+            fluxf.append(tuple(r.floats(3, "FAMIGT FFF FFACE")))
+        qnam = np.zeros((flux_entries, nffr))
+        qnph = np.zeros((flux_entries, nffr))
+        for k in range(nffr):
+            r.raw()  # ALPHAQ
+            for j in range(flux_entries):
+                qnam[j, k], qnph[j, k] = r.floats(2, "QNAM QNPH")
     elev_sta = _output(r, "NOUTE TOUTSE TOUTFE NSPOOLE")
     st_e = _stations(r, "NSTAE")
     vel_sta = _output(r, "NOUTV TOUTSV TOUTFV NSPOOLV")
@@ -297,13 +322,14 @@ def read_fort15(path: str, neta: int, flux_boundaries: bool = False) -> Fort15:
                     cori=cori, h0=h0, velmin=velmin, a00=a00, b00=b00, c00=c00, itmax=itmax,
                     convcr=convcr, ilump=ilump, anginn=anginn, g=g, amig=tuple(amig),
                     ff=tuple(ff), face=tuple(face), holland=None, tipot=tuple(tipot),
-                    slam0=slam0, sfea0=sfea0)
+                    slam0=slam0, sfea0=sfea0, fluxf=tuple(fluxf))
     return Fort15(rundes, runid, run, emo, efa, tags, elev, vel, nfover=nfover, nabout=nabout,
                   nscreen=nscreen, ihot=ihot, ics=ics, im=im, nws=nws, nramp=nramp, ncor=ncor,
                   ntip=ntip, slam0=slam0, sfea0=sfea0, ititer=ititer, isldia=isldia,
                   nodedrymin=ndry, nodewetmin=nwet, vortex_line=vortex, elev_sta=elev_sta,
                   vel_sta=vel_sta, met_sta=met_sta, met=met, stations_e=st_e, stations_v=st_v,
-                  stations_m=st_m, nfreq=nfreq, nhstar=nhstar, nhsinc=nhsinc)
+                  stations_m=st_m, nfreq=nfreq, nhstar=nhstar, nhsinc=nhsinc,
+                  fbountag=fbountag, qnam=qnam, qnph=qnph)
 
 
 def _g(v: float) -> str:
@@ -376,6 +402,16 @@ def write_fort15(path: str, f: Fort15) -> None:
         for j in range(f.emo.shape[0]):
             L.append(f"{_g(f.emo[j, k])} {_g(f.efa[j, k])}")
     add([r.anginn], "ANGINN")
+    if r.fluxf:
+        add([len(r.fluxf)], "NFFR")
+        ftags = list(f.fbountag) + [f"Q{k + 1}" for k in range(len(f.fbountag), len(r.fluxf))]
+        for k, c in enumerate(r.fluxf):
+            L.append(ftags[k][:5])
+            add(list(c), "FAMIGT FFF FFACE")
+        for k in range(len(r.fluxf)):
+            L.append(ftags[k][:10])
+            for j in range(f.qnam.shape[0]):
+                L.append(f"{_g(f.qnam[j, k])} {_g(f.qnph[j, k])}")
 
     def out(o: Output, name: str):
         add([o.nout, o.touts, o.toutf, o.nspool], name)
@@ -415,4 +451,5 @@ def from_case(case, elev: Optional[Output] = None, vel: Optional[Output] = None,
                   np.asarray(case.emo, np.float64), np.asarray(case.efa, np.float64),
                   [f"F{k + 1}" for k in range(r.nbfr)], elev or Output(), vel or Output(),
                   nws=r.nws, nramp=r.nramp, ititer=-1 if r.ilump else 1,
-                  im=111112 if r.ilump else 0)
+                  im=111112 if r.ilump else 0, ntip=r.ntip, slam0=r.slam0, sfea0=r.sfea0,
+                  qnam=case.qnam, qnph=case.qnph)

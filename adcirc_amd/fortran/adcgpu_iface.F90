@@ -139,6 +139,28 @@ module adcgpu_iface
        real(c_double), value :: timeh, rampelev
        real(c_double), intent(out) :: out(*)
      end function
+     ! specified normal flux boundaries: BNDLEN2O3 once, then each step's QN2
+     ! (src/timestep.F:861-903) after adcgpu_begin_step
+     integer(c_int) function adcgpu_set_flux_boundary(ctx, bndlen2o3) &
+          bind(C, name="adcgpu_set_flux_boundary")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: bndlen2o3(*)
+     end function
+     integer(c_int) function adcgpu_set_flux(ctx, qn2) bind(C, name="adcgpu_set_flux")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: qn2(*)
+     end function
+     integer(c_int) function adcgpu_flux_bc(nvel, lbcodei, nffr, famig, fff, fface, qnam, &
+          qnph, timeh, rampextflux, out) bind(C, name="adcgpu_flux_bc")
+       import :: c_int, c_double
+       integer(c_int), value :: nvel, nffr
+       integer(c_int), intent(in) :: lbcodei(*)
+       real(c_double), intent(in) :: famig(*), fff(*), fface(*), qnam(*), qnph(*)
+       real(c_double), value :: timeh, rampextflux
+       real(c_double), intent(out) :: out(*)
+     end function
   end interface
 
 contains

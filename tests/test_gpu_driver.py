@@ -30,12 +30,12 @@ def _per_step_model(d, holland):
     from the files the driver read."""
     mesh = fort14.read_fort14(os.path.join(d, "fort.14"))
     neta = sum(s.shape[0] for s in mesh.open_bnd)
-    f = fort15.read_fort15(os.path.join(d, "fort.15"), neta)
+    f = fort15.read_fort15(os.path.join(d, "fort.15"), neta, driver.flux_entries(mesh))
     run = f.run
     run.holland = holland
     t = prep.build_tables(mesh, run.nolifa, run.h0, run.anginn)
     eta2, nc, noff = cold_wetdry_state(t, run)
-    g = Model(t, run, emo=f.emo, efa=Model.radians(f.efa))
+    g = Model(t, run, emo=f.emo, efa=Model.radians(f.efa), qnam=f.qnam, qnph=f.qnph)
     g.cold_start(eta2, nc, noff)
     if holland is not None:
         g.holland(0.0, holland)
@@ -46,11 +46,12 @@ def _expect(vals, nodecode):
     return np.array([float(outputs.e20(v)) for v in np.where(nodecode == 0, -99999.0, vals)])
 
 
-@pytest.mark.parametrize("name", ["basin", "beach_storm"])
+@pytest.mark.parametrize("name", ["basin", "beach_storm", "basin_river"])
 def test_driver_outputs_equal_per_step_state(tmp_path, name):
     d = str(tmp_path / name)
-    if name == "basin":
-        case, n, hol = meshgen.basin(), 600, None
+    if name in ("basin", "basin_river"):
+        case = meshgen.basin() if name == "basin" else meshgen.basin_river()
+        n, hol = 600, None
         elev, vel = fort15.Output(1, 0.0, 1.0, 200), fort15.Output(1, 0.0, 1.0, 300)
     else:
         case, n = meshgen.beach(), 900

@@ -33,15 +33,21 @@ def test_number_formats_match_flang():
         assert outputs.record_header(t, k) == line + "\n"
 
 
-@pytest.mark.parametrize("name", ["basin", "beach", "beach_lumped"])
+@pytest.mark.parametrize("name", ["basin", "beach", "beach_lumped", "basin_river"])
 def test_fort15_roundtrip(tmp_path, name):
-    case = meshgen.beach() if name.startswith("beach") else meshgen.basin()
+    case = (meshgen.beach() if name.startswith("beach") else
+            meshgen.basin_river() if name == "basin_river" else meshgen.basin())
     if name == "beach_lumped":
         case.run.ilump = 1
     f = fort15.from_case(case, fort15.Output(1, 0.0, 1.0, 300), fort15.Output(-1, 0.5, 2.0, 60))
     p = str(tmp_path / "fort.15")
     fort15.write_fort15(p, f)
-    g = fort15.read_fort15(p, case.emo.shape[0])
+    from adcirc_amd import driver
+    g = fort15.read_fort15(p, case.emo.shape[0], driver.flux_entries(case.mesh))
+    if case.qnam is not None:
+        # the NFFR block: one line per entry of the IBTYPE 2 and 22 segments
+        assert np.array_equal(g.qnam, case.qnam) and np.array_equal(g.qnph, case.qnph)
+        assert g.qnam.shape[0] == 20 and len(g.run.fluxf) == 2
     a, b = dataclasses.asdict(case.run), dataclasses.asdict(g.run)
     a.pop("holland"), b.pop("holland")
     assert a == b
