@@ -34,7 +34,7 @@ EXPORTS = ("adcgpu_create", "adcgpu_destroy", "adcgpu_last_error", "adcgpu_set_f
            "adcgpu_ml_collect", "adcgpu_ml_samples", "adcgpu_ml_train", "adcgpu_ml_grad",
            "adcgpu_ml_features", "adcgpu_ml_correction", "adcgpu_state_bytes",
            "adcgpu_state_save", "adcgpu_state_load", "adcgpu_last_solve",
-           "adcgpu_set_jcg_blocks", "adcgpu_set_tide_potential", "adcgpu_set_flux_boundary",
+           "adcgpu_set_jcg_blocks", "adcgpu_set_tide_potential", "adcgpu_set_hybrid_friction", "adcgpu_set_flux_boundary",
            "adcgpu_set_flux", "adcgpu_set_flux_table", "adcgpu_flux_bc", "adcgpu_batch_create", "adcgpu_batch_run", "adcgpu_batch_destroy")
 ML_NPARAM = 22913
 # error codes (ADCGPU_E_*)
@@ -106,6 +106,7 @@ def load() -> C.CDLL:
     lib.adcgpu_set_jcg_blocks.argtypes = [vp, C.c_int]
     lib.adcgpu_set_tide_potential.argtypes = [vp, C.c_int, dp_, dp_, dp_, dp_, dp_, dp_, dp_,
                                               C.c_int, C.c_double, C.c_double, C.c_double]
+    lib.adcgpu_set_hybrid_friction.argtypes = [vp, C.c_double, C.c_double, C.c_double]
     lib.adcgpu_set_flux_boundary.argtypes = [vp, dp_]
     lib.adcgpu_set_flux.argtypes = [vp, dp_]
     lib.adcgpu_set_flux_table.argtypes = [vp, C.c_int, dp_]

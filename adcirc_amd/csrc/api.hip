@@ -66,6 +66,7 @@ struct adcgpu_ctx {
   int np = 0, ne = 0, mnei = 0, neta = 0, nch = 0;
   adcgpu_params prm{};
   double ifnlfa = 1, ifnlct = 1, ifnlcat = 1, iflinbf = 0, ifnlbf = 1, tkmax = 1;
+  Hybrid hyb{0.0, 1.0, 10.0, 1.0 / 3.0};  // hybrid friction (NOLIBF = 2)
   double habsmin = 0, hoff = 0, errorelev = 1000.0;
   // mesh
   int *nm = nullptr, *inc_ptr = nullptr, *inc = nullptr, *slot = nullptr, *nneigh = nullptr;
@@ -510,7 +511,6 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     return fail(ADCGPU_E_CAPACITY, "mesh too large for 32-bit offsets");
   if (mnei > LHS_MAXW)
     return fail(ADCGPU_E_UNSUPPORTED, "MNEI above " + std::to_string(LHS_MAXW));
-  if (P->nolibf == 2) return fail(ADCGPU_E_UNSUPPORTED, "hybrid bottom friction (NOLIBF=2)");
   if (P->nolibf < 0 || P->nolibf > 3) return fail(ADCGPU_E_ARG, "NOLIBF");
   if (P->itmax <= 0 || P->itmax > JCG_MAX_ITMAX) return fail(ADCGPU_E_ARG, "ITMAX out of range");
   if (P->precision != ADCGPU_FP64 && P->precision != ADCGPU_MIXED)
@@ -697,6 +697,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   c->ifnlcat = P->nolicat == 0 ? 0.0 : 1.0;
   c->iflinbf = P->nolibf == 0 ? 1.0 : 0.0;
   c->ifnlbf = (P->nolibf == 1 || P->nolibf == 3) ? 1.0 : 0.0;
+  c->hyb.on = P->nolibf == 2 ? 1.0 : 0.0;  // IFHYBF (src/nodalattr.F:1984-1988)
   c->tkmax = (double)2.0f / P->dtdp;
   c->habsmin = 0.8 * P->h0;
   c->hoff = 1.2 * P->h0;
@@ -1088,7 +1089,7 @@ int adcgpu_set_flux(adcgpu_ctx* c, const double* qn2) {
 
 static FricPrm fric_prm(const adcgpu_ctx* c) {
   return FricPrm{c->uu1, c->vv1, c->dp, c->eta2, c->fric, c->ifnlfa, c->iflinbf, c->ifnlbf,
-                 c->tkmax};
+                 c->tkmax, c->hyb};
 }
 
 int adcgpu_friction(adcgpu_ctx* c) {
@@ -1286,7 +1287,7 @@ static int wetdry_device(adcgpu_ctx* c, bool alpha_pass) {
   }
   hipLaunchKernelGGL(k_wd_d1, dim3(nblk(np, 256)), dim3(256), 0, c->stream, c->mesh(),
                      c->state(), c->flags, c->prm.h0, c->habsmin);
-  WdPrm q{c->hoff, c->prm.velmin, c->iflinbf, c->ifnlbf, c->tkmax};
+  WdPrm q{c->hoff, c->prm.velmin, c->iflinbf, c->ifnlbf, c->tkmax, c->hyb};
   const int nb = nblk(ne, 256);
   hipLaunchKernelGGL(k_wd_elem, dim3(nb), dim3(256), 0, c->stream, c->mesh(), c->state(), q,
                      c->wd);
@@ -1701,6 +1702,17 @@ int adcgpu_set_jcg_blocks(adcgpu_ctx* c, int blocks) {
     (void)set_jcg_grid(c);
     return rc;
   }
+  return 0;
+}
+
+int adcgpu_set_hybrid_friction(adcgpu_ctx* c, double hbreak, double ftheta, double fgamma) {
+  if (!c) return fail(ADCGPU_E_ARG, "null context");
+  if (!(hbreak > 0.0) || !(ftheta > 0.0) || !(fgamma >= 0.0) || !std::isfinite(hbreak) ||
+      !std::isfinite(ftheta) || !std::isfinite(fgamma))
+    return fail(ADCGPU_E_ARG, "HBREAK and FTHETA must be positive, FGAMMA non-negative");
+  c->hyb.hbreak = hbreak;
+  c->hyb.ftheta = ftheta;
+  c->hyb.fgamma = fgamma;
   return 0;
 }
 

@@ -181,8 +181,23 @@ struct MomPrm {
   int* flags;        // run-time error flags (ERR_ELEV, ERR_NONFINITE, F_ERRSTEP)
 };
 
+// Hybrid bottom friction (NOLIBF = 2): IFHYBF and fort.15's HBREAK, FTHETA,
+// FGAMMA; `on` is 0 for the other friction laws.
+struct Hybrid {
+  double on, hbreak, ftheta, fgamma;
+};
+
+// IFHYBF*(1 + (HBREAK/H)**FTHETA)**(FGAMMA/FTHETA) (src/nodalattr.F:2406-2410,
+// src/wetdry.F:415-418, 447-451): the depth factor of the hybrid law, zero
+// without it.  The device's pow differs from the C library's in the last bit
+// for some arguments.
+__device__ __forceinline__ double hybrid_term(const Hybrid& h, double H) {
+  return h.on != 0.0 ? h.on * pow(1.0 + pow(h.hbreak / H, h.ftheta), h.fgamma / h.ftheta) : 0.0;
+}
+
 struct WdPrm {
   double hoff, velmin, iflinbf, ifnlbf, tkmax;
+  Hybrid hyb;
 };
 
 // Wetting candidates of one step (src/wetdry.F:368-475 and its two twins).
@@ -216,6 +231,7 @@ struct HollandPrm {
 struct FricPrm {
   const double *uu1, *vv1, *dp, *eta2, *fric;
   double ifnlfa, iflinbf, ifnlbf, tkmax;
+  Hybrid hyb;
 };
 
 struct JcgArgs {

@@ -44,14 +44,13 @@ __global__ __launch_bounds__(256) void k_tide_pot(int np, TipPrm p, const double
   tip2[i] = v;
 }
 
-// Apply2DBottomFriction (src/nodalattr.F:2400-2411) for IFHYBF = 0, and the
-// TKM = (TK, TK, 0) of Apply2DInternalWaveDrag without wave drag; TKM is
-// represented by TK alone.
+// Apply2DBottomFriction (src/nodalattr.F:2400-2411), and the TKM = (TK, TK, 0)
+// of Apply2DInternalWaveDrag without wave drag; TKM is represented by TK alone.
 __device__ __forceinline__ void friction_node(int i, const FricPrm& f, double* __restrict__ tk) {
   double u = f.uu1[i], v = f.vv1[i];
   double uv1 = __builtin_sqrt(u * u + v * v);
   double h1 = f.dp[i] + f.ifnlfa * f.eta2[i];
-  double hyb = 0.0;  // IFHYBF * (...) with IFHYBF = 0
+  const double hyb = hybrid_term(f.hyb, h1);
   double a = f.fric[i] * (f.iflinbf + (uv1 / h1) * (f.ifnlbf + hyb));
   tk[i] = fmin(a, f.tkmax);
 }
@@ -993,11 +992,12 @@ __global__ __launch_bounds__(256) void k_wd_elem(DevMesh m, DevState s, WdPrm q,
         if (deleta < 0.0) deleta = 0.0;
         const double H1 = s.eta2[nm123] + m.dp[nm123];
         const double fr = s.fric[nm123];
-        const double tkwet = fmax(fr * (q.iflinbf + (q.velmin / H1) * (q.ifnlbf + 0.0)), 0.0001);
+        const double hyb = hybrid_term(q.hyb, H1);
+        const double tkwet = fmax(fr * (q.iflinbf + (q.velmin / H1) * (q.ifnlbf + hyb)), 0.0001);
         const double vel = m.g * (deleta / deldist) / tkwet;
         if (vel > q.velmin) {
           s.nnodecode[nc] = 1;
-          w.tkc[e] = fmin(fr * (q.iflinbf + (1.0 * vel / H1) * (q.ifnlbf + 0.0)), q.tkmax);
+          w.tkc[e] = fmin(fr * (q.iflinbf + (1.0 * vel / H1) * (q.ifnlbf + hyb)), q.tkmax);
           w.lc[e] = l123 | ((l123 == b ? a : b) << 2);
           cand = true;
         }

@@ -15,7 +15,7 @@ line: IM other than 0, 111111 (the same defaults spelled out) or 111112
 (NWP > 0); the tidal potential with self-attraction (NTIP = 2, fort.24); a
 tidal-potential ramp other than the elevation ramp; met forcing other than NWS = 0 or the
 parametric vortex NWS = 8; TAU0 schemes other than a constant, -1 and -2;
-hybrid friction (NOLIBF 2, 3); NRS/ice variants of NWS; normal flux read from
+the 3D-coupled friction of NOLIBF = 3; NRS/ice variants of NWS; normal flux read from
 fort.20 (NFFR 0 or -1; the periodic flux of NFFR > 0 on IBTYPE 2, 12, 22
 boundaries is read); a flux-forcing ramp other than the elevation ramp;
 station lists read from a separate file (NSTA < 0).
@@ -248,12 +248,16 @@ def read_fort15(path: str, neta: int, flux_entries: int = 0) -> Fort15:
     else:
         h0, ndry, nwet, velmin = r.float1("H0"), 0, 0, 0.01
     slam0, sfea0 = r.floats(2, "SLAM0 SFEA0")
+    hbreak, ftheta, fgamma = 1.0, 10.0, 1.0 / 3.0
     if nolibf == 0:
         cf = r.float1("TAU")
     elif nolibf == 1:
         cf = r.float1("CF")
+    elif nolibf == 2:
+        # src/read_input.F: CF (the minimum), HBREAK, FTHETA, FGAMMA
+        cf, hbreak, ftheta, fgamma = r.floats(4, "CF HBREAK FTHETA FGAMMA")
     else:
-        r.fail(f"NOLIBF = {nolibf}: hybrid friction is not supported")
+        r.fail(f"NOLIBF = {nolibf}: supported are 0, 1 and 2")
     eslm = r.float1("ESLM")
     if eslm < 0:
         r.fail("ESLM < 0 (Smagorinsky viscosity) is not supported")
@@ -322,7 +326,8 @@ def read_fort15(path: str, neta: int, flux_entries: int = 0) -> Fort15:
                     cori=cori, h0=h0, velmin=velmin, a00=a00, b00=b00, c00=c00, itmax=itmax,
                     convcr=convcr, ilump=ilump, anginn=anginn, g=g, amig=tuple(amig),
                     ff=tuple(ff), face=tuple(face), holland=None, tipot=tuple(tipot),
-                    slam0=slam0, sfea0=sfea0, fluxf=tuple(fluxf))
+                    slam0=slam0, sfea0=sfea0, fluxf=tuple(fluxf), hbreak=hbreak,
+                    ftheta=ftheta, fgamma=fgamma)
     return Fort15(rundes, runid, run, emo, efa, tags, elev, vel, nfover=nfover, nabout=nabout,
                   nscreen=nscreen, ihot=ihot, ics=ics, im=im, nws=nws, nramp=nramp, ncor=ncor,
                   ntip=ntip, slam0=slam0, sfea0=sfea0, ititer=ititer, isldia=isldia,
@@ -385,7 +390,10 @@ def write_fort15(path: str, f: Fort15) -> None:
     else:
         add([r.h0], "H0")
     add([f.slam0, f.sfea0], "SLAM0 SFEA0")
-    add([r.cf], "TAU" if r.nolibf == 0 else "CF")
+    if r.nolibf == 2:
+        add([r.cf, r.hbreak, r.ftheta, r.fgamma], "CF HBREAK FTHETA FGAMMA")
+    else:
+        add([r.cf], "TAU" if r.nolibf == 0 else "CF")
     add([r.eslm], "ESLM")
     add([r.cori], "CORI")
     add([len(r.tipot)], "NTIF")

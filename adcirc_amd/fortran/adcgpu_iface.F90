@@ -139,6 +139,13 @@ module adcgpu_iface
        real(c_double), value :: timeh, rampelev
        real(c_double), intent(out) :: out(*)
      end function
+     ! hybrid bottom friction (NOLIBF = 2): fort.15's HBREAK, FTHETA, FGAMMA
+     integer(c_int) function adcgpu_set_hybrid_friction(ctx, hbreak, ftheta, fgamma) &
+          bind(C, name="adcgpu_set_hybrid_friction")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: hbreak, ftheta, fgamma
+     end function
      ! specified normal flux boundaries: BNDLEN2O3 once, then each step's QN2
      ! (src/timestep.F:861-903) after adcgpu_begin_step
      integer(c_int) function adcgpu_set_flux_boundary(ctx, bndlen2o3) &

@@ -42,7 +42,11 @@ class RunConfig:
     nolica: int = 1
     nolicat: int = 1
     nolibf: int = 1               # 0 linear, 1 quadratic, 2 hybrid
-    cf: float = 0.0025
+    cf: float = 0.0025            # CF (NOLIBF 1), CFmin (2), TAU (0)
+    # hybrid friction (NOLIBF = 2): break depth (m) and the two exponents
+    hbreak: float = 1.0
+    ftheta: float = 10.0
+    fgamma: float = 1.0 / 3.0
     tau0: float = 0.005
     eslm: float = 5.0
     cori: float = 1.0e-4

@@ -201,6 +201,9 @@ class Model:
             from .prep import lonlat_cpp
             slam, sfea = lonlat_cpp(t["x"], t["y"], run.slam0, run.sfea0)
             self.set_tide_potential(run.tipot, slam, sfea, run)
+        if run.nolibf == 2:
+            check(lib.adcgpu_set_hybrid_friction(self.h, float(run.hbreak), float(run.ftheta),
+                                                 float(run.fgamma)), "set_hybrid_friction")
         self.flux = None
         if getattr(run, "fluxf", ()):
             if qnam is None or qnph is None or "bndlen2o3" not in t:

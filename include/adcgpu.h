@@ -117,8 +117,7 @@ enum adcgpu_field {
 
 /* Create a context on `device` (HIP ordinal) from the mesh and parameters.
  * Returns ADCGPU_E_UNSUPPORTED for configurations outside the implemented
- * subset (hybrid friction, 3D, sponge, subgrid, weirs, flow-boundary codes
- * above 29).  Specified-flux codes (2, 12, 22) carry a zero flux until
+ * subset (3D, sponge, subgrid, weirs, flow-boundary codes above 29).  Specified-flux codes (2, 12, 22) carry a zero flux until
  * adcgpu_set_flux_boundary. */
 int adcgpu_create(const adcgpu_mesh *mesh, const adcgpu_params *prm, int device,
                   adcgpu_ctx **out);
@@ -227,6 +226,15 @@ int adcgpu_set_tide_potential(adcgpu_ctx *ctx, int ntif, const double *tpk, cons
                               const double *etrf, const double *fft, const double *facet,
                               const double *slam, const double *sfea, int nramp,
                               double dramptip, double statim, double reftim);
+
+/* Hybrid bottom friction (NOLIBF = 2): fort.15's HBREAK, FTHETA, FGAMMA, the
+ * break depth and exponents of FRIC*(1 + (HBREAK/H)**FTHETA)**(FGAMMA/FTHETA)
+ * (src/nodalattr.F:2400-2411; the wetting criterion's friction too,
+ * src/wetdry.F:415-451).  A context created with NOLIBF = 2 starts from
+ * HBREAK = 1, FTHETA = 10, FGAMMA = 1/3.  The device's pow can differ from
+ * the C library's in the last bit, so hybrid runs agree with the reference
+ * to rounding, not bit for bit. */
+int adcgpu_set_hybrid_friction(adcgpu_ctx *ctx, double hbreak, double ftheta, double fgamma);
 
 /* Specified normal flux boundaries (NFLUXF = 1; LBCODEI 2, 12 and 22 among the
  * land codes).  bndlen2o3: BNDLEN2O3(NVEL), two thirds of each flow-boundary

@@ -94,6 +94,8 @@ typedef struct {
   const double *eta2_0;
   /* GWCE LHS: 0 consistent (JCG), 1 lumped diagonal (explicit solve) */
   int ilump;
+  /* hybrid friction (NOLIBF = 2): HBREAK, FTHETA, FGAMMA of fort.15 */
+  double hbreak, ftheta, fgamma;
 } adco_setup;
 
 typedef struct adco_model adco_model;

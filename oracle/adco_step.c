@@ -125,9 +125,9 @@ adco_model *adco_create(const adco_setup *s, int order) {
   m->iflinbf = s->nolibf == 0 ? 1 : 0;
   m->ifnlbf = (s->nolibf == 1 || s->nolibf == 3) ? 1 : 0;
   m->ifhybf = s->nolibf == 2 ? 1 : 0;
-  m->hbreak = 1.0;
-  m->ftheta = 10.0;
-  m->fgamma = 1.0 / 3.0;
+  m->hbreak = s->hbreak;
+  m->ftheta = s->ftheta;
+  m->fgamma = s->fgamma;
   /* initializeWettingAndDrying, src/wetdry.F:114-130 */
   m->habsmin = 0.8 * s->h0;
   m->hoff = 1.2 * s->h0;

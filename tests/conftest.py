@@ -53,6 +53,21 @@ def basin_river_golden():
 
 
 @pytest.fixture(scope="session")
+def beach_hybrid_golden():
+    return load_golden("beach_hybrid")
+
+
+def beach_hybrid_run():
+    """The storm beach with hybrid friction (NOLIBF = 2: CFmin 0.0025,
+    HBREAK = 2 m, FTHETA = 10, FGAMMA = 1/3), as tests/golden/make_golden.py
+    runs it."""
+    from adcirc_amd import meshgen
+    run = meshgen.beach().run
+    run.nolibf, run.cf, run.hbreak, run.ftheta, run.fgamma = 2, 0.0025, 2.0, 10.0, 1.0 / 3.0
+    return run
+
+
+@pytest.fixture(scope="session")
 def basin_golden():
     return load_golden("basin")
 

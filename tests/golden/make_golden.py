@@ -8,7 +8,7 @@ in the build container only; the fixtures it writes are what the tests use
 everywhere else.
 
     python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [basin_linear] [jcg] [tau0]
-                                       [coastal20k] [basin_tide] [basin_river]
+                                       [coastal20k] [basin_tide] [basin_river] [beach_hybrid]
 """
 from __future__ import annotations
 
@@ -193,6 +193,13 @@ def make_tau0() -> str:
     return path
 
 
+def beach_hybrid():
+    case = meshgen.beach()
+    r = case.run
+    r.nolibf, r.cf, r.hbreak, r.ftheta, r.fgamma = 2, 0.0025, 2.0, 10.0, 1.0 / 3.0
+    return case
+
+
 def main(argv):
     todo = argv or ["basin", "beach", "beach_lumped", "basin_linear", "jcg", "tau0", "coastal20k"]
     if "basin" in todo:
@@ -215,6 +222,11 @@ def main(argv):
         # the basin with the tidal potential of M2, S2, K1, O1 (NTIP = 1) at
         # 75 W, 35 N; one simulated hour
         print(run_case("basin_tide", meshgen.basin_tide(), nsteps=1800, early=10,
+                       dump_every=600))
+    if "beach_hybrid" in todo:
+        # the wet/dry storm beach with hybrid friction (NOLIBF = 2: CFmin,
+        # HBREAK = 2 m, FTHETA = 10, FGAMMA = 1/3), half an hour
+        print(run_case("beach_hybrid", beach_hybrid(), nsteps=1800, early=10,
                        dump_every=600))
     if "basin_river" in todo:
         # the basin with two rivers (IBTYPE 2 and 22), steady and M2 flux;

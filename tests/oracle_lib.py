@@ -38,6 +38,7 @@ class Setup(C.Structure):
         ("nramp", C.c_int), ("nolifa", C.c_int), ("nolica", C.c_int), ("nolicat", C.c_int),
         ("nolibf", C.c_int), ("itmax", C.c_int), ("nws", C.c_int),
         ("nodecode0", _ip), ("noff0", _ip), ("eta2_0", _dp), ("ilump", C.c_int),
+        ("hbreak", C.c_double), ("ftheta", C.c_double), ("fgamma", C.c_double),
     ]
 
 
@@ -157,6 +158,9 @@ class OracleModel:
         s.nodecode0 = ia("nodecode0", d["nodecode0"]); s.noff0 = ia("noff0", d["noff0"])
         s.eta2_0 = da("eta2_0", d["eta2_0"])
         s.ilump = int(getattr(run, "ilump", 0))
+        s.hbreak = float(getattr(run, "hbreak", 1.0))
+        s.ftheta = float(getattr(run, "ftheta", 10.0))
+        s.fgamma = float(getattr(run, "fgamma", 1.0 / 3.0))
         self.setup = s
         self.run = run
         self.h = self.lib.adco_create(C.byref(s), order)

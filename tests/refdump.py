@@ -40,7 +40,8 @@ def write_inputs(case, workdir: str, nsteps: int, dump_every: int = 0,
         "dtdp": fl(r.dtdp), "statim": fl(r.statim), "reftim": fl(r.reftim),
         "nramp": str(r.nramp), "drampelev": fl(r.drampelev), "nolifa": str(r.nolifa),
         "nolica": str(r.nolica), "nolicat": str(r.nolicat), "nolibf": str(r.nolibf),
-        "cf": fl(r.cf), "tau0_in": fl(r.tau0), "eslm": fl(r.eslm), "cori": fl(r.cori),
+        "cf": fl(r.cf), "hbreak_in": fl(r.hbreak), "ftheta_in": fl(r.ftheta),
+        "fgamma_in": fl(r.fgamma), "tau0_in": fl(r.tau0), "eslm": fl(r.eslm), "cori": fl(r.cori),
         "h0": fl(r.h0), "velmin": fl(r.velmin), "a00": fl(r.a00), "b00": fl(r.b00),
         "c00": fl(r.c00), "itmax": str(r.itmax), "convcr": fl(r.convcr),
         "nws": str(r.nws), "nbfr": str(r.nbfr), "anginn_in": fl(r.anginn),

@@ -163,7 +163,7 @@ def test_fort15_documented_layout(tmp_path):
 
 @pytest.mark.parametrize("line,value,word", [
     (7, "2", "ICS"), (8, "1", "IM"), (8, "511112", "IM"), (13, "1", "NWP"), (15, "2", "NTIP"),
-    (16, "12", "NWS"), (19, "-3", "TAU0"), (9, "2", "NOLIBF")])
+    (16, "12", "NWS"), (19, "-3", "TAU0"), (9, "3", "NOLIBF")])
 def test_fort15_unsupported_options_raise(tmp_path, line, value, word):
     lines = DOCUMENTED.splitlines()
     lines[line - 1] = f"{value}   ! changed"

@@ -284,6 +284,15 @@ def test_basin_tidal_potential_matches_reference(basin_tide_golden):
     _run(d, meshgen.basin_tide().run, int(d["run_nsteps"]))
 
 
+def test_beach_hybrid_friction_matches_reference(beach_hybrid_golden):
+    """Hybrid bottom friction (NOLIBF = 2, src/nodalattr.F:2400-2411) in the
+    friction pass and in the wetting criterion (src/wetdry.F:415-451) on the
+    wet/dry storm beach, half an hour: bit-identical every step."""
+    from conftest import beach_hybrid_run
+    d = beach_hybrid_golden
+    _run(d, beach_hybrid_run(), int(d["run_nsteps"]))
+
+
 def test_basin_rivers_match_reference(basin_river_golden):
     """The basin with two rivers, a specified normal flux as an essential
     condition (IBTYPE 2) and as a natural one (IBTYPE 22), each with a steady
