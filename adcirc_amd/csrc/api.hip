@@ -117,7 +117,9 @@ struct adcgpu_ctx {
   std::vector<int> h_nbv, h_bnd_index;
   bool nfluxf = false;
   int *fb_ptr = nullptr, *fb_ent = nullptr, *nbv0 = nullptr;
-  double *bndlen = nullptr, *qn0 = nullptr, *qn1 = nullptr, *qn2 = nullptr;
+  double *bndlen = nullptr, *qn0 = nullptr, *qn1 = nullptr, *qn2 = nullptr, *qrad = nullptr;
+  bool has_rad = false;   // outward-radiating entries (LBCODEI 30)
+  bool has_spec = false;  // specified-flux entries (LBCODEI 2, 12, 22)
   double* qn_table = nullptr;
   int qn_table_steps = 0, qn_table_cap = 0;
   int64_t run_iters = 0, run_rebuilds = 0, run_itmax = 0, run_errstep = 0, run_mlused = 0;
@@ -192,6 +194,7 @@ struct adcgpu_ctx {
     s.qn0 = nfluxf ? qn0 : nullptr;
     s.qn1 = nfluxf ? qn1 : nullptr;
     s.qn2 = nfluxf ? qn2 : nullptr;
+    s.qrad = nfluxf ? qrad : nullptr;
     return s;
   }
 };
@@ -646,7 +649,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     lbc[I] = M->lbcodei[I];
     csii[I] = M->csii[I];
     siii[I] = M->siii[I];
-    if (lbc[I] < 0 || lbc[I] > 29)
+    if (lbc[I] < 0 || (lbc[I] > 29 && lbc[I] != 30))
       return fail(ADCGPU_E_UNSUPPORTED, "flow boundary type " + std::to_string(lbc[I]));
   }
   for (int J = 0; J < M->nvelme; ++J) {
@@ -728,6 +731,10 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   U(&c->csrpos, csrpos); U(&c->bnd_index, bnd_index); U(&c->nbd, nbd0); U(&c->obc_q, obc_q);
   U(&c->obc_s, obc_s); c->nobc = nobc;
   c->nvel = M->nvel;
+  for (int I = 0; I < M->nvel; ++I) {
+    c->has_rad = c->has_rad || lbc[I] == 30;
+    c->has_spec = c->has_spec || lbc[I] == 2 || lbc[I] == 12 || lbc[I] == 22;
+  }
   c->h_nbv = h_nbv;
   c->h_bnd_index = bnd_index;
   U(&c->obcg_ptr, obcg_ptr); U(&c->obcg, obcg); U(&c->vbc_ptr, vbc_ptr); U(&c->vbc, vbc);
@@ -1037,15 +1044,19 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
 // device (null: zero until adcgpu_set_flux).
 static int begin_step(adcgpu_ctx* c, int it, const double* qn_next) {
   c->cur_it = it;
-  if (c->nfluxf)
-    hipLaunchKernelGGL(k_flux_shift, dim3(nblk(c->nvel, 256)), dim3(256), 0, c->stream, c->qn0,
-                       c->qn1, c->qn2, qn_next, c->nvel);
+  if (c->has_rad && !c->nfluxf)
+    return fail(ADCGPU_E_ARG, "outward-radiating boundaries need adcgpu_set_flux_boundary");
   if (it == 1)
     hipLaunchKernelGGL(k_init_h, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->dp,
                        c->eta2, c->ifnlfa, c->h1, c->h2);
   // QX1 <- QX2, UU1 <- UU2, ... WSX1 <- WSX2, PR1 <- PR2 (src/timestep.F:385-431)
   std::swap(c->qx1, c->qx2); std::swap(c->qy1, c->qy2);
   std::swap(c->uu1, c->uu2); std::swap(c->vv1, c->vv2);
+  // the normal flux's time levels (after the velocities' shift: radiation
+  // reads the last step's velocities as UU1/VV1)
+  if (c->nfluxf)
+    hipLaunchKernelGGL(k_flux_shift, dim3(nblk(c->nvel, 256)), dim3(256), 0, c->stream,
+                       c->mesh(), c->state(), qn_next, c->nvel);
   if (c->prm.nws != 0) {
     std::swap(c->wsx1, c->wsx2); std::swap(c->wsy1, c->wsy2); std::swap(c->pr1, c->pr2);
   }
@@ -1392,7 +1403,7 @@ int adcgpu_holland(adcgpu_ctx* c, double t, const double* prm13) {
 static int run_prepare(adcgpu_ctx* c, int nsteps, const double* eta_bc_table,
                        const double* holland) {
   if (c->prm.nws != 0 && !holland) return fail(ADCGPU_E_ARG, "met forcing required");
-  if (c->nfluxf && c->qn_table_steps < nsteps)
+  if (c->nfluxf && c->has_spec && c->qn_table_steps < nsteps)
     return fail(ADCGPU_E_ARG, "flux boundaries: adcgpu_set_flux_table rows for every step required");
   if (c->neta > 0) {
     if (!eta_bc_table) return fail(ADCGPU_E_ARG, "eta_bc_table required");
@@ -1415,7 +1426,8 @@ static int run_prepare(adcgpu_ctx* c, int nsteps, const double* eta_bc_table,
 // A resident step up to the GWCE solve: begin, friction (with this step's
 // Holland forcing in the same pass), the assembly.
 static int run_step_assemble(adcgpu_ctx* c, int it, int s, const double* holland, GwceStep* g) {
-  int rc = begin_step(c, it, c->nfluxf ? c->qn_table + (size_t)s * c->nvel : nullptr);
+  const bool rows = c->nfluxf && c->qn_table_steps > s;
+  int rc = begin_step(c, it, rows ? c->qn_table + (size_t)s * c->nvel : nullptr);
   if (!rc) {
     if (holland) {
       // friction and this step's forcing in one pass
@@ -1762,7 +1774,7 @@ int adcgpu_set_flux_boundary(adcgpu_ctx* c, const double* bndlen2o3) {
   if (int rc = upload(&c->nbv0, c->h_nbv)) return rc;
   if (int rc = upload(&c->bndlen, len)) return rc;
   if (!c->qn0) {
-    for (double** p : {&c->qn0, &c->qn1, &c->qn2}) {
+    for (double** p : {&c->qn0, &c->qn1, &c->qn2, &c->qrad}) {
       if (int rc = dmalloc(p, (size_t)nvel)) return rc;
       c->allocs.push_back((void*)*p);
     }

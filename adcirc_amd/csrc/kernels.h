@@ -117,9 +117,11 @@ struct DevState {
   double *tip1, *tip2;   // tidal potential at time levels s and s+1 (null: none)
   double *qn0, *qn1, *qn2;  // normal flux per flow-boundary entry at three time
                             // levels (null: no flux boundaries)
+  double* qrad;             // per entry: the step's QFORCE of an outward-radiating
+                            // entry (LBCODEI 30), formed at the step's start
 };
 
-__global__ void k_flux_shift(double* qn0, double* qn1, double* qn2, const double* next, int nvel);
+__global__ void k_flux_shift(DevMesh m, DevState s, const double* next, int nvel);
 
 // The tidal potential of one step (src/timestep.F:1527-1557, NTIP = 1), per
 // constituent: its species, ARGT (the phase at the step, reduced by whole

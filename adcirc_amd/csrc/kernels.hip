@@ -803,9 +803,11 @@ __global__ __launch_bounds__(256) void k_rhs_elem32(DevMesh m, DevState s, RhsPr
   contrib[m.slot[3 * e + 2]] = R3 * nce;
 }
 
-// QFORCE of flow-boundary entry K for a specified (or zero) flux, codes up to
-// 29 (src/gwce.F:1741, 1783): (QN2 - QN0)/DT2 + Tau0Var*QN1.
+// QFORCE of flow-boundary entry K: for a specified (or zero) flux, codes up
+// to 29, (QN2 - QN0)/DT2 + Tau0Var*QN1 (src/gwce.F:1741, 1783); for outward
+// radiation (30) the value k_flux_shift formed at the step's start.
 __device__ __forceinline__ double qforce(const DevMesh& m, const DevState& s, int K) {
+  if (m.lbcodei[K] == 30) return s.qrad[K];
   return (s.qn2[K] - s.qn0[K]) / (m.dt * 2.0) + m.tau0[m.nbv[K]] * s.qn1[K];
 }
 
@@ -825,17 +827,28 @@ __device__ __forceinline__ double flux_terms(const DevMesh& m, const DevState& s
   return acc;
 }
 
-// The shift of the specified normal flux to the previous time levels and the
-// step's new QN2 (src/timestep.F:605-612, then 861-903): next == null leaves
-// QN2 zero, as the reference's shift does.
-__global__ __launch_bounds__(256) void k_flux_shift(double* __restrict__ qn0, double* __restrict__ qn1,
-                                                    double* __restrict__ qn2,
+// The shift of the normal flux to the previous time levels and the step's
+// new QN2 (src/timestep.F:605-612, then 861-903): next == null leaves QN2
+// zero, as the reference's shift does.  An outward-radiating entry (30) takes
+// QN1 from the last velocities and depth (src/timestep.F:929-940) and its
+// QFORCE, -sqrt(G H1) ETAS/DT - Tau0Var QN1 with H1 the depth the GWCE will
+// hold (the last step's H2) and ETAS the last increment (src/gwce.F:1750-1754,
+// 1785-1789), is formed here, before the assembly rewrites ETAS.
+__global__ __launch_bounds__(256) void k_flux_shift(DevMesh m, DevState s,
                                                     const double* __restrict__ next, int nvel) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nvel) return;
-  qn0[k] = qn1[k];
-  qn1[k] = qn2[k];
-  qn2[k] = next ? next[k] : 0.0;
+  s.qn0[k] = s.qn1[k];
+  double q1 = s.qn2[k];
+  s.qn2[k] = next ? next[k] : 0.0;
+  if (m.lbcodei[k] == 30) {
+    const int b = m.nbv[k];
+    const double un1 = s.uu1[b] * m.csii[k] + s.vv1[b] * m.siii[k];
+    q1 = s.h2[b] * un1;
+    const double celerity = __builtin_sqrt(m.g * s.h2[b]);
+    s.qrad[k] = -celerity * s.etas[b] / m.dt - m.tau0[b] * q1;
+  }
+  s.qn1[k] = q1;
 }
 
 // Node gather of GWCE_LV, the time-level shift of ETA and the elevation

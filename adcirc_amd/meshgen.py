@@ -223,6 +223,27 @@ def basin_river(seed: int = 1) -> Case:
     return case
 
 
+def basin_radiation(seed: int = 1) -> Case:
+    """The basin with 30 km of its north edge open as an outward-radiating
+    boundary (IBTYPE 30): the normal flux there follows the elevation
+    change, c = sqrt(g H)."""
+    nx, ny = 101, 51
+    case = basin(seed, nx, ny)
+
+    def nid(i, j):
+        return i * ny + j + 1
+
+    north = [nid(i, ny - 1) for i in range(nx - 1, -1, -1)]
+    west = [nid(0, j) for j in range(ny - 2, -1, -1)]
+    south = [nid(i, 0) for i in range(1, nx)]
+    ring = north + west + south
+    r0, r1 = ring.index(nid(70, ny - 1)), ring.index(nid(40, ny - 1))
+    segs = [(0, ring[:r0 + 1]), (30, ring[r0:r1 + 1]), (0, ring[r1:])]
+    case.mesh.land_bnd = [(ib, np.array(v, np.int32)) for ib, v in segs]
+    case.mesh.name = f"basin radiation {nx}x{ny} seed {seed}"
+    return case
+
+
 def beach(seed: int = 3, nx: int = 41, ny: int = 21, lx: float = 8.0e3,
           ly: float = 4.0e3, storm: bool = True) -> Case:
     """A small sloping beach (6 m offshore at x=lx to -1.5 m at x=0) that wets

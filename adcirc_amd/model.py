@@ -209,6 +209,10 @@ class Model:
             if qnam is None or qnph is None or "bndlen2o3" not in t:
                 raise ValueError("flux boundaries need QNAM/QNPH and the tables' BNDLEN2O3")
             self.set_flux_boundary(t["lbcodei"], t["bndlen2o3"], run.fluxf, qnam, qnph)
+        elif int(t.get("nvel", 0)) and (np.asarray(t["lbcodei"]) == 30).any():
+            # outward radiation only: the line integrals without flux forcing
+            bl = np.ascontiguousarray(t["bndlen2o3"], np.float64)
+            check(lib.adcgpu_set_flux_boundary(self.h, _p(bl, dp_)), "set_flux_boundary")
 
     @staticmethod
     def radians(deg: np.ndarray) -> np.ndarray:

@@ -113,7 +113,8 @@ def csr_positions(neitab: np.ndarray, nneigh: np.ndarray, nm0: np.ndarray) -> np
 _ME_OVERLAP_CODES = (2, 12, 22, 3, 13, 23, 30, 32, 40, 41, 52, 64)
 
 
-FLOW_TYPES = (0, 10, 20, 2, 12, 22)   # land and specified-normal-flux boundaries
+# land, specified-normal-flux and outward-radiation boundaries
+FLOW_TYPES = (0, 10, 20, 2, 12, 22, 30)
 FLUX_TYPES = (2, 12, 22)
 
 

@@ -117,7 +117,7 @@ enum adcgpu_field {
 
 /* Create a context on `device` (HIP ordinal) from the mesh and parameters.
  * Returns ADCGPU_E_UNSUPPORTED for configurations outside the implemented
- * subset (3D, sponge, subgrid, weirs, flow-boundary codes above 29).  Specified-flux codes (2, 12, 22) carry a zero flux until
+ * subset (3D, sponge, subgrid, weirs, flow-boundary codes above 30).  Specified-flux codes (2, 12, 22) carry a zero flux until
  * adcgpu_set_flux_boundary. */
 int adcgpu_create(const adcgpu_mesh *mesh, const adcgpu_params *prm, int device,
                   adcgpu_ctx **out);
@@ -237,14 +237,18 @@ int adcgpu_set_tide_potential(adcgpu_ctx *ctx, int ntif, const double *tpk, cons
 int adcgpu_set_hybrid_friction(adcgpu_ctx *ctx, double hbreak, double ftheta, double fgamma);
 
 /* Specified normal flux boundaries (NFLUXF = 1; LBCODEI 2, 12 and 22 among the
- * land codes).  bndlen2o3: BNDLEN2O3(NVEL), two thirds of each flow-boundary
+ * land codes) and outward-radiating ones (NFLUXRBC = 1; LBCODEI 30, which
+ * need this call before the first step).  bndlen2o3: BNDLEN2O3(NVEL), two thirds of each flow-boundary
  * edge to the next entry, zero after each segment's last node, as the
  * reference's mesh set-up forms it (src/mesh.F:2300-2303); NULL turns the
  * flux boundaries off.  From then on each adcgpu_begin_step shifts
  * QN0 <- QN1 <- QN2 and zeroes QN2 (src/timestep.F:605-612), the GWCE takes
  * the flux line integrals of every boundary edge (src/gwce.F:1737-1834,
  * codes up to 29) and momentum imposes -QN2/H2 as the normal velocity of
- * codes 0-19 (src/momentum.F:810-848).  QN0..QN2 start at zero. */
+ * codes 0-19 (src/momentum.F:810-848).  QN0..QN2 start at zero.
+ * Radiating entries take QN1 from the last velocities each step
+ * (src/timestep.F:929-940) and the radiation QFORCE (src/gwce.F:1750-1754);
+ * they need no QN2. */
 int adcgpu_set_flux_boundary(adcgpu_ctx *ctx, const double *bndlen2o3);
 /* This step's QN2(NVEL) (src/timestep.F:861-903: the caller's periodic or
  * fort.20 flux, e.g. from adcgpu_flux_bc), host pointer; between

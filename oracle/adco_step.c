@@ -237,9 +237,10 @@ double adco_ramptip(const adco_model *m) { return m->ramp; }
 int adco_set_flux(adco_model *m, const double *bndlen2o3) {
   const int nvel = m->nvel;
   if (nvel < 1) return -1;
-  /* the codes whose fluxes are specified or zero (LBCODEI <= 29) */
+  /* the codes whose fluxes are specified or zero (LBCODEI <= 29) and the
+   * outward radiation (30) */
   for (int k = 0; k < nvel; ++k)
-    if (m->s.lbcodei[k] < 0 || m->s.lbcodei[k] > 29) return -1;
+    if (m->s.lbcodei[k] < 0 || (m->s.lbcodei[k] > 29 && m->s.lbcodei[k] != 30)) return -1;
   if (!m->bndlen) {
     m->bndlen = dalloc(nvel);
     m->qn0 = dalloc(nvel);
@@ -286,17 +287,26 @@ void adco_flux_bc(int nvel, const int *lbcodei, int nffr, const double *famig,
  * boundary codes up to 29: each boundary edge J-1..J adds
  * NCI*NCJ*BNDLEN2O3(J-1)/4 * (2 QFORCE(own) + QFORCE(other)) to both of its
  * nodes, in edge order, I's node first. */
+static double qforce(const adco_model *m, int K, int b) {
+  const double *t0 = m->s.tau0var;
+  if (m->s.lbcodei[K] == 30) {
+    /* radiation: src/gwce.F:1750-1754, 1785-1789 (H1 is the saved H1, the
+     * last step's H2; ETAS the last increment) */
+    const double celerity = sqrt(m->s.g * m->h1[b]);
+    return -celerity * m->etas[b] / m->dt - t0[b] * m->qn1[K];
+  }
+  return (m->qn2[K] - m->qn0[K]) / (m->dt * 2.0) + t0[b] * m->qn1[K];
+}
+
 static void gwce_flux(adco_model *m) {
   const int *nbv = m->s.nbv;
-  const double *t0 = m->s.tau0var;
-  const double dt2 = m->dt * 2.0;
   int nbdj = nbv[0] - 1;
-  double qforcej = (m->qn2[0] - m->qn0[0]) / dt2 + t0[nbdj] * m->qn1[0];
+  double qforcej = qforce(m, 0, nbdj);
   for (int J = 1; J < m->nvel; ++J) {
     const int nbdi = nbdj;
     nbdj = nbv[J] - 1;
     const double qforcei = qforcej;
-    qforcej = (m->qn2[J] - m->qn0[J]) / dt2 + t0[nbdj] * m->qn1[J];
+    qforcej = qforce(m, J, nbdj);
     const int nci = m->nodecode[nbdi], ncj = m->nodecode[nbdj];
     const double w = (double)(nci * ncj) * m->bndlen[J - 1] / 4.0;
     m->gwce_lv[nbdi] = m->gwce_lv[nbdi] + w * (2.0 * qforcei + qforcej);
@@ -1085,6 +1095,14 @@ int adco_step(adco_model *m, int it, const double *wsx2, const double *wsy2,
       m->qn2[k] = m->qn_pending ? m->qn_next[k] : 0.0;
     }
     m->qn_pending = 0;
+    /* outward radiation (LBCODEI 30): the flux leaving through the boundary
+     * from the last velocities, src/timestep.F:929-940 */
+    for (int k = 0; k < m->nvel; ++k)
+      if (m->s.lbcodei[k] == 30) {
+        const int b = m->s.nbv[k] - 1;
+        const double un1 = m->uu1[b] * m->s.csii[k] + m->vv1[b] * m->s.siii[k];
+        m->qn1[k] = m->h2[b] * un1;
+      }
   }
   if (m->s.nws != 0 && wsx2 && wsy2 && pr2) {
     memcpy(m->wsx2, wsx2, sizeof(double) * np);

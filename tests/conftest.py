@@ -53,6 +53,11 @@ def basin_river_golden():
 
 
 @pytest.fixture(scope="session")
+def basin_radiation_golden():
+    return load_golden("basin_radiation")
+
+
+@pytest.fixture(scope="session")
 def beach_hybrid_golden():
     return load_golden("beach_hybrid")
 

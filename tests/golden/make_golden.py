@@ -9,6 +9,7 @@ everywhere else.
 
     python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [basin_linear] [jcg] [tau0]
                                        [coastal20k] [basin_tide] [basin_river] [beach_hybrid]
+                                       [basin_radiation]
 """
 from __future__ import annotations
 
@@ -67,8 +68,10 @@ def run_case(name: str, case, nsteps: int, early: int, dump_every: int,
     if snap_keys is SNAP_KEYS:
         out["s1_coef"] = snaps[1]["coef"]
     if case.qnam is not None:
-        # the flux forcing per flux-boundary node, and the reference's edge lengths
+        # the flux forcing per flux-boundary node
         out["qnam"], out["qnph"] = case.qnam, case.qnph
+    if case.qnam is not None or any(ib == 30 for ib, _ in case.mesh.land_bnd):
+        # the reference's boundary edge lengths
         out["bndlen2o3"] = b["bndlen2o3"]
     out["run_nsteps"] = np.array(nsteps)
     path = os.path.join(HERE, f"{name}.npz")
@@ -227,6 +230,11 @@ def main(argv):
         # the wet/dry storm beach with hybrid friction (NOLIBF = 2: CFmin,
         # HBREAK = 2 m, FTHETA = 10, FGAMMA = 1/3), half an hour
         print(run_case("beach_hybrid", beach_hybrid(), nsteps=1800, early=10,
+                       dump_every=600))
+    if "basin_radiation" in todo:
+        # the basin with 30 km of the north edge radiating (IBTYPE 30); one
+        # simulated hour
+        print(run_case("basin_radiation", meshgen.basin_radiation(), nsteps=1800, early=10,
                        dump_every=600))
     if "basin_river" in todo:
         # the basin with two rivers (IBTYPE 2 and 22), steady and M2 flux;

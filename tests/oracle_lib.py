@@ -178,6 +178,11 @@ class OracleModel:
                                            _p(slam, _dp), _p(sfea, _dp))
             assert rc == 0, "adco_set_tidepot"
         self.flux = None
+        if not getattr(run, "fluxf", ()) and "bndlen2o3" in d and \
+                (np.asarray(d["lbcodei"]) == 30).any():
+            # outward radiation only: the flux line integrals without forcing
+            self.keep["bndlen2o3"] = np.ascontiguousarray(d["bndlen2o3"], np.float64)
+            assert self.lib.adco_set_flux(self.h, _p(self.keep["bndlen2o3"], _dp)) == 0
         if getattr(run, "fluxf", ()):
             lbc = np.asarray(d["lbcodei"], np.int32)
             am, ph = flux_arrays(lbc, np.asarray(d["qnam"]), np.asarray(d["qnph"]))

@@ -78,3 +78,31 @@ def test_rivers_snapshot_restart_and_batch(basin_river_golden):
         ref = a.get(k)
         assert np.array_equal(b.get(k), ref), k
         assert np.array_equal(c.get(k), ref), k
+
+
+def test_radiation_bitexact_and_resident(basin_radiation_golden):
+    """The basin with an outward-radiating stretch of its north edge
+    (IBTYPE 30; QN1 from the last velocities and the radiation QFORCE formed
+    at each step's start): 1,800 per-step calls bit-exact against the oracle,
+    within 1e-12 of the reference at its snapshots, and a resident run (no
+    flux rows needed) equal to the per-step calls."""
+    d = basin_radiation_golden
+    run = meshgen.basin_radiation().run
+    log("case basin_radiation: 1800 steps bit-exact vs oracle; resident 600")
+    g, o = build_pair(d, run)
+    snaps = set(int(s) for s in d["snap_steps"])
+    for it in range(1, 1801):
+        step_pair(g, o, it)
+        if it <= 3 or it % 50 == 0:
+            bad = compare(g, o, it)
+            assert not bad, f"step {it}: " + "; ".join(bad)
+        if it in snaps:
+            for k in ("eta2", "uu2", "vv2"):
+                dk = np.abs(g.get(k) - d[f"s{it}_{k}"]).max()
+                assert dk <= 1e-12, f"step {it}: {k} differs from the reference by {dk:.3e}"
+        if it == 600:
+            e600 = {k: g.get(k) for k in ("eta2", "uu2", "vv2", "etas")}
+    r, _ = build_pair(d, run)
+    r.run_resident(1, 600)
+    for k, v in e600.items():
+        assert np.array_equal(r.get(k), v), k

@@ -304,6 +304,22 @@ def test_basin_rivers_match_reference(basin_river_golden):
     assert np.abs(m.flux["qn2"]).max() > 0.2  # the ramp is still rising
 
 
+def test_basin_radiation_matches_reference(basin_radiation_golden):
+    """The basin with an outward-radiating stretch of its north edge
+    (IBTYPE 30: QN1 from the last velocities, src/timestep.F:929-940, and the
+    radiation QFORCE of src/gwce.F:1750-1754, 1785-1789), one hour:
+    bit-identical every step; the tables match the reference's."""
+    from adcirc_amd import prep
+    d = basin_radiation_golden
+    case = meshgen.basin_radiation()
+    t = prep.build_tables(case.mesh, case.run.nolifa, case.run.h0, case.run.anginn)
+    for k in ("nbv", "lbcodei", "me2gw", "bndlen2o3"):
+        assert np.array_equal(t[k], d[k]), k
+    m = _run(d, case.run, int(d["run_nsteps"]))
+    q = m.array("qn1", int(d["nvel"]))
+    assert np.abs(q[np.asarray(d["lbcodei"]) == 30]).max() > 0.0
+
+
 def test_river_tables_match_reference(basin_river_golden):
     """The product's flow-boundary tables for a mesh with flux segments
     (LBCODEI 2, 22, the ME2GW precedence of flux codes at shared end nodes,
