@@ -46,11 +46,12 @@ def _expect(vals, nodecode):
     return np.array([float(outputs.e20(v)) for v in np.where(nodecode == 0, -99999.0, vals)])
 
 
-@pytest.mark.parametrize("name", ["basin", "beach_storm", "basin_river"])
+@pytest.mark.parametrize("name", ["basin", "beach_storm", "basin_river", "basin_radiation"])
 def test_driver_outputs_equal_per_step_state(tmp_path, name):
     d = str(tmp_path / name)
-    if name in ("basin", "basin_river"):
-        case = meshgen.basin() if name == "basin" else meshgen.basin_river()
+    if name in ("basin", "basin_river", "basin_radiation"):
+        case = {"basin": meshgen.basin, "basin_river": meshgen.basin_river,
+                "basin_radiation": meshgen.basin_radiation}[name]()
         n, hol = 600, None
         elev, vel = fort15.Output(1, 0.0, 1.0, 200), fort15.Output(1, 0.0, 1.0, 300)
     else:
