@@ -118,7 +118,8 @@ struct adcgpu_ctx {
   bool nfluxf = false;
   int *fb_ptr = nullptr, *fb_ent = nullptr, *nbv0 = nullptr;
   double *bndlen = nullptr, *qn0 = nullptr, *qn1 = nullptr, *qn2 = nullptr, *qrad = nullptr;
-  bool has_rad = false;   // outward-radiating entries (LBCODEI 30)
+  double* elevdisc = nullptr;  // river radiation's reference elevation per entry
+  bool has_rad = false;   // radiating entries (LBCODEI 30, 52)
   bool has_spec = false;  // specified-flux entries (LBCODEI 2, 12, 22)
   double* qn_table = nullptr;
   int qn_table_steps = 0, qn_table_cap = 0;
@@ -195,6 +196,7 @@ struct adcgpu_ctx {
     s.qn1 = nfluxf ? qn1 : nullptr;
     s.qn2 = nfluxf ? qn2 : nullptr;
     s.qrad = nfluxf ? qrad : nullptr;
+    s.elevdisc = nfluxf ? elevdisc : nullptr;
     return s;
   }
 };
@@ -649,7 +651,7 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
     lbc[I] = M->lbcodei[I];
     csii[I] = M->csii[I];
     siii[I] = M->siii[I];
-    if (lbc[I] < 0 || (lbc[I] > 29 && lbc[I] != 30))
+    if (lbc[I] < 0 || (lbc[I] > 29 && lbc[I] != 30 && lbc[I] != 52))
       return fail(ADCGPU_E_UNSUPPORTED, "flow boundary type " + std::to_string(lbc[I]));
   }
   for (int J = 0; J < M->nvelme; ++J) {
@@ -732,8 +734,8 @@ int adcgpu_create(const adcgpu_mesh* M, const adcgpu_params* P, int device, adcg
   U(&c->obc_s, obc_s); c->nobc = nobc;
   c->nvel = M->nvel;
   for (int I = 0; I < M->nvel; ++I) {
-    c->has_rad = c->has_rad || lbc[I] == 30;
-    c->has_spec = c->has_spec || lbc[I] == 2 || lbc[I] == 12 || lbc[I] == 22;
+    c->has_rad = c->has_rad || lbc[I] == 30 || lbc[I] == 52;
+    c->has_spec = c->has_spec || lbc[I] == 2 || lbc[I] == 12 || lbc[I] == 22 || lbc[I] == 52;
   }
   c->h_nbv = h_nbv;
   c->h_bnd_index = bnd_index;
@@ -997,6 +999,8 @@ int adcgpu_get_field(adcgpu_ctx* c, int f, void* host, int64_t count) {
   return 0;
 }
 
+static int set_elevdisc(adcgpu_ctx* c);
+
 int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode,
                       const int32_t* noff) {
   if (!c || !eta2 || !nodecode || !noff) return fail(ADCGPU_E_ARG, "null argument");
@@ -1037,6 +1041,8 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
   f[F_NCCHANGE] = 1;
   HIPCHK(hipMemcpyAsync(c->flags, f, sizeof(f), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->nfluxf)
+    if (int rc = set_elevdisc(c)) return rc;
   return check_launch("cold start");
 }
 
@@ -1045,7 +1051,7 @@ int adcgpu_cold_start(adcgpu_ctx* c, const double* eta2, const int32_t* nodecode
 static int begin_step(adcgpu_ctx* c, int it, const double* qn_next) {
   c->cur_it = it;
   if (c->has_rad && !c->nfluxf)
-    return fail(ADCGPU_E_ARG, "outward-radiating boundaries need adcgpu_set_flux_boundary");
+    return fail(ADCGPU_E_ARG, "radiating boundaries need adcgpu_set_flux_boundary");
   if (it == 1)
     hipLaunchKernelGGL(k_init_h, dim3(nblk(c->np, 256)), dim3(256), 0, c->stream, c->np, c->dp,
                        c->eta2, c->ifnlfa, c->h1, c->h2);
@@ -1717,6 +1723,23 @@ int adcgpu_set_jcg_blocks(adcgpu_ctx* c, int blocks) {
   return 0;
 }
 
+// River radiation's reference elevation ElevDisc: with no flux-settling time
+// the first step takes the cold-start elevation (EtaDisc, src/cstart.F:
+// 1188-1192, src/timestep.F:917-925); formed from the current ETA2 at the
+// flux set-up and at every cold start.
+static int set_elevdisc(adcgpu_ctx* c) {
+  const int nvel = c->nvel;
+  std::vector<double> eta((size_t)c->np), ed((size_t)nvel, 0.0);
+  std::vector<int> lbc((size_t)nvel);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(eta.data(), c->eta2, (size_t)c->np * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(lbc.data(), c->lbcodei, (size_t)nvel * sizeof(int), hipMemcpyDeviceToHost));
+  for (int k = 0; k < nvel; ++k)
+    if (lbc[k] == 52) ed[k] = eta[c->h_nbv[k]];
+  HIPCHK(hipMemcpy(c->elevdisc, ed.data(), (size_t)nvel * 8, hipMemcpyHostToDevice));
+  return 0;
+}
+
 int adcgpu_set_hybrid_friction(adcgpu_ctx* c, double hbreak, double ftheta, double fgamma) {
   if (!c) return fail(ADCGPU_E_ARG, "null context");
   if (!(hbreak > 0.0) || !(ftheta > 0.0) || !(fgamma >= 0.0) || !std::isfinite(hbreak) ||
@@ -1774,7 +1797,7 @@ int adcgpu_set_flux_boundary(adcgpu_ctx* c, const double* bndlen2o3) {
   if (int rc = upload(&c->nbv0, c->h_nbv)) return rc;
   if (int rc = upload(&c->bndlen, len)) return rc;
   if (!c->qn0) {
-    for (double** p : {&c->qn0, &c->qn1, &c->qn2, &c->qrad}) {
+    for (double** p : {&c->qn0, &c->qn1, &c->qn2, &c->qrad, &c->elevdisc}) {
       if (int rc = dmalloc(p, (size_t)nvel)) return rc;
       c->allocs.push_back((void*)*p);
     }
@@ -1783,7 +1806,7 @@ int adcgpu_set_flux_boundary(adcgpu_ctx* c, const double* bndlen2o3) {
   for (double* p : {c->qn0, c->qn1, c->qn2}) HIPCHK(hipMemset(p, 0, (size_t)nvel * 8));
   HIPCHK(hipMemcpy(c->bnd_index, bidx.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
   c->nfluxf = true;
-  return 0;
+  return set_elevdisc(c);
 }
 
 int adcgpu_set_flux_table(adcgpu_ctx* c, int nsteps, const double* qn2_table) {
@@ -2018,7 +2041,7 @@ std::vector<StatePart> state_parts(adcgpu_ctx* c) {
     v.push_back({c->tip2, np * 8});
   }
   if (c->nfluxf)
-    for (double* p : {c->qn0, c->qn1, c->qn2}) v.push_back({p, (size_t)c->nvel * 8});
+    for (double* p : {c->qn0, c->qn1, c->qn2, c->elevdisc}) v.push_back({p, (size_t)c->nvel * 8});
   return v;
 }
 }  // namespace

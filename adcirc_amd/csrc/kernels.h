@@ -118,7 +118,9 @@ struct DevState {
   double *qn0, *qn1, *qn2;  // normal flux per flow-boundary entry at three time
                             // levels (null: no flux boundaries)
   double* qrad;             // per entry: the step's QFORCE of an outward-radiating
-                            // entry (LBCODEI 30), formed at the step's start
+                            // entry (LBCODEI 30), or the radiation part of a
+                            // river-radiation entry's (52), formed at the step's start
+  double* elevdisc;         // per entry: river radiation's reference elevation
 };
 
 __global__ void k_flux_shift(DevMesh m, DevState s, const double* next, int nvel);

@@ -808,7 +808,9 @@ __global__ __launch_bounds__(256) void k_rhs_elem32(DevMesh m, DevState s, RhsPr
 // radiation (30) the value k_flux_shift formed at the step's start.
 __device__ __forceinline__ double qforce(const DevMesh& m, const DevState& s, int K) {
   if (m.lbcodei[K] == 30) return s.qrad[K];
-  return (s.qn2[K] - s.qn0[K]) / (m.dt * 2.0) + m.tau0[m.nbv[K]] * s.qn1[K];
+  const double q = (s.qn2[K] - s.qn0[K]) / (m.dt * 2.0) + m.tau0[m.nbv[K]] * s.qn1[K];
+  // river radiation (52) past the flux-settling steps (src/gwce.F:1767-1775)
+  return m.lbcodei[K] == 52 ? q - s.qrad[K] : q;
 }
 
 // The specified-flux line integrals a node receives (src/gwce.F:1777-1832):
@@ -847,6 +849,11 @@ __global__ __launch_bounds__(256) void k_flux_shift(DevMesh m, DevState s,
     q1 = s.h2[b] * un1;
     const double celerity = __builtin_sqrt(m.g * s.h2[b]);
     s.qrad[k] = -celerity * s.etas[b] / m.dt - m.tau0[b] * q1;
+  } else if (m.lbcodei[k] == 52) {
+    // Celerity*(ETAS/DT + Tau0Var*(ETA1 - ElevDisc)): ETA2 now is the GWCE's ETA1
+    const int b = m.nbv[k];
+    const double celerity = __builtin_sqrt(m.g * s.h2[b]);
+    s.qrad[k] = celerity * (s.etas[b] / m.dt + m.tau0[b] * (s.eta2[b] - s.elevdisc[k]));
   }
   s.qn1[k] = q1;
 }

@@ -31,7 +31,7 @@ import numpy as np
 from . import fort14, fort15, outputs, prep
 from .model import Model, cold_wetdry_state
 
-FLUX_CODES = (2, 12, 22)
+FLUX_CODES = (2, 12, 22, 52)
 
 
 def flux_entries(mesh) -> int:

@@ -187,7 +187,7 @@ def basin_tide(seed: int = 1) -> Case:
     return case
 
 
-def basin_river(seed: int = 1) -> Case:
+def basin_river(seed: int = 1, west_ibtype: int = 2) -> Case:
     """The basin with two rivers besides its boundary tide: 8 km of the
     shallow west edge take a specified normal flux as an essential condition
     (IBTYPE 2), 10 km of the north edge as a natural one (IBTYPE 22).  Each
@@ -208,8 +208,9 @@ def basin_river(seed: int = 1) -> Case:
     ring = north + west + south
     r_n0, r_n1 = ring.index(nid(40, ny - 1)), ring.index(nid(30, ny - 1))
     r_w0, r_w1 = ring.index(nid(0, 29)), ring.index(nid(0, 21))
+    # west_ibtype = 52: the west river radiates too (river radiation)
     segs = [(0, ring[:r_n0 + 1]), (22, ring[r_n0:r_n1 + 1]), (0, ring[r_n1:r_w0 + 1]),
-            (2, ring[r_w0:r_w1 + 1]), (0, ring[r_w1:])]
+            (west_ibtype, ring[r_w0:r_w1 + 1]), (0, ring[r_w1:])]
     case.mesh.land_bnd = [(ib, np.array(v, np.int32)) for ib, v in segs]
     case.mesh.name = f"basin river {nx}x{ny} seed {seed}"
     case.run.fluxf = ((0.0, 1.0, 0.0), (M2_OMEGA, 1.0, 30.0))

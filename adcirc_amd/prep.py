@@ -114,8 +114,8 @@ _ME_OVERLAP_CODES = (2, 12, 22, 3, 13, 23, 30, 32, 40, 41, 52, 64)
 
 
 # land, specified-normal-flux and outward-radiation boundaries
-FLOW_TYPES = (0, 10, 20, 2, 12, 22, 30)
-FLUX_TYPES = (2, 12, 22)
+FLOW_TYPES = (0, 10, 20, 2, 12, 22, 30, 52)
+FLUX_TYPES = (2, 12, 22, 52)
 
 
 def flow_boundaries(x: np.ndarray, y: np.ndarray, land, costset: float):

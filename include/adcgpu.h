@@ -117,7 +117,8 @@ enum adcgpu_field {
 
 /* Create a context on `device` (HIP ordinal) from the mesh and parameters.
  * Returns ADCGPU_E_UNSUPPORTED for configurations outside the implemented
- * subset (3D, sponge, subgrid, weirs, flow-boundary codes above 30).  Specified-flux codes (2, 12, 22) carry a zero flux until
+ * subset (3D, sponge, subgrid, weirs, flow-boundary codes above 29 other than
+ * 30 and 52).  Specified-flux codes (2, 12, 22) carry a zero flux until
  * adcgpu_set_flux_boundary. */
 int adcgpu_create(const adcgpu_mesh *mesh, const adcgpu_params *prm, int device,
                   adcgpu_ctx **out);
@@ -237,8 +238,10 @@ int adcgpu_set_tide_potential(adcgpu_ctx *ctx, int ntif, const double *tpk, cons
 int adcgpu_set_hybrid_friction(adcgpu_ctx *ctx, double hbreak, double ftheta, double fgamma);
 
 /* Specified normal flux boundaries (NFLUXF = 1; LBCODEI 2, 12 and 22 among the
- * land codes) and outward-radiating ones (NFLUXRBC = 1; LBCODEI 30, which
- * need this call before the first step).  bndlen2o3: BNDLEN2O3(NVEL), two thirds of each flow-boundary
+ * land codes, and 52, river radiation with no flux-settling time: its
+ * reference elevation is the cold-start ETA2, src/cstart.F:1188-1192) and
+ * outward-radiating ones (NFLUXRBC = 1; LBCODEI 30).  30 and 52 need this
+ * call before the first step.  bndlen2o3: BNDLEN2O3(NVEL), two thirds of each flow-boundary
  * edge to the next entry, zero after each segment's last node, as the
  * reference's mesh set-up forms it (src/mesh.F:2300-2303); NULL turns the
  * flux boundaries off.  From then on each adcgpu_begin_step shifts

@@ -58,6 +58,7 @@ struct adco_model {
    * (src/global.F, src/boundaries.F); qn_next is the next step's QN2 */
   int nfluxf, qn_pending;
   double *bndlen, *qn0, *qn1, *qn2, *qn_next;
+  double *elevdisc; /* river radiation (52): the cold-start elevation (ElevDisc) */
 };
 
 #define NM(ie, k) (m->s.nm[(size_t)(ie)*3 + (k)] - 1)
@@ -173,7 +174,7 @@ void adco_destroy(adco_model *m) {
                m->pr2, m->etamax, m->totalarea, m->tk_tmp, m->tkm, m->auv, m->nodecode,
                m->nnodecode, m->mju, m->etamax_it, m->umax_it, m->umax, m->nibcnt, m->noff, m->noffold, m->alphal, m->coef,
                m->obccoef, m->wksp, m->ln, m->slam, m->tip1, m->tip2, m->bndlen, m->qn0,
-               m->qn1, m->qn2, m->qn_next};
+               m->qn1, m->qn2, m->qn_next, m->elevdisc};
   for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
   free(m);
 }
@@ -240,14 +241,22 @@ int adco_set_flux(adco_model *m, const double *bndlen2o3) {
   /* the codes whose fluxes are specified or zero (LBCODEI <= 29) and the
    * outward radiation (30) */
   for (int k = 0; k < nvel; ++k)
-    if (m->s.lbcodei[k] < 0 || (m->s.lbcodei[k] > 29 && m->s.lbcodei[k] != 30)) return -1;
+    if (m->s.lbcodei[k] < 0 ||
+        (m->s.lbcodei[k] > 29 && m->s.lbcodei[k] != 30 && m->s.lbcodei[k] != 52))
+      return -1;
   if (!m->bndlen) {
     m->bndlen = dalloc(nvel);
     m->qn0 = dalloc(nvel);
     m->qn1 = dalloc(nvel);
     m->qn2 = dalloc(nvel);
     m->qn_next = dalloc(nvel);
+    m->elevdisc = dalloc(nvel);
   }
+  /* river radiation's reference elevation: with no flux-settling time the
+   * first step takes the cold-start elevation kept in EtaDisc
+   * (src/cstart.F:1188-1192, src/timestep.F:917-925) */
+  for (int k = 0; k < nvel; ++k)
+    if (m->s.lbcodei[k] == 52) m->elevdisc[k] = m->eta2[m->s.nbv[k] - 1];
   memcpy(m->bndlen, bndlen2o3, sizeof(double) * nvel);
   m->nfluxf = 1;
   m->qn_pending = 0;
@@ -295,7 +304,14 @@ static double qforce(const adco_model *m, int K, int b) {
     const double celerity = sqrt(m->s.g * m->h1[b]);
     return -celerity * m->etas[b] / m->dt - t0[b] * m->qn1[K];
   }
-  return (m->qn2[K] - m->qn0[K]) / (m->dt * 2.0) + t0[b] * m->qn1[K];
+  double q = (m->qn2[K] - m->qn0[K]) / (m->dt * 2.0) + t0[b] * m->qn1[K];
+  if (m->s.lbcodei[K] == 52) {
+    /* river radiation after the flux-settling steps (none here, IT > 0):
+     * src/gwce.F:1767-1775, 1808-1816 */
+    const double celerity = sqrt(m->s.g * m->h1[b]);
+    q = q - celerity * (m->etas[b] / m->dt + t0[b] * (m->eta1[b] - m->elevdisc[K]));
+  }
+  return q;
 }
 
 static void gwce_flux(adco_model *m) {

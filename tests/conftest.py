@@ -58,6 +58,11 @@ def basin_radiation_golden():
 
 
 @pytest.fixture(scope="session")
+def basin_river52_golden():
+    return load_golden("basin_river52")
+
+
+@pytest.fixture(scope="session")
 def beach_hybrid_golden():
     return load_golden("beach_hybrid")
 

@@ -9,7 +9,7 @@ everywhere else.
 
     python tests/golden/make_golden.py [basin] [beach] [beach_lumped] [basin_linear] [jcg] [tau0]
                                        [coastal20k] [basin_tide] [basin_river] [beach_hybrid]
-                                       [basin_radiation]
+                                       [basin_radiation] [basin_river52]
 """
 from __future__ import annotations
 
@@ -235,6 +235,10 @@ def main(argv):
         # the basin with 30 km of the north edge radiating (IBTYPE 30); one
         # simulated hour
         print(run_case("basin_radiation", meshgen.basin_radiation(), nsteps=1800, early=10,
+                       dump_every=600))
+    if "basin_river52" in todo:
+        # the same rivers with the west one as river radiation (IBTYPE 52)
+        print(run_case("basin_river52", meshgen.basin_river(west_ibtype=52), nsteps=1800, early=10,
                        dump_every=600))
     if "basin_river" in todo:
         # the basin with two rivers (IBTYPE 2 and 22), steady and M2 flux;

@@ -106,3 +106,29 @@ def test_radiation_bitexact_and_resident(basin_radiation_golden):
     r.run_resident(1, 600)
     for k, v in e600.items():
         assert np.array_equal(r.get(k), v), k
+
+
+def test_river_radiation_bitexact(basin_river52_golden):
+    """The rivers with the west one as river radiation (IBTYPE 52): 1,800
+    per-step calls bit-exact against the oracle, within 1e-12 of the
+    reference at its snapshots; a resident run of 600 steps equal to them."""
+    d = basin_river52_golden
+    run = meshgen.basin_river(west_ibtype=52).run
+    log("case basin_river52: 1800 steps bit-exact vs oracle; resident 600")
+    g, o = build_pair(d, run)
+    snaps = set(int(s) for s in d["snap_steps"])
+    for it in range(1, 1801):
+        step_pair(g, o, it)
+        if it <= 3 or it % 50 == 0:
+            bad = compare(g, o, it)
+            assert not bad, f"step {it}: " + "; ".join(bad)
+        if it in snaps:
+            for k in ("eta2", "uu2", "vv2"):
+                dk = np.abs(g.get(k) - d[f"s{it}_{k}"]).max()
+                assert dk <= 1e-12, f"step {it}: {k} differs from the reference by {dk:.3e}"
+        if it == 600:
+            e600 = {k: g.get(k) for k in ("eta2", "uu2", "vv2", "etas")}
+    r, _ = build_pair(d, run)
+    r.run_resident(1, 600)
+    for k, v in e600.items():
+        assert np.array_equal(r.get(k), v), k

@@ -320,6 +320,17 @@ def test_basin_radiation_matches_reference(basin_radiation_golden):
     assert np.abs(q[np.asarray(d["lbcodei"]) == 30]).max() > 0.0
 
 
+def test_river_radiation_matches_reference(basin_river52_golden):
+    """The rivers with the west one as river radiation (IBTYPE 52: the
+    specified flux plus Celerity*(ETAS/DT + Tau0*(ETA1 - ElevDisc)) with no
+    flux-settling time, ElevDisc the cold-start elevation; src/gwce.F:
+    1767-1775, src/timestep.F:906-925, src/cstart.F:1188-1192), one hour:
+    bit-identical every step."""
+    d = basin_river52_golden
+    assert 52 in set(d["lbcodei"].tolist())
+    _run(d, meshgen.basin_river(west_ibtype=52).run, int(d["run_nsteps"]))
+
+
 def test_river_tables_match_reference(basin_river_golden):
     """The product's flow-boundary tables for a mesh with flux segments
     (LBCODEI 2, 22, the ME2GW precedence of flux codes at shared end nodes,
