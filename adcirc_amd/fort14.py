@@ -64,7 +64,8 @@ def write_fort14(path: str, m: Mesh) -> None:
 
 def read_fort14(path: str) -> Mesh:
     """Read the subset of fort.14 this package supports (IBTYPE 0/10/20 land,
-    2/12/22 specified-flux and 30 outward-radiating boundaries,
+    2/12/22 specified-flux, 52 river-radiation and 30 outward-radiating
+    boundaries,
     elevation-specified open boundaries)."""
     with open(path) as f:
         tokens_lines = f.read().splitlines()

@@ -16,7 +16,7 @@ line: IM other than 0, 111111 (the same defaults spelled out) or 111112
 tidal-potential ramp other than the elevation ramp; met forcing other than NWS = 0 or the
 parametric vortex NWS = 8; TAU0 schemes other than a constant, -1 and -2;
 the 3D-coupled friction of NOLIBF = 3; NRS/ice variants of NWS; normal flux read from
-fort.20 (NFFR 0 or -1; the periodic flux of NFFR > 0 on IBTYPE 2, 12, 22
+fort.20 (NFFR 0 or -1; the periodic flux of NFFR > 0 on IBTYPE 2, 12, 22, 52
 boundaries is read); a flux-forcing ramp other than the elevation ramp;
 station lists read from a separate file (NSTA < 0).
 """
